@@ -1,0 +1,226 @@
+"""Benchmark: 3D Ising 256^3 checkerboard Metropolis on one MI355X.
+
+Workload (one "step" = one full checkerboard sweep, both colour passes):
+  3D Ising on a 256^3 periodic cubic lattice (16,777,216 int8 spins),
+  beta = 0.2216546, J = 1, h = 0, cold start (all +1), energy and
+  magnetization reduced on the device every 10 sweeps.  Metropolis is the
+  headline; the heat bath is timed separately on the same workload.
+
+Reported:
+  value        whole-job site-updates/ns over the timed sweeps (wall clock,
+               lattice resident in HBM, device synchronised on both sides);
+  roofline     the colour-pass kernel: algorithmic bytes per launch (2 B per
+               site-update: int8 read + write; N/2 site-updates per launch)
+               over its average launch duration, timed with HIP events on the
+               chain's stream, against the 8 TB/s HBM peak;
+  cpu_baseline the reference CPU path as written (C restatement of
+               LocalUpdate{Metropolis, Checkerboard}: usize neighbour table, u8
+               parity table, full scan per colour, f64 exp, PCG64 draws), one
+               core, on a bounded sample of the same lattice.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "site-updates/ns (% HBM roofline); tau_int-corrected samples/s vs plain"
+HBM_PEAK_GBS = 8000.0
+SHAPE = [256, 256, 256]
+BETA = 0.2216546
+SEED = 20260728
+
+
+class HipEvents:
+    """hipEvent timing on a given stream through libamdhip64."""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def event(self):
+        e = C.c_void_p()
+        rc = self.hip.hipEventCreate(C.byref(e))
+        if rc != 0:
+            raise RuntimeError(f"hipEventCreate failed ({rc})")
+        return e
+
+    def record(self, e, stream):
+        rc = self.hip.hipEventRecord(e, C.c_void_p(stream))
+        if rc != 0:
+            raise RuntimeError(f"hipEventRecord failed ({rc})")
+
+    def elapsed_ms(self, a, b):
+        self.hip.hipEventSynchronize(b)
+        ms = C.c_float()
+        self.hip.hipEventElapsedTime(C.byref(ms), a, b)
+        return ms.value
+
+
+def run_kernel(P, ev, kernel, steps, warmup, kernel_sweeps):
+    import numpy as np
+
+    n = int(np.prod(SHAPE))
+    chain = P.ising_gpu_chain(SHAPE, 1.0, 0.0, BETA, SEED, np.zeros(n, np.uint8), kernel=kernel)
+    chain.advance(warmup)
+    chain.sync()
+    # whole job: K sweeps with E, M reduced every 10 sweeps
+    chain.record(10, steps // 10 + 1)
+    stream = chain.stream()
+    e0, e1 = ev.event(), ev.event()
+    chain.sync()
+    t0 = time.perf_counter()
+    ev.record(e0, stream)
+    chain.advance(steps)
+    ev.record(e1, stream)
+    chain.sync()
+    wall = time.perf_counter() - t0
+    gpu_ms = ev.elapsed_ms(e0, e1)
+    series = chain.series()
+    # kernel-only region: colour passes alone (no measurement launches)
+    chain.record(0, 0)
+    chain.sync()
+    k0, k1 = ev.event(), ev.event()
+    ev.record(k0, stream)
+    chain.advance(kernel_sweeps)
+    ev.record(k1, stream)
+    chain.sync()
+    pass_ms = ev.elapsed_ms(k0, k1) / (2 * kernel_sweeps)
+    chain.close()
+    return {
+        "wall_s": wall,
+        "gpu_ms": gpu_ms,
+        "site_updates_per_ns": steps * n / (wall * 1e9),
+        "site_updates_per_ns_gpu_clock": steps * n / (gpu_ms * 1e6),
+        "pass_ms": pass_ms,
+        "last_energy_per_site": series[-1].energy / n if series else None,
+        "last_abs_m_per_site": abs(series[-1].spin_sum) / n if series else None,
+        "records": len(series),
+    }
+
+
+def cpu_baseline(seconds_target: float):
+    """The reference CPU checkerboard path on one core, bounded sample."""
+    import numpy as np
+
+    import oracle as O
+
+    n = int(np.prod(SHAPE))
+    st = np.zeros(n, np.uint8)
+    st, secs = O.pmode_ising_sweeps(SHAPE, O.KERNEL_METROPOLIS, BETA, 1.0, 0.0, SEED, st, 1)
+    sweeps = max(1, int(seconds_target / max(secs, 1e-3)))
+    st, secs2 = O.pmode_ising_sweeps(SHAPE, O.KERNEL_METROPOLIS, BETA, 1.0, 0.0, SEED, st, sweeps)
+    total_sweeps = 1 + sweeps
+    total_secs = secs + secs2
+    return {
+        "value": total_sweeps * n / (total_secs * 1e9),
+        "unit": "site-updates/ns",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"256^3 Ising cold start, beta={BETA}, {total_sweeps} checkerboard Metropolis sweeps "
+                  f"({total_secs:.1f} s on one host core; C restatement of updater.rs:77-96,316-324 "
+                  f"with the PCG64 stream, table build excluded)",
+    }
+
+
+def read_traffic():
+    """HBM bytes per colour-pass launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    path = os.path.join(REPO, "profiles", "ising3d_256_pass_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--kernel-sweeps", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on these single-GPU boxes "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import plaquette_amd as P
+
+    if not P.gpu_available():
+        print("no HIP device", file=sys.stderr)
+        sys.exit(1)
+    ev = HipEvents()
+    n = int(np.prod(SHAPE))
+    metro = run_kernel(P, ev, P.Kernel.METROPOLIS, args.steps, args.warmup, args.kernel_sweeps)
+    hb = run_kernel(P, ev, P.Kernel.HEAT_BATH, args.steps, args.warmup, args.kernel_sweeps)
+
+    bytes_per_launch = 2.0 * (n // 2)  # int8 read + write per site-update, N/2 updates per pass
+    achieved = bytes_per_launch / (metro["pass_ms"] * 1e-3) / 1e9
+    traffic = read_traffic()
+    out = {
+        "metric": METRIC,
+        "value": round(metro["site_updates_per_ns"], 2),
+        "unit": "site-updates/ns",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(metro["wall_s"] * 1e3 / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {
+            "workload": "3D Ising 256^3 periodic, checkerboard Metropolis, beta=0.2216546, cold start, "
+                        "E and M reduced on device every 10 sweeps",
+            "lattice": SHAPE,
+            "sites": n,
+            "beta": BETA,
+            "seed": SEED,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": "ising_pass_swar<3,0> (one colour pass)",
+            "avg_launch_us": round(metro["pass_ms"] * 1e3, 3),
+            "site_updates_per_ns_kernel": round((n // 2) / (metro["pass_ms"] * 1e6), 1),
+        },
+        "metropolis": {k: (round(v, 6) if isinstance(v, float) else v) for k, v in metro.items()},
+        "heat_bath": {k: (round(v, 6) if isinstance(v, float) else v) for k, v in hb.items()},
+        "heat_bath_roofline_frac": round(bytes_per_launch / (hb["pass_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,166 @@
+/*
+ * plaquette_hip.h -- C ABI of the MI355X (gfx950) device path of plaquette.
+ *
+ * This is the drop-in boundary: the extern "C" surface a Rust `extern "C"`
+ * block in the reference's src/device.rs and src/gpu_cluster.rs binds in place
+ * of the wgpu device.  Plain pointers and sizes only; no HIP or torch types.
+ *
+ * Conventions shared by every entry point:
+ *   - Return value: PLQ_OK (0) or a negative PLQ_E* code; plq_last_error()
+ *     then holds a message (thread-local).  These codes replace the
+ *     reference's panics (device.rs:618-646, 114-127; gpu_cluster.rs:267-283,
+ *     512-518) and its `Option` for a missing adapter (device.rs:42-77).
+ *   - States cross the ABI as uint8 state indices in lattice-index order:
+ *     sites mixed-radix with axis 0 fastest (lattice.rs:75-78), links
+ *     link = site * D + dir (lattice.rs:259-263).  The reference uses one u32
+ *     word per variable (device.rs:721-732); the device layout (colour-
+ *     interleaved int8) stays private to the library.
+ *   - One handle is single-owner; calls on it serialise (the reference's
+ *     `&mut self`).  Different handles may be driven from different threads.
+ *   - `advance` only enqueues work on the handle's stream; `next`, `read`,
+ *     `measure`, `series` and `sync` wait for it (device.rs:383-386, 462-470).
+ *   - Randomness is the reference's keyed device stream (rng.wgsl:34-52) keyed
+ *     by fold_seed(seed) (device.rs:761-763), the variable index and the
+ *     chain's global sweep counter, so results are independent of the launch
+ *     shape.  Acceptance tests are exact integer comparisons against
+ *     thresholds the host derives from the reference's f64 CPU arithmetic
+ *     (PLQ_PROFILE_F64, updater.rs:77-167) or the WGSL f32 arithmetic
+ *     (PLQ_PROFILE_F32, metropolis.wgsl / heat_bath.wgsl, with
+ *     exp_f32(x) = (float)exp((double)x)).
+ */
+#ifndef PLAQUETTE_HIP_H
+#define PLAQUETTE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PLQ_OK 0
+#define PLQ_EINVAL (-22)  /* bad descriptor: odd extent, q out of range, size mismatch */
+#define PLQ_ENODEV (-19)  /* no HIP device (reference: Gpu::new() -> None) */
+#define PLQ_ENOMEM (-12)  /* device allocation failed / lattice too large */
+#define PLQ_ENOCONV (-62) /* cluster labelling exceeded its pass cap */
+#define PLQ_EDEVICE (-5)  /* a HIP runtime call or kernel launch failed */
+
+#define PLQ_MAX_DIM 8
+#define PLQ_MAX_Q 16
+
+enum { PLQ_MODEL_ISING = 0, PLQ_MODEL_POTTS = 1, PLQ_MODEL_Z2 = 2 };
+enum { PLQ_KERNEL_METROPOLIS = 0, PLQ_KERNEL_HEAT_BATH = 1 };
+enum { PLQ_PROFILE_F64 = 0, PLQ_PROFILE_F32 = 1 };
+enum { PLQ_EXTENT_ALL = 0, PLQ_EXTENT_SEEDED = 1 };
+enum { PLQ_RELABEL_REDRAW = 0, PLQ_RELABEL_FORCED_CHANGE = 1 };
+
+/* A local-update chain: GpuChain::build + the model's gpu_chain constructor
+ * (device.rs:609-681; models/ising/gpu.rs:57-95, potts/gpu.rs, gauge/gpu.rs:54-110).
+ * Ising and Potts live on sites, Z2 on links (cell follows from the model). */
+typedef struct {
+    int32_t dim;                   /* 1..4 (Z2: 2..4) */
+    int64_t shape[PLQ_MAX_DIM];    /* every extent even (device.rs:714-719) */
+    int32_t model;                 /* PLQ_MODEL_* */
+    int32_t q;                     /* 2 for Ising and Z2; 2..8 for Potts */
+    int32_t kernel;                /* PLQ_KERNEL_* */
+    int32_t profile;               /* PLQ_PROFILE_* */
+    double beta;
+    double j;                      /* coupling J */
+    double h;                      /* Ising field */
+    double offsets[PLQ_MAX_Q];     /* Potts per-label offsets h[a] */
+    uint64_t seed;                 /* folded with fold_seed on the host */
+    uint64_t sweeps_between;       /* Chain::next advances this many sweeps */
+} plq_local_desc;
+
+/* A cluster chain: GpuClusterChain::new (gpu_cluster.rs:256-472) with the
+ * model's bond gap (BondAction::bond_energy_gap: J for Potts, 2J for Ising). */
+typedef struct {
+    int32_t dim;                   /* 1..4 */
+    int64_t shape[PLQ_MAX_DIM];    /* any positive extents */
+    int32_t q;                     /* 2..16 */
+    int32_t extent;                /* PLQ_EXTENT_ALL (Swendsen-Wang) / _SEEDED (Wolff) */
+    int32_t relabel;               /* PLQ_RELABEL_* */
+    int32_t profile;               /* bond test: f64 p (CPU) or f32 p (WGSL) */
+    double beta;
+    double gap;                    /* >= 0 and finite (updater.rs:447-478) */
+    uint64_t seed;
+    uint64_t sweeps_between;
+} plq_cluster_desc;
+
+/* One measurement: exact integer sums, reduced on the device, and the model
+ * energy formed from them in f64 (ising/mod.rs:97-117,
+ * potts/mod.rs:237-271, gauge/mod.rs:275-286). */
+typedef struct {
+    uint64_t sweep;                /* sweeps done when measured */
+    int64_t n_vars;
+    int64_t bond_sum;              /* Ising: sum over bonds of s_i s_j; Potts: agreeing bonds;
+                                      Z2: signed plaquette sum */
+    int64_t spin_sum;              /* Ising: magnetization M = sum s_i; otherwise 0 */
+    int64_t counts[PLQ_MAX_Q];     /* label populations */
+    double energy;                 /* H of the configuration */
+} plq_obs;
+
+/* Statistics of the last cluster sweep. */
+typedef struct {
+    uint64_t sweep;
+    int64_t n_clusters;
+    int64_t largest;               /* size of the largest cluster */
+    int64_t moved;                 /* sites relabelled (Wolff: the seed's cluster size) */
+    int64_t size_hist[40];         /* clusters with size in [2^k, 2^(k+1)) */
+} plq_cluster_stats;
+
+typedef struct plq_chain plq_chain;
+
+/* Gpu::new (device.rs:42-77): number of usable HIP devices (0 -> PLQ_ENODEV). */
+int plq_device_count(int32_t *count);
+const char *plq_last_error(void);
+
+/* GpuChain::build (device.rs:609-681).  `start` holds n state indices < q. */
+int plq_chain_create_local(const plq_local_desc *desc, const uint8_t *start, size_t n,
+                           plq_chain **out);
+/* GpuClusterChain::new (gpu_cluster.rs:256-472). */
+int plq_chain_create_cluster(const plq_cluster_desc *desc, const uint8_t *start, size_t n,
+                             plq_chain **out);
+/* GpuChain::advance / GpuClusterChain::advance (device.rs:387-399,
+ * gpu_cluster.rs:477-526): enqueue `sweeps` sweeps, return without waiting.
+ * Cluster chains wait only if a labelling needed more passes than its cap. */
+int plq_chain_advance(plq_chain *chain, uint64_t sweeps);
+/* Iterator::next (device.rs:692-701, gpu_cluster.rs:630-637): advance
+ * sweeps_between sweeps and copy the configuration out. */
+int plq_chain_next(plq_chain *chain, uint8_t *out, size_t n);
+int plq_chain_read(plq_chain *chain, uint8_t *out, size_t n);
+int plq_chain_write(plq_chain *chain, const uint8_t *in, size_t n);
+/* On-device measurement of the current configuration (replaces the host
+ * readback + measure of device.rs:435-494 and ising/mod.rs:177-186). */
+int plq_chain_measure(plq_chain *chain, plq_obs *out);
+/* Record a measurement every `every` sweeps during advance, fused into the
+ * sweep kernels where the layout allows; up to `capacity` records are kept
+ * (older ones are dropped first).  every = 0 turns recording off. */
+int plq_chain_record(plq_chain *chain, uint64_t every, uint64_t capacity);
+int plq_chain_series(plq_chain *chain, plq_obs *out, uint64_t capacity, uint64_t *n_out);
+int plq_chain_sweeps_done(plq_chain *chain, uint64_t *out);
+/* Resume: with keyed draws a checkpoint is (states, sweep counter, seed). */
+int plq_chain_set_sweep(plq_chain *chain, uint64_t sweep);
+int plq_chain_sync(plq_chain *chain);
+/* The hipStream_t the chain's kernels run on (for event timing). */
+int plq_chain_stream(plq_chain *chain, void **stream);
+/* Cluster chains only: the canonical partition of the last sweep (each site's
+ * cluster root, the cluster's minimum site index) and its statistics. */
+int plq_chain_cluster_roots(plq_chain *chain, int64_t *roots, size_t n);
+int plq_chain_cluster_stats(plq_chain *chain, plq_cluster_stats *out);
+/* Cluster chains only: lower the labelling pass cap (gpu_cluster.rs:620-625);
+ * used to force PLQ_ENOCONV from the host. */
+int plq_chain_set_pass_cap(plq_chain *chain, uint64_t cap);
+int plq_chain_destroy(plq_chain *chain);
+
+/* Host-side helpers shared with the device path, exposed for tests. */
+uint32_t plq_fold_seed(uint64_t seed);
+/* The integer acceptance table the local kernels compare the 24-bit draw
+ * against: Ising/Z2 classes are (current state, number of "down" (state 1)
+ * neighbours or negative staple products).  Returns the number of entries. */
+int plq_threshold_table(const plq_local_desc *desc, uint32_t *out, size_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

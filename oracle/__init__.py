@@ -1,0 +1,321 @@
+"""ctypes front of the CPU oracle (``oracle/libplq_oracle.so``).
+
+TEST INFRASTRUCTURE ONLY.  This package is the checker: only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg import it.
+The product (``plaquette_amd``) never imports it.
+
+Each wrapper names the C function it calls; the C source cites the reference
+lines (``src/updater.rs``, ``src/wgsl/*.wgsl``, ``src/lattice.rs``,
+``src/cluster.rs``, ``src/statistics.rs``, ``src/models/*``) it restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libplq_oracle.so")
+
+MODEL_ISING, MODEL_POTTS, MODEL_Z2 = 0, 1, 2
+KERNEL_METROPOLIS, KERNEL_HEAT_BATH = 0, 1
+PROFILE_F64, PROFILE_F32 = 0, 1
+MAX_DIM, MAX_Q = 8, 16
+
+
+def build() -> None:
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, "libplq_oracle.so"], check=True)
+
+
+class LocalDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("shape", C.c_int64 * MAX_DIM),
+        ("model", C.c_int32),
+        ("q", C.c_int32),
+        ("kernel", C.c_int32),
+        ("profile", C.c_int32),
+        ("beta", C.c_double),
+        ("j", C.c_double),
+        ("h", C.c_double),
+        ("offsets", C.c_double * MAX_Q),
+        ("seed32", C.c_uint32),
+    ]
+
+
+class ClusterDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("shape", C.c_int64 * MAX_DIM),
+        ("q", C.c_int32),
+        ("seeded", C.c_int32),
+        ("forced", C.c_int32),
+        ("profile", C.c_int32),
+        ("beta", C.c_double),
+        ("gap", C.c_double),
+        ("seed32", C.c_uint32),
+    ]
+
+
+class Estimate(C.Structure):
+    _fields_ = [("mean", C.c_double), ("stderr", C.c_double), ("tau_int", C.c_double), ("n_eff", C.c_double)]
+
+
+class Derived(C.Structure):
+    _fields_ = [("value", C.c_double), ("stderr", C.c_double), ("n_blocks", C.c_int64)]
+
+
+class Pcg64(C.Structure):
+    _fields_ = [("state_lo", C.c_uint64), ("state_hi", C.c_uint64), ("inc_lo", C.c_uint64), ("inc_hi", C.c_uint64)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        P = C.POINTER
+        sig = {
+            "orc_lowbias32": (C.c_uint32, [C.c_uint32]),
+            "orc_keyed_bits": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),
+            "orc_keyed_uniform": (C.c_float, [C.c_uint32, C.c_uint32, C.c_uint32]),
+            "orc_fold_seed": (C.c_uint32, [C.c_uint64]),
+            "orc_n_sites": (C.c_int64, [C.c_int32, P(C.c_int64)]),
+            "orc_neighbor_table": (None, [C.c_int32, P(C.c_int64), P(C.c_int64)]),
+            "orc_parity_table": (None, [C.c_int32, P(C.c_int64), P(C.c_uint8)]),
+            "orc_plaquette_links": (None, [C.c_int32, P(C.c_int64), C.c_int64, P(C.c_int64)]),
+            "orc_staple_table": (None, [C.c_int32, P(C.c_int64), P(C.c_int64)]),
+            "orc_local_sweeps": (C.c_int, [P(LocalDesc), P(C.c_uint8), C.c_uint32, C.c_uint32]),
+            "orc_ising_metropolis_accept": (C.c_int, [P(LocalDesc), C.c_int32, C.c_int64, C.c_uint32]),
+            "orc_ising_heat_bath_choice": (C.c_int, [P(LocalDesc), C.c_int32, C.c_int64, C.c_uint32]),
+            "orc_cluster_sweeps": (C.c_int, [P(ClusterDesc), P(C.c_uint8), C.c_uint32, C.c_uint32, P(C.c_int64)]),
+            "orc_cluster_bonds": (C.c_int, [P(ClusterDesc), P(C.c_uint8), C.c_uint32, P(C.c_uint8)]),
+            "orc_components": (C.c_int, [C.c_int32, P(C.c_int64), P(C.c_uint8), P(C.c_int64)]),
+            "orc_ising_sums": (None, [C.c_int32, P(C.c_int64), P(C.c_uint8), P(C.c_int64), P(C.c_int64)]),
+            "orc_potts_sums": (None, [C.c_int32, P(C.c_int64), C.c_int32, P(C.c_uint8), P(C.c_int64), P(C.c_int64)]),
+            "orc_plaquette_sum": (C.c_int64, [C.c_int32, P(C.c_int64), P(C.c_uint8)]),
+            "orc_reduce": (Estimate, [P(C.c_double), C.c_int64]),
+            "orc_specific_heat": (Derived, [P(C.c_double), C.c_int64, C.c_double, C.c_double]),
+            "orc_susceptibility": (Derived, [P(C.c_double), C.c_int64, C.c_double, C.c_double]),
+            "orc_binder": (Derived, [P(C.c_double), C.c_int64]),
+            "orc_pcg64_seed_from_u64": (None, [P(Pcg64), C.c_uint64]),
+            "orc_pcg64_next_u64": (C.c_uint64, [P(Pcg64)]),
+            "orc_pcg64_next_f64": (C.c_double, [P(Pcg64)]),
+            "orc_pmode_ising_sweeps": (C.c_double, [C.c_int32, P(C.c_int64), C.c_int32, C.c_double, C.c_double,
+                                                    C.c_double, C.c_uint64, P(C.c_uint8), C.c_uint32]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _shape(shape):
+    arr = (C.c_int64 * MAX_DIM)()
+    for i, l in enumerate(shape):
+        arr[i] = int(l)
+    return arr
+
+
+def _u8(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+def _i64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def _f64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# ------------------------------------------------------------------ RNG ---
+
+def lowbias32(x: int) -> int:
+    return lib().orc_lowbias32(x & 0xFFFFFFFF)
+
+
+def keyed_bits(seed32: int, var: int, sweep: int) -> int:
+    return lib().orc_keyed_bits(seed32 & 0xFFFFFFFF, var & 0xFFFFFFFF, sweep & 0xFFFFFFFF)
+
+
+def keyed_uniform(seed32: int, var: int, sweep: int) -> float:
+    return lib().orc_keyed_uniform(seed32 & 0xFFFFFFFF, var & 0xFFFFFFFF, sweep & 0xFFFFFFFF)
+
+
+def fold_seed(seed: int) -> int:
+    return lib().orc_fold_seed(seed & 0xFFFFFFFFFFFFFFFF)
+
+
+# ------------------------------------------------------------- geometry ---
+
+def n_sites(shape) -> int:
+    return int(np.prod(shape))
+
+
+def neighbor_table(shape) -> np.ndarray:
+    d = len(shape)
+    out = np.zeros((n_sites(shape), 2 * d), dtype=np.int64)
+    lib().orc_neighbor_table(d, _shape(shape), _i64(out))
+    return out
+
+
+def parity_table(shape) -> np.ndarray:
+    out = np.zeros(n_sites(shape), dtype=np.uint8)
+    lib().orc_parity_table(len(shape), _shape(shape), _u8(out))
+    return out
+
+
+def plaquette_links(shape, p: int) -> list:
+    out = np.zeros(4, dtype=np.int64)
+    lib().orc_plaquette_links(len(shape), _shape(shape), p, _i64(out))
+    return [int(x) for x in out]
+
+
+def staple_table(shape) -> np.ndarray:
+    d = len(shape)
+    out = np.zeros((n_sites(shape) * d, 6 * (d - 1)), dtype=np.int64)
+    lib().orc_staple_table(d, _shape(shape), _i64(out))
+    return out
+
+
+# --------------------------------------------------------------- sweeps ---
+
+def local_desc(shape, model=MODEL_ISING, q=2, kernel=KERNEL_METROPOLIS, profile=PROFILE_F64,
+               beta=0.44, j=1.0, h=0.0, offsets=None, seed=0) -> LocalDesc:
+    d = LocalDesc()
+    d.dim = len(shape)
+    d.shape = _shape(shape)
+    d.model = model
+    d.q = q
+    d.kernel = kernel
+    d.profile = profile
+    d.beta = beta
+    d.j = j
+    d.h = h
+    if offsets is not None:
+        for i, o in enumerate(offsets):
+            d.offsets[i] = float(o)
+    d.seed32 = fold_seed(seed)
+    return d
+
+
+def local_sweeps(desc: LocalDesc, state: np.ndarray, sweep0: int, n_sweeps: int) -> np.ndarray:
+    st = np.ascontiguousarray(state, dtype=np.uint8).copy()
+    rc = lib().orc_local_sweeps(C.byref(desc), _u8(st), sweep0, n_sweeps)
+    if rc != 0:
+        raise ValueError("orc_local_sweeps rejected the description")
+    return st
+
+
+def ising_metropolis_accept(desc: LocalDesc, cur: int, nsum: int, m: int) -> int:
+    return lib().orc_ising_metropolis_accept(C.byref(desc), cur, nsum, m)
+
+
+def ising_heat_bath_choice(desc: LocalDesc, cur: int, nsum: int, m: int) -> int:
+    return lib().orc_ising_heat_bath_choice(C.byref(desc), cur, nsum, m)
+
+
+def cluster_desc(shape, q=3, seeded=False, forced=False, profile=PROFILE_F64, beta=1.0, gap=1.0,
+                 seed=0) -> ClusterDesc:
+    d = ClusterDesc()
+    d.dim = len(shape)
+    d.shape = _shape(shape)
+    d.q = q
+    d.seeded = int(seeded)
+    d.forced = int(forced)
+    d.profile = profile
+    d.beta = beta
+    d.gap = gap
+    d.seed32 = fold_seed(seed)
+    return d
+
+
+def cluster_sweeps(desc: ClusterDesc, state: np.ndarray, sweep0: int, n_sweeps: int):
+    st = np.ascontiguousarray(state, dtype=np.uint8).copy()
+    roots = np.zeros(st.size, dtype=np.int64)
+    rc = lib().orc_cluster_sweeps(C.byref(desc), _u8(st), sweep0, n_sweeps, _i64(roots))
+    if rc != 0:
+        raise ValueError("orc_cluster_sweeps rejected the description")
+    return st, roots
+
+
+def cluster_bonds(desc: ClusterDesc, state: np.ndarray, sweep: int) -> np.ndarray:
+    st = np.ascontiguousarray(state, dtype=np.uint8)
+    bonds = np.zeros(st.size * desc.dim, dtype=np.uint8)
+    lib().orc_cluster_bonds(C.byref(desc), _u8(st), sweep, _u8(bonds))
+    return bonds
+
+
+def components(shape, bonds: np.ndarray) -> np.ndarray:
+    b = np.ascontiguousarray(bonds, dtype=np.uint8)
+    roots = np.zeros(n_sites(shape), dtype=np.int64)
+    lib().orc_components(len(shape), _shape(shape), _u8(b), _i64(roots))
+    return roots
+
+
+# ---------------------------------------------------------- observables ---
+
+def ising_sums(shape, state):
+    st = np.ascontiguousarray(state, dtype=np.uint8)
+    b, s = C.c_int64(), C.c_int64()
+    lib().orc_ising_sums(len(shape), _shape(shape), _u8(st), C.byref(b), C.byref(s))
+    return b.value, s.value
+
+
+def potts_sums(shape, q, state):
+    st = np.ascontiguousarray(state, dtype=np.uint8)
+    a = C.c_int64()
+    counts = np.zeros(q, dtype=np.int64)
+    lib().orc_potts_sums(len(shape), _shape(shape), q, _u8(st), C.byref(a), _i64(counts))
+    return a.value, counts
+
+
+def plaquette_sum(shape, links) -> int:
+    st = np.ascontiguousarray(links, dtype=np.uint8)
+    return lib().orc_plaquette_sum(len(shape), _shape(shape), _u8(st))
+
+
+# ----------------------------------------------------------- statistics ---
+
+def reduce(series):
+    x = np.ascontiguousarray(series, dtype=np.float64)
+    return lib().orc_reduce(_f64(x), x.size)
+
+
+def specific_heat(e, beta, n):
+    x = np.ascontiguousarray(e, dtype=np.float64)
+    return lib().orc_specific_heat(_f64(x), x.size, beta, n)
+
+
+def susceptibility(m, beta, n):
+    x = np.ascontiguousarray(m, dtype=np.float64)
+    return lib().orc_susceptibility(_f64(x), x.size, beta, n)
+
+
+def binder(m):
+    x = np.ascontiguousarray(m, dtype=np.float64)
+    return lib().orc_binder(_f64(x), x.size)
+
+
+# ------------------------------------------------------------- P-mode ---
+
+def pcg64(seed: int) -> Pcg64:
+    g = Pcg64()
+    lib().orc_pcg64_seed_from_u64(C.byref(g), seed)
+    return g
+
+
+def pmode_ising_sweeps(shape, kernel, beta, j, h, seed, state, n_sweeps):
+    st = np.ascontiguousarray(state, dtype=np.uint8).copy()
+    secs = lib().orc_pmode_ising_sweeps(len(shape), _shape(shape), kernel, beta, j, h, seed, _u8(st), n_sweeps)
+    return st, secs

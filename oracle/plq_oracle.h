@@ -1,0 +1,148 @@
+/*
+ * plq_oracle.h -- CPU oracle for the plaquette data-parallel update path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product (plaquette_amd, libplaquette_hip.so) never links, loads or calls
+ * anything here, and must fail loudly when its HIP extension is missing.
+ *
+ * It restates, sequentially and with the reference's own data structures
+ * (neighbour table, parity table, staple table built in plaquette order), the
+ * semantics of swvasquez/plaquette's CPU checkerboard path:
+ *
+ *   - keyed RNG            src/wgsl/rng.wgsl:34-52, src/device.rs:761-763
+ *   - geometry             src/lattice.rs:55-163, 259-313, 420-441
+ *   - Metropolis step      src/updater.rs:77-96, 659-668; src/wgsl/metropolis.wgsl:32-63
+ *   - heat bath step       src/updater.rs:109-167; src/wgsl/heat_bath.wgsl:28-62
+ *   - checkerboard order   src/updater.rs:291-347 (site: colour 0 then 1;
+ *                          link: for dir, colour 0 then 1, link = site*D+dir)
+ *   - energy deltas        src/models/ising/mod.rs:119-143,
+ *                          src/models/potts/mod.rs:273-309,
+ *                          src/models/gauge/mod.rs:287-328  (f64 profile);
+ *                          src/models/{ising,potts,gauge}/model.wgsl (f32 profile)
+ *   - cluster sweep        src/wgsl/cluster_prelude.wgsl:61-147,
+ *                          src/wgsl/cluster_relabel.wgsl:39-90,
+ *                          src/updater.rs:447-613, src/cluster.rs:44-227
+ *   - observables          src/models/ising/mod.rs:97-186,
+ *                          src/models/potts/mod.rs:142-181, 237-365,
+ *                          src/models/gauge/mod.rs:99-122
+ *   - statistics (FP64)    src/statistics.rs:83-317
+ *   - PCG64 CPU stream     src/rng.rs:30-50 (rand_pcg 0.10.2 Lcg128Xsl64,
+ *                          rand_core 0.10.1 seed_from_u64) -- parity unpinned:
+ *                          no reference test fixes its output values.
+ *
+ * Exactness contract ("K-mode"): the reference checkerboard order, driven by
+ * the reference's keyed per-variable stream m = hash(seed32, var, sweep) >> 8,
+ * with the acceptance arithmetic of one of two profiles:
+ *   PLQ_PROFILE_F64: the CPU path's f64 arithmetic (u = m * 2^-24 as f64,
+ *                    glibc exp), i.e. updater.rs:88-91 and 141-163;
+ *   PLQ_PROFILE_F32: the WGSL kernels' f32 arithmetic with the canonical
+ *                    exp_f32(x) = (float)exp((double)x).
+ */
+#ifndef PLQ_ORACLE_H
+#define PLQ_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_MODEL_ISING = 0, ORC_MODEL_POTTS = 1, ORC_MODEL_Z2 = 2 };
+enum { ORC_KERNEL_METROPOLIS = 0, ORC_KERNEL_HEAT_BATH = 1 };
+enum { ORC_PROFILE_F64 = 0, ORC_PROFILE_F32 = 1 };
+enum { ORC_MAX_DIM = 8, ORC_MAX_Q = 16 };
+
+typedef struct {
+    int32_t dim;
+    int64_t shape[ORC_MAX_DIM];
+    int32_t model;      /* ORC_MODEL_* ; Z2 lives on links, the others on sites */
+    int32_t q;          /* 2 for Ising / Z2 */
+    int32_t kernel;     /* ORC_KERNEL_* */
+    int32_t profile;    /* ORC_PROFILE_* */
+    double beta;
+    double j;
+    double h;                 /* Ising field */
+    double offsets[ORC_MAX_Q];/* Potts per-label offsets h[a] */
+    uint32_t seed32;          /* fold_seed(seed) */
+} orc_local_desc;
+
+typedef struct {
+    int32_t dim;
+    int64_t shape[ORC_MAX_DIM];
+    int32_t q;
+    int32_t seeded;     /* 1 = Wolff extent (relabel only the seed's cluster) */
+    int32_t forced;     /* 1 = forced change, 0 = redraw */
+    int32_t profile;    /* bond test in f64 (CPU) or f32 (WGSL) */
+    double beta;
+    double gap;         /* bond energy gap: J (Potts), 2J (Ising) */
+    uint32_t seed32;
+} orc_cluster_desc;
+
+/* keyed RNG (rng.wgsl:34-52) */
+uint32_t orc_lowbias32(uint32_t x);
+uint32_t orc_keyed_bits(uint32_t seed, uint32_t var, uint32_t sweep); /* 24-bit m */
+float orc_keyed_uniform(uint32_t seed, uint32_t var, uint32_t sweep);
+uint32_t orc_fold_seed(uint64_t seed);
+
+/* geometry (lattice.rs) */
+int64_t orc_n_sites(int32_t dim, const int64_t *shape);
+/* 2*dim neighbour entries per site, order +0,-0,+1,-1,... */
+void orc_neighbor_table(int32_t dim, const int64_t *shape, int64_t *out);
+void orc_parity_table(int32_t dim, const int64_t *shape, uint8_t *out);
+/* number of plaquettes = n_sites * dim*(dim-1)/2; 4 links each */
+void orc_plaquette_links(int32_t dim, const int64_t *shape, int64_t plaquette, int64_t out[4]);
+/* staples: 6*(dim-1) link indices per link, groups in increasing plaquette order */
+void orc_staple_table(int32_t dim, const int64_t *shape, int64_t *out);
+
+/* K-mode checkerboard sweeps; state is lattice-ordered (site, or link=site*D+dir) */
+int orc_local_sweeps(const orc_local_desc *d, uint8_t *state, uint32_t sweep0, uint32_t n_sweeps);
+/* The acceptance table entries the device uses are checked against the literal
+ * per-site arithmetic through these: the Metropolis / heat bath decision of a
+ * single variable with a given 24-bit draw. */
+int orc_ising_metropolis_accept(const orc_local_desc *d, int32_t cur, int64_t nsum, uint32_t m);
+int orc_ising_heat_bath_choice(const orc_local_desc *d, int32_t cur, int64_t nsum, uint32_t m);
+
+/* K-mode cluster sweeps; roots_out (n_sites, may be NULL) receives the
+ * min-site-index root of each site's cluster in the LAST sweep, sizes of the
+ * seeded cluster etc. are derivable from it. */
+int orc_cluster_sweeps(const orc_cluster_desc *d, uint8_t *state, uint32_t sweep0,
+                       uint32_t n_sweeps, int64_t *roots_out);
+/* Only the bond draw of one sweep (bond = site*D+axis), 1 open / 0 closed. */
+int orc_cluster_bonds(const orc_cluster_desc *d, const uint8_t *state, uint32_t sweep, uint8_t *bonds);
+/* Connected components of the open bonds, canonical: root = min site index. */
+int orc_components(int32_t dim, const int64_t *shape, const uint8_t *bonds, int64_t *roots);
+
+/* observables (integer sums) */
+void orc_ising_sums(int32_t dim, const int64_t *shape, const uint8_t *state,
+                    int64_t *bond_sum, int64_t *spin_sum);
+void orc_potts_sums(int32_t dim, const int64_t *shape, int32_t q, const uint8_t *state,
+                    int64_t *agree, int64_t *counts);
+int64_t orc_plaquette_sum(int32_t dim, const int64_t *shape, const uint8_t *links);
+
+/* statistics (statistics.rs:83-317), FP64 */
+typedef struct { double mean, stderr_, tau_int, n_eff; } orc_estimate;
+typedef struct { double value, stderr_; int64_t n_blocks; } orc_derived;
+orc_estimate orc_reduce(const double *series, int64_t n);
+orc_derived orc_specific_heat(const double *e, int64_t n, double beta, double n_sites);
+orc_derived orc_susceptibility(const double *m, int64_t n, double beta, double n_sites);
+orc_derived orc_binder(const double *m, int64_t n);
+
+/* P-mode: the reference CPU path as written (PCG64 stream, f64, table
+ * geometry), used as the timed CPU baseline.  Parity unpinned. */
+typedef struct { uint64_t state_lo, state_hi, inc_lo, inc_hi; } orc_pcg64;
+void orc_pcg64_seed_from_u64(orc_pcg64 *g, uint64_t seed);
+uint64_t orc_pcg64_next_u64(orc_pcg64 *g);
+double orc_pcg64_next_f64(orc_pcg64 *g);
+/* Runs n_sweeps of LocalUpdate{Metropolis|HeatBath, Checkerboard} on a D-dim
+ * Ising site field with the PCG64 stream; returns wall seconds spent inside the
+ * sweeps (table construction excluded). */
+double orc_pmode_ising_sweeps(int32_t dim, const int64_t *shape, int32_t kernel,
+                              double beta, double j, double h, uint64_t seed,
+                              uint8_t *state, uint32_t n_sweeps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,397 @@
+"""plaquette_amd -- MI355X (gfx950) device path for plaquette's update seam.
+
+Python mirror of the reference's device chain interface over the C ABI in
+``include/plaquette_hip.h`` (``libplaquette_hip.so``):
+
+* :class:`GpuChain` -- ``GpuChain`` (src/device.rs:590-701) built by the
+  model constructors ``ising_gpu_chain`` (src/models/ising/gpu.rs:57-95),
+  ``potts_gpu_chain`` (src/models/potts/gpu.rs) and ``gauge_gpu_chain``
+  (src/models/gauge/gpu.rs:54-110);
+* :class:`GpuClusterChain` -- ``GpuClusterChain`` (src/gpu_cluster.rs:144-637)
+  with ``swendsen_wang`` / ``wolff`` constructors;
+* :func:`gpu_available` -- ``Gpu::new() -> Option<Gpu>`` (src/device.rs:42-77).
+
+Chains are iterators: ``next(chain)`` advances ``sweeps_between`` sweeps and
+returns the configuration as a ``numpy.uint8`` array of state indices in
+lattice order, like ``Iterator::next`` on the reference chains.  Errors the
+reference raises as panics surface as :class:`PlaquetteError`.
+
+The library is loaded from this directory and never replaced by a fallback:
+if ``libplaquette_hip.so`` is missing, importing the chain types raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libplaquette_hip.so")
+
+MAX_DIM = 8
+MAX_Q = 16
+
+PLQ_OK = 0
+PLQ_EINVAL = -22
+PLQ_ENODEV = -19
+PLQ_ENOMEM = -12
+PLQ_ENOCONV = -62
+PLQ_EDEVICE = -5
+
+_CODE_NAMES = {PLQ_EINVAL: "EINVAL", PLQ_ENODEV: "ENODEV", PLQ_ENOMEM: "ENOMEM", PLQ_ENOCONV: "ENOCONV",
+               PLQ_EDEVICE: "EDEVICE"}
+
+
+class Kernel(enum.IntEnum):
+    """updater.rs Kernel: the single-variable rule a local chain applies."""
+    METROPOLIS = 0
+    HEAT_BATH = 1
+
+
+class Model(enum.IntEnum):
+    ISING = 0
+    POTTS = 1
+    Z2 = 2
+
+
+class Profile(enum.IntEnum):
+    """Acceptance arithmetic the integer thresholds reproduce."""
+    F64 = 0   # the reference CPU path (updater.rs:77-167)
+    F32 = 1   # the reference WGSL kernels (metropolis.wgsl, heat_bath.wgsl)
+
+
+class Extent(enum.IntEnum):
+    """updater.rs Extent."""
+    ALL = 0
+    SEEDED = 1
+
+
+class Relabel(enum.IntEnum):
+    """updater.rs Relabel."""
+    REDRAW = 0
+    FORCED_CHANGE = 1
+
+
+class PlaquetteError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{_CODE_NAMES.get(code, code)}: {message}")
+        self.code = code
+        self.message = message
+
+
+class LocalDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("shape", C.c_int64 * MAX_DIM),
+        ("model", C.c_int32),
+        ("q", C.c_int32),
+        ("kernel", C.c_int32),
+        ("profile", C.c_int32),
+        ("beta", C.c_double),
+        ("j", C.c_double),
+        ("h", C.c_double),
+        ("offsets", C.c_double * MAX_Q),
+        ("seed", C.c_uint64),
+        ("sweeps_between", C.c_uint64),
+    ]
+
+
+class ClusterDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("shape", C.c_int64 * MAX_DIM),
+        ("q", C.c_int32),
+        ("extent", C.c_int32),
+        ("relabel", C.c_int32),
+        ("profile", C.c_int32),
+        ("beta", C.c_double),
+        ("gap", C.c_double),
+        ("seed", C.c_uint64),
+        ("sweeps_between", C.c_uint64),
+    ]
+
+
+class Obs(C.Structure):
+    _fields_ = [
+        ("sweep", C.c_uint64),
+        ("n_vars", C.c_int64),
+        ("bond_sum", C.c_int64),
+        ("spin_sum", C.c_int64),
+        ("counts", C.c_int64 * MAX_Q),
+        ("energy", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {"sweep": self.sweep, "n_vars": self.n_vars, "bond_sum": self.bond_sum, "spin_sum": self.spin_sum,
+                "counts": list(self.counts), "energy": self.energy}
+
+
+class ClusterStats(C.Structure):
+    _fields_ = [
+        ("sweep", C.c_uint64),
+        ("n_clusters", C.c_int64),
+        ("largest", C.c_int64),
+        ("moved", C.c_int64),
+        ("size_hist", C.c_int64 * 40),
+    ]
+
+
+# Every symbol the header declares, with its ctypes signature.
+_P = C.POINTER
+_HANDLE = C.c_void_p
+SIGNATURES = {
+    "plq_device_count": (C.c_int, [_P(C.c_int32)]),
+    "plq_last_error": (C.c_char_p, []),
+    "plq_chain_create_local": (C.c_int, [_P(LocalDesc), _P(C.c_uint8), C.c_size_t, _P(_HANDLE)]),
+    "plq_chain_create_cluster": (C.c_int, [_P(ClusterDesc), _P(C.c_uint8), C.c_size_t, _P(_HANDLE)]),
+    "plq_chain_advance": (C.c_int, [_HANDLE, C.c_uint64]),
+    "plq_chain_next": (C.c_int, [_HANDLE, _P(C.c_uint8), C.c_size_t]),
+    "plq_chain_read": (C.c_int, [_HANDLE, _P(C.c_uint8), C.c_size_t]),
+    "plq_chain_write": (C.c_int, [_HANDLE, _P(C.c_uint8), C.c_size_t]),
+    "plq_chain_measure": (C.c_int, [_HANDLE, _P(Obs)]),
+    "plq_chain_record": (C.c_int, [_HANDLE, C.c_uint64, C.c_uint64]),
+    "plq_chain_series": (C.c_int, [_HANDLE, _P(Obs), C.c_uint64, _P(C.c_uint64)]),
+    "plq_chain_sweeps_done": (C.c_int, [_HANDLE, _P(C.c_uint64)]),
+    "plq_chain_set_sweep": (C.c_int, [_HANDLE, C.c_uint64]),
+    "plq_chain_sync": (C.c_int, [_HANDLE]),
+    "plq_chain_stream": (C.c_int, [_HANDLE, _P(C.c_void_p)]),
+    "plq_chain_cluster_roots": (C.c_int, [_HANDLE, _P(C.c_int64), C.c_size_t]),
+    "plq_chain_cluster_stats": (C.c_int, [_HANDLE, _P(ClusterStats)]),
+    "plq_chain_set_pass_cap": (C.c_int, [_HANDLE, C.c_uint64]),
+    "plq_chain_destroy": (C.c_int, [_HANDLE]),
+    "plq_fold_seed": (C.c_uint32, [C.c_uint64]),
+    "plq_threshold_table": (C.c_int, [_P(LocalDesc), _P(C.c_uint32), C.c_size_t]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load ``libplaquette_hip.so``; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m plaquette_amd.build` "
+                               "(there is no non-HIP fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> int:
+    if rc < 0:
+        msg = lib().plq_last_error()
+        raise PlaquetteError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def gpu_available() -> bool:
+    """Gpu::new().is_some() (device.rs:42-77)."""
+    n = C.c_int32(0)
+    return lib().plq_device_count(C.byref(n)) == PLQ_OK and n.value > 0
+
+
+def fold_seed(seed: int) -> int:
+    return lib().plq_fold_seed(seed & 0xFFFFFFFFFFFFFFFF)
+
+
+def _shape_arr(shape):
+    arr = (C.c_int64 * MAX_DIM)()
+    for i, l in enumerate(shape):
+        arr[i] = int(l)
+    return arr
+
+
+def local_desc(shape, model=Model.ISING, q=2, kernel=Kernel.METROPOLIS, profile=Profile.F64, beta=0.44, j=1.0,
+               h=0.0, offsets=None, seed=0, sweeps_between=1) -> LocalDesc:
+    d = LocalDesc()
+    d.dim = len(shape)
+    d.shape = _shape_arr(shape)
+    d.model = int(model)
+    d.q = int(q)
+    d.kernel = int(kernel)
+    d.profile = int(profile)
+    d.beta = float(beta)
+    d.j = float(j)
+    d.h = float(h)
+    if offsets is not None:
+        for i, o in enumerate(offsets):
+            d.offsets[i] = float(o)
+    d.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    d.sweeps_between = int(sweeps_between)
+    return d
+
+
+def threshold_table(desc: LocalDesc) -> np.ndarray:
+    n = _check(lib().plq_threshold_table(C.byref(desc), None, 0))
+    out = np.zeros(n, dtype=np.uint32)
+    _check(lib().plq_threshold_table(C.byref(desc), out.ctypes.data_as(_P(C.c_uint32)), n))
+    return out
+
+
+class _Chain:
+    """Common handle management for the two chain kinds."""
+
+    def __init__(self, handle, n_vars: int, n_sites: int, sweeps_between: int):
+        self._h = handle
+        self.n_vars = n_vars
+        self.n_sites = n_sites
+        self.sweeps_between = sweeps_between
+
+    # -- lifecycle ----------------------------------------------------------
+    def close(self):
+        if self._h:
+            lib().plq_chain_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- the reference chain surface ----------------------------------------
+    def advance(self, sweeps: int) -> None:
+        """GpuChain::advance: enqueue sweeps without producing a snapshot."""
+        _check(lib().plq_chain_advance(self._h, int(sweeps)))
+
+    def __iter__(self):
+        return self
+
+    def __next__(self) -> np.ndarray:
+        out = np.empty(self.n_vars, dtype=np.uint8)
+        _check(lib().plq_chain_next(self._h, out.ctypes.data_as(_P(C.c_uint8)), out.size))
+        return out
+
+    def read(self) -> np.ndarray:
+        out = np.empty(self.n_vars, dtype=np.uint8)
+        _check(lib().plq_chain_read(self._h, out.ctypes.data_as(_P(C.c_uint8)), out.size))
+        return out
+
+    def write(self, states) -> None:
+        st = np.ascontiguousarray(states, dtype=np.uint8)
+        _check(lib().plq_chain_write(self._h, st.ctypes.data_as(_P(C.c_uint8)), st.size))
+
+    def measure(self) -> Obs:
+        o = Obs()
+        _check(lib().plq_chain_measure(self._h, C.byref(o)))
+        return o
+
+    def record(self, every: int, capacity: int) -> None:
+        _check(lib().plq_chain_record(self._h, int(every), int(capacity)))
+
+    def series(self) -> list:
+        n = C.c_uint64(0)
+        _check(lib().plq_chain_series(self._h, None, 0, C.byref(n)))
+        arr = (Obs * max(n.value, 1))()
+        _check(lib().plq_chain_series(self._h, arr, n.value, C.byref(n)))
+        return [arr[i] for i in range(n.value)]
+
+    @property
+    def sweeps_done(self) -> int:
+        v = C.c_uint64(0)
+        _check(lib().plq_chain_sweeps_done(self._h, C.byref(v)))
+        return v.value
+
+    def set_sweep(self, sweep: int) -> None:
+        _check(lib().plq_chain_set_sweep(self._h, int(sweep)))
+
+    def sync(self) -> None:
+        _check(lib().plq_chain_sync(self._h))
+
+    def stream(self) -> int:
+        p = C.c_void_p(0)
+        _check(lib().plq_chain_stream(self._h, C.byref(p)))
+        return p.value or 0
+
+
+class GpuChain(_Chain):
+    """GpuChain (device.rs:590-701): a checkerboard chain on one device."""
+
+    @classmethod
+    def build(cls, desc: LocalDesc, start) -> "GpuChain":
+        st = np.ascontiguousarray(start, dtype=np.uint8).ravel()
+        h = C.c_void_p(0)
+        _check(lib().plq_chain_create_local(C.byref(desc), st.ctypes.data_as(_P(C.c_uint8)), st.size, C.byref(h)))
+        n_sites = int(np.prod([desc.shape[i] for i in range(desc.dim)]))
+        return cls(h, st.size, n_sites, desc.sweeps_between)
+
+
+class GpuClusterChain(_Chain):
+    """GpuClusterChain (gpu_cluster.rs:144-637): SW / Wolff on one device."""
+
+    @classmethod
+    def new(cls, shape, q, gap, extent, relabel, beta, seed, start, sweeps_between=1,
+            profile=Profile.F64) -> "GpuClusterChain":
+        d = ClusterDesc()
+        d.dim = len(shape)
+        d.shape = _shape_arr(shape)
+        d.q = int(q)
+        d.extent = int(extent)
+        d.relabel = int(relabel)
+        d.profile = int(profile)
+        d.beta = float(beta)
+        d.gap = float(gap)
+        d.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        d.sweeps_between = int(sweeps_between)
+        st = np.ascontiguousarray(start, dtype=np.uint8).ravel()
+        h = C.c_void_p(0)
+        _check(lib().plq_chain_create_cluster(C.byref(d), st.ctypes.data_as(_P(C.c_uint8)), st.size, C.byref(h)))
+        return cls(h, st.size, st.size, int(sweeps_between))
+
+    @classmethod
+    def swendsen_wang(cls, shape, q, gap, beta, seed, start, sweeps_between=1, profile=Profile.F64):
+        return cls.new(shape, q, gap, Extent.ALL, Relabel.REDRAW, beta, seed, start, sweeps_between, profile)
+
+    @classmethod
+    def wolff(cls, shape, q, gap, beta, seed, start, sweeps_between=1, profile=Profile.F64):
+        return cls.new(shape, q, gap, Extent.SEEDED, Relabel.FORCED_CHANGE, beta, seed, start, sweeps_between,
+                       profile)
+
+    def roots(self) -> np.ndarray:
+        out = np.empty(self.n_sites, dtype=np.int64)
+        _check(lib().plq_chain_cluster_roots(self._h, out.ctypes.data_as(_P(C.c_int64)), out.size))
+        return out
+
+    def stats(self) -> ClusterStats:
+        s = ClusterStats()
+        _check(lib().plq_chain_cluster_stats(self._h, C.byref(s)))
+        return s
+
+    def set_pass_cap(self, cap: int) -> None:
+        _check(lib().plq_chain_set_pass_cap(self._h, int(cap)))
+
+
+# -- model constructors (the reference's per-model gpu_chain functions) -------
+
+def ising_gpu_chain(shape, j, h, beta, seed, start, sweeps_between=1, kernel=Kernel.METROPOLIS,
+                    profile=Profile.F64) -> GpuChain:
+    """models/ising/gpu.rs:57-95."""
+    d = local_desc(shape, Model.ISING, 2, kernel, profile, beta, j, h, None, seed, sweeps_between)
+    return GpuChain.build(d, start)
+
+
+def potts_gpu_chain(shape, q, j, offsets, beta, seed, start, sweeps_between=1, kernel=Kernel.METROPOLIS,
+                    profile=Profile.F64) -> GpuChain:
+    """models/potts/gpu.rs (per-label offsets h[a])."""
+    d = local_desc(shape, Model.POTTS, q, kernel, profile, beta, j, 0.0, offsets, seed, sweeps_between)
+    return GpuChain.build(d, start)
+
+
+def gauge_gpu_chain(shape, j, beta, seed, start, sweeps_between=1, kernel=Kernel.METROPOLIS,
+                    profile=Profile.F64) -> GpuChain:
+    """models/gauge/gpu.rs:54-110: links, link = site * D + dir."""
+    d = local_desc(shape, Model.Z2, 2, kernel, profile, beta, j, 0.0, None, seed, sweeps_between)
+    return GpuChain.build(d, start)

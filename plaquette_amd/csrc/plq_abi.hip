@@ -1,0 +1,739 @@
+// plq_abi.hip -- the extern "C" boundary (include/plaquette_hip.h): chain
+// objects, validation, the sweep loop and the host side of measurement.
+//
+// A chain owns one HIP stream and its device buffers.  Sweeps are enqueued
+// asynchronously; the entry points that hand data back synchronise.  The
+// sweep counter is the reference's global per-chain counter (device.rs:217,
+// 393-395; gpu_cluster.rs:173, 525): every colour pass of a sweep keys its
+// draws with the same value.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/plaquette_hip.h"
+#include "plq_common.hpp"
+#include "plq_internal.hpp"
+#include "plq_tables.hpp"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string &msg) {
+    g_error = msg;
+    return code;
+}
+
+#define PLQ_HIP(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(PLQ_EDEVICE, std::string(#expr " failed: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+enum Kind { KIND_SITE = 0, KIND_LINK = 1, KIND_CLUSTER = 2 };
+
+bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
+uint32_t ilog2(int64_t v) {
+    uint32_t r = 0;
+    while ((int64_t(1) << (r + 1)) <= v) ++r;
+    return r;
+}
+
+}  // namespace
+
+struct plq_chain {
+    std::mutex mu;
+    Kind kind;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int D = 0;
+    int64_t shape[PLQ_MAX_DIM] = {0};
+    int64_t n_sites = 0, n_vars = 0;
+    int model = 0, q = 2, kernel = 0, profile = 0;
+    double beta = 0, j = 1, h = 0;
+    double offsets[PLQ_MAX_Q] = {0};
+    uint32_t seed32 = 0;
+    uint64_t sweeps_done = 0, sweeps_between = 1;
+    // local fields
+    plq::Geom g{};
+    plq::SwarGeom sg{};
+    bool swar = false;
+    uint8_t *col[2] = {nullptr, nullptr};  // site colours
+    uint8_t *links[2 * plq::MAX_D] = {nullptr};
+    uint8_t **d_links = nullptr;           // device copy of the link array pointers
+    uint32_t *d_table = nullptr;
+    size_t table_len = 0;
+    // cluster
+    plq::CCGeom cg{};
+    uint8_t *labels = nullptr;
+    int32_t *parent = nullptr;
+    int32_t *roots = nullptr;
+    int32_t *sizes = nullptr;
+    unsigned long long *hist = nullptr;
+    int *d_flag = nullptr;
+    uint32_t bond_t = 0;
+    int seeded = 0, forced = 0;
+    uint64_t pass_cap = 0;
+    int64_t last_relabel_blocks = 0;
+    bool have_partition = false;
+    // shared scratch
+    uint8_t *d_lat = nullptr;              // lattice-order staging
+    long long *d_partials = nullptr;       // 2 * MAX_PASS_BLOCKS * REC_K
+    long long *d_meas = nullptr;           // REC_K (+ cluster stats)
+    long long *d_rec = nullptr;            // rec_cap * REC_K
+    uint64_t rec_every = 0, rec_cap = 0, rec_count = 0;
+    std::vector<uint64_t> rec_sweeps;
+    std::vector<hipEvent_t> pending;
+
+    ~plq_chain() {
+        (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto p : {(void *)col[0], (void *)col[1], (void *)d_links, (void *)d_table, (void *)labels,
+                       (void *)parent, (void *)roots, (void *)sizes, (void *)hist, (void *)d_flag, (void *)d_lat,
+                       (void *)d_partials, (void *)d_meas, (void *)d_rec})
+            if (p) (void)hipFree(p);
+        for (int k = 0; k < 2 * plq::MAX_D; ++k)
+            if (links[k]) (void)hipFree(links[k]);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int check_device(int *dev) {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0)
+        return fail(PLQ_ENODEV, "no HIP device is available (the reference's Gpu::new() would return None)");
+    PLQ_HIP(hipGetDevice(dev));
+    return PLQ_OK;
+}
+
+int validate_states(const uint8_t *start, size_t n, int q) {
+    for (size_t i = 0; i < n; ++i)
+        if (start[i] >= q) {
+            char buf[128];
+            snprintf(buf, sizeof buf, "state %u at index %zu is not below q = %d", start[i], i, q);
+            return fail(PLQ_EINVAL, buf);
+        }
+    return PLQ_OK;
+}
+
+int alloc(void **p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(PLQ_ENOMEM, std::string("hipMalloc of ") + std::to_string(bytes) + " bytes failed: " +
+                                    hipGetErrorString(e));
+    }
+    return PLQ_OK;
+}
+
+#define PLQ_TRY(expr)           \
+    do {                        \
+        int rc_ = (expr);       \
+        if (rc_ != PLQ_OK) {    \
+            return rc_;         \
+        }                       \
+    } while (0)
+
+// Build the geometry of the colour-interleaved layout.
+void make_geom(plq_chain *c) {
+    plq::Geom &g = c->g;
+    g.D = c->D;
+    g.N = c->n_sites;
+    for (int mu = 0; mu < plq::MAX_D; ++mu) g.L[mu] = mu < c->D ? (int32_t)c->shape[mu] : 1;
+    g.H = g.L[0] / 2;
+    g.R = (int32_t)(c->n_sites / g.L[0]);
+    int32_t st = 1;
+    g.rstride[0] = 0;
+    for (int mu = 1; mu < plq::MAX_D; ++mu) {
+        g.rstride[mu] = st;
+        st *= g.L[mu];
+    }
+}
+
+int upload_table(plq_chain *c) {
+    plq::TableParams tp{};
+    tp.D = c->D;
+    tp.model = c->model;
+    tp.q = c->q;
+    tp.kernel = c->kernel;
+    tp.profile = c->profile;
+    tp.beta = c->beta;
+    tp.j = c->j;
+    tp.h = c->h;
+    for (int a = 0; a < PLQ_MAX_Q; ++a) tp.offsets[a] = c->offsets[a];
+    std::vector<uint32_t> table;
+    std::string err;
+    if (!plq::build_table(tp, table, err)) return fail(PLQ_EINVAL, err);
+    if (c->d_table) (void)hipFree(c->d_table);
+    c->d_table = nullptr;
+    PLQ_TRY(alloc((void **)&c->d_table, table.size() * sizeof(uint32_t)));
+    PLQ_HIP(hipMemcpy(c->d_table, table.data(), table.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    c->table_len = table.size();
+    return PLQ_OK;
+}
+
+int alloc_scratch(plq_chain *c) {
+    PLQ_TRY(alloc((void **)&c->d_lat, (size_t)c->n_vars));
+    PLQ_TRY(alloc((void **)&c->d_partials, sizeof(long long) * 2 * plq::MAX_PASS_BLOCKS * plq::REC_K));
+    PLQ_TRY(alloc((void **)&c->d_meas, sizeof(long long) * plq::REC_K));
+    return PLQ_OK;
+}
+
+// ---------------------------------------------------------- local sweeps ---
+
+plq::SitePassArgs site_args(plq_chain *c, int color, uint32_t sweep, int mode, int meas) {
+    plq::SitePassArgs a{};
+    a.oth = c->col[1 - color];
+    a.own = c->col[color];
+    a.g = c->g;
+    a.sg = c->sg;
+    a.swar = c->swar;
+    a.model = c->model == PLQ_MODEL_POTTS ? 1 : 0;
+    a.mode = mode;
+    a.color = color;
+    a.seed = c->seed32;
+    a.skey = sweep * plq::GOLDEN;
+    a.q = c->q;
+    a.table = c->d_table;
+    a.partials = c->d_partials + (color ? plq::MAX_PASS_BLOCKS * plq::REC_K : 0);
+    a.meas = meas;
+    a.max_blocks = plq::MAX_PASS_BLOCKS;
+    a.stream = c->stream;
+    return a;
+}
+
+// Enqueue a measurement of the current site field into rec (REC_K entries).
+void enqueue_site_measure(plq_chain *c, long long *rec) {
+    plq::SitePassArgs a0 = site_args(c, 0, 0, 2, 1);
+    plq::SitePassArgs a1 = site_args(c, 1, 0, 2, 1);
+    plq::launch_site_pass(a0);
+    plq::launch_site_pass(a1);
+    plq::launch_finalize_record(a0.partials, plq::site_pass_blocks(a0), a1.partials, plq::site_pass_blocks(a1), rec,
+                                c->stream);
+}
+
+void enqueue_measure(plq_chain *c, long long *rec) {
+    if (c->kind == KIND_SITE) {
+        enqueue_site_measure(c, rec);
+    } else if (c->kind == KIND_LINK) {
+        plq::launch_plaquette_sum(c->d_links, c->g, c->d_partials, plq::MAX_PASS_BLOCKS, rec, c->stream);
+    } else {
+        plq::launch_potts_measure(c->labels, c->cg, c->q, c->d_partials, plq::MAX_PASS_BLOCKS, rec, c->stream);
+    }
+}
+
+long long *rec_slot(plq_chain *c) {
+    uint64_t slot = c->rec_count % c->rec_cap;
+    c->rec_sweeps[slot] = c->sweeps_done;
+    c->rec_count += 1;
+    return c->d_rec + slot * plq::REC_K;
+}
+
+int enqueue_local_sweep(plq_chain *c) {
+    uint32_t sweep = (uint32_t)c->sweeps_done;
+    bool record = c->rec_every && ((c->sweeps_done + 1) % c->rec_every == 0);
+    int mode = c->kernel == PLQ_KERNEL_METROPOLIS ? 0 : 1;
+    if (c->kind == KIND_SITE) {
+        plq::SitePassArgs a0 = site_args(c, 0, sweep, mode, record);
+        plq::SitePassArgs a1 = site_args(c, 1, sweep, mode, record);
+        plq::launch_site_pass(a0);
+        plq::launch_site_pass(a1);
+        c->sweeps_done += 1;
+        if (record)
+            plq::launch_finalize_record(a0.partials, plq::site_pass_blocks(a0), a1.partials,
+                                        plq::site_pass_blocks(a1), rec_slot(c), c->stream);
+    } else {
+        for (int dir = 0; dir < c->D; ++dir)
+            for (int color = 0; color < 2; ++color) {
+                plq::LinkPassArgs a{};
+                a.arr = c->d_links;
+                a.g = c->g;
+                a.dir = dir;
+                a.color = color;
+                a.mode = mode;
+                a.seed = c->seed32;
+                a.skey = sweep * plq::GOLDEN;
+                a.table = c->d_table;
+                a.stream = c->stream;
+                plq::launch_link_pass(a);
+            }
+        c->sweeps_done += 1;
+        if (record) enqueue_measure(c, rec_slot(c));
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(PLQ_EDEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
+    return PLQ_OK;
+}
+
+int check_flag(plq_chain *c) {
+    if (c->kind != KIND_CLUSTER) return PLQ_OK;
+    int flag = 0;
+    PLQ_HIP(hipMemcpyAsync(&flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    PLQ_HIP(hipStreamSynchronize(c->stream));
+    if (flag) {
+        char buf[256];
+        snprintf(buf, sizeof buf,
+                 "cluster labelling did not converge within its budget of %llu hops on %lld sites; "
+                 "the configuration is undefined",
+                 (unsigned long long)c->pass_cap, (long long)c->n_sites);
+        return fail(PLQ_ENOCONV, buf);
+    }
+    return PLQ_OK;
+}
+
+int enqueue_cluster_sweep(plq_chain *c) {
+    plq::ClusterSweepArgs a{};
+    a.labels = c->labels;
+    a.parent = c->parent;
+    a.cg = c->cg;
+    a.seed32 = c->seed32;
+    a.sweep = (uint32_t)c->sweeps_done;
+    a.bond_t = c->bond_t;
+    a.q = c->q;
+    a.seeded = c->seeded;
+    a.forced = c->forced;
+    a.cap = (uint32_t)(c->pass_cap > 0xffffffffull ? 0xffffffffull : c->pass_cap);
+    a.flag = c->d_flag;
+    a.partials = c->d_partials + plq::MAX_PASS_BLOCKS * plq::REC_K;  // the measure kernels use the first half
+    a.max_blocks = plq::MAX_PASS_BLOCKS;
+    a.stream = c->stream;
+    c->last_relabel_blocks = plq::launch_cluster_sweep(a);
+    c->have_partition = true;
+    c->sweeps_done += 1;
+    bool record = c->rec_every && (c->sweeps_done % c->rec_every == 0);
+    if (record) enqueue_measure(c, rec_slot(c));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(PLQ_EDEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
+    return PLQ_OK;
+}
+
+int read_lattice(plq_chain *c, uint8_t *out, size_t n) {
+    if (n != (size_t)c->n_vars) return fail(PLQ_EINVAL, "output length does not match the variable count");
+    if (c->kind == KIND_SITE) plq::launch_unpack_sites(c->d_lat, c->col[0], c->col[1], c->g, c->stream);
+    else if (c->kind == KIND_LINK) plq::launch_unpack_links(c->d_lat, c->d_links, c->g, c->stream);
+    const uint8_t *src = c->kind == KIND_CLUSTER ? c->labels : c->d_lat;
+    PLQ_HIP(hipMemcpyAsync(out, src, n, hipMemcpyDeviceToHost, c->stream));
+    PLQ_HIP(hipStreamSynchronize(c->stream));
+    return check_flag(c);
+}
+
+int write_lattice(plq_chain *c, const uint8_t *in, size_t n) {
+    if (n != (size_t)c->n_vars) return fail(PLQ_EINVAL, "input length does not match the variable count");
+    PLQ_TRY(validate_states(in, n, c->q));
+    uint8_t *dst = c->kind == KIND_CLUSTER ? c->labels : c->d_lat;
+    PLQ_HIP(hipMemcpyAsync(dst, in, n, hipMemcpyHostToDevice, c->stream));
+    if (c->kind == KIND_SITE) plq::launch_pack_sites(c->d_lat, c->col[0], c->col[1], c->g, c->stream);
+    else if (c->kind == KIND_LINK) plq::launch_pack_links(c->d_lat, c->d_links, c->g, c->stream);
+    PLQ_HIP(hipStreamSynchronize(c->stream));
+    c->have_partition = false;
+    return PLQ_OK;
+}
+
+void fill_obs(const plq_chain *c, const long long *rec, uint64_t sweep, plq_obs *o) {
+    std::memset(o, 0, sizeof *o);
+    o->sweep = sweep;
+    o->n_vars = c->n_vars;
+    o->bond_sum = rec[0];
+    for (int a = 0; a < PLQ_MAX_Q; ++a) o->counts[a] = rec[2 + a];
+    if (c->kind == KIND_LINK) {
+        o->energy = -c->j * (double)rec[0];  // gauge/mod.rs:275-286
+    } else if (c->model == PLQ_MODEL_ISING) {
+        o->spin_sum = rec[2] - rec[3];
+        o->energy = -c->j * (double)o->bond_sum - c->h * (double)o->spin_sum;  // ising/mod.rs:97-117
+    } else {
+        double offset = 0.0;  // potts/mod.rs:253-271
+        bool symmetric = true;
+        for (int a = 0; a < c->q; ++a) symmetric = symmetric && c->offsets[a] == 0.0;
+        if (!symmetric)
+            for (int a = 0; a < c->q; ++a) offset += c->offsets[a] * (double)rec[2 + a];
+        o->energy = -c->j * (double)o->bond_sum - offset;
+    }
+}
+
+int max_box_sites(int device) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 16384;
+    // One int32 parent per box site; stay within 64 KiB of LDS per block so
+    // the launch needs no opt-in attribute and two boxes fit one CU.
+    size_t lds = prop.sharedMemPerBlock < 65536 ? prop.sharedMemPerBlock : 65536;
+    int sites = 1024;
+    while ((size_t)sites * 2 * sizeof(int32_t) <= lds) sites *= 2;
+    return sites;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *plq_last_error(void) { return g_error.c_str(); }
+
+uint32_t plq_fold_seed(uint64_t seed) { return plq::fold_seed(seed); }
+
+int plq_device_count(int32_t *count) {
+    if (!count) return fail(PLQ_EINVAL, "count is null");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    *count = e == hipSuccess ? n : 0;
+    if (*count <= 0) return fail(PLQ_ENODEV, "no HIP device is available");
+    return PLQ_OK;
+}
+
+int plq_threshold_table(const plq_local_desc *d, uint32_t *out, size_t cap) {
+    if (!d) return fail(PLQ_EINVAL, "descriptor is null");
+    plq::TableParams tp{};
+    tp.D = d->dim;
+    tp.model = d->model;
+    tp.q = d->q;
+    tp.kernel = d->kernel;
+    tp.profile = d->profile;
+    tp.beta = d->beta;
+    tp.j = d->j;
+    tp.h = d->h;
+    for (int a = 0; a < PLQ_MAX_Q; ++a) tp.offsets[a] = d->offsets[a];
+    std::vector<uint32_t> t;
+    std::string err;
+    if (!plq::build_table(tp, t, err)) return fail(PLQ_EINVAL, err);
+    if (out) {
+        if (cap < t.size()) return fail(PLQ_EINVAL, "table buffer too small");
+        std::memcpy(out, t.data(), t.size() * sizeof(uint32_t));
+    }
+    return (int)t.size();
+}
+
+int plq_chain_create_local(const plq_local_desc *d, const uint8_t *start, size_t n, plq_chain **out) {
+    if (!d || !out) return fail(PLQ_EINVAL, "descriptor or output handle is null");
+    *out = nullptr;
+    bool link = d->model == PLQ_MODEL_Z2;
+    if (d->model < 0 || d->model > 2) return fail(PLQ_EINVAL, "unknown model");
+    if (d->kernel < 0 || d->kernel > 1) return fail(PLQ_EINVAL, "unknown kernel");
+    if (d->profile < 0 || d->profile > 1) return fail(PLQ_EINVAL, "unknown threshold profile");
+    if (d->dim < (link ? 2 : 1) || d->dim > plq::MAX_D)
+        return fail(PLQ_EINVAL, link ? "the Z2 gauge device path supports dimensions 2..4"
+                                     : "the site device path supports dimensions 1..4");
+    if ((d->model != PLQ_MODEL_POTTS && d->q != 2) || d->q < 2 || d->q > 8)
+        return fail(PLQ_EINVAL, "q must be 2 for Ising and Z2, and 2..8 for Potts");
+    int64_t ns = 1;
+    for (int mu = 0; mu < d->dim; ++mu) {
+        if (d->shape[mu] < 2 || (d->shape[mu] & 1)) {
+            return fail(PLQ_EINVAL, "the parallel checkerboard needs even lattice extents (device.rs:714-719)");
+        }
+        ns *= d->shape[mu];
+        if (ns > (int64_t(1) << 31)) return fail(PLQ_ENOMEM, "lattice exceeds 2^31 sites");
+    }
+    int64_t nv = link ? ns * d->dim : ns;
+    if (nv > (int64_t(1) << 32) - 1) return fail(PLQ_ENOMEM, "variable index exceeds 32 bits");
+    if (!start || n != (size_t)nv) return fail(PLQ_EINVAL, "start configuration and lattice disagree on variable count");
+    PLQ_TRY(validate_states(start, n, d->q));
+    int dev = 0;
+    PLQ_TRY(check_device(&dev));
+
+    plq_chain *c = new plq_chain();
+    c->kind = link ? KIND_LINK : KIND_SITE;
+    c->device = dev;
+    c->D = d->dim;
+    for (int mu = 0; mu < d->dim; ++mu) c->shape[mu] = d->shape[mu];
+    c->n_sites = ns;
+    c->n_vars = nv;
+    c->model = d->model;
+    c->q = d->q;
+    c->kernel = d->kernel;
+    c->profile = d->profile;
+    c->beta = d->beta;
+    c->j = d->j;
+    c->h = d->h;
+    for (int a = 0; a < PLQ_MAX_Q; ++a) c->offsets[a] = d->offsets[a];
+    c->seed32 = plq::fold_seed(d->seed);
+    c->sweeps_between = d->sweeps_between;
+    make_geom(c);
+    bool all_pow2 = true;
+    for (int mu = 0; mu < d->dim; ++mu) all_pow2 = all_pow2 && is_pow2(d->shape[mu]);
+    // PLQ_FORCE_GENERIC=1 selects the one-site-per-lane kernel even where the
+    // four-sites-per-lane one applies; tests use it to check that both launch
+    // shapes produce identical bits.
+    const char *force = getenv("PLQ_FORCE_GENERIC");
+    bool generic_only = force && force[0] == '1';
+    c->swar = !generic_only && c->kind == KIND_SITE && d->model == PLQ_MODEL_ISING && all_pow2 && d->shape[0] >= 8;
+    if (c->swar) {
+        c->sg.lh4 = ilog2(d->shape[0] / 8);
+        c->sg.L1 = d->dim > 1 ? (uint32_t)d->shape[1] : 1;
+        c->sg.L2 = d->dim > 2 ? (uint32_t)d->shape[2] : 1;
+        c->sg.L3 = d->dim > 3 ? (uint32_t)d->shape[3] : 1;
+        c->sg.ll1 = ilog2(c->sg.L1);
+        c->sg.ll2 = ilog2(c->sg.L2);
+    }
+    int rc = PLQ_OK;
+    auto bail = [&](int code) {
+        delete c;
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(PLQ_EDEVICE, "hipStreamCreate failed"));
+    if ((rc = upload_table(c)) != PLQ_OK) return bail(rc);
+    if ((rc = alloc_scratch(c)) != PLQ_OK) return bail(rc);
+    size_t half = (size_t)(ns / 2) + 16;
+    if (c->kind == KIND_SITE) {
+        for (int k = 0; k < 2; ++k)
+            if ((rc = alloc((void **)&c->col[k], half)) != PLQ_OK) return bail(rc);
+    } else {
+        for (int k = 0; k < 2 * d->dim; ++k)
+            if ((rc = alloc((void **)&c->links[k], half)) != PLQ_OK) return bail(rc);
+        if ((rc = alloc((void **)&c->d_links, sizeof(uint8_t *) * 2 * plq::MAX_D)) != PLQ_OK) return bail(rc);
+        if (hipMemcpy(c->d_links, c->links, sizeof(uint8_t *) * 2 * plq::MAX_D, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(PLQ_EDEVICE, "hipMemcpy of link pointers failed"));
+    }
+    if ((rc = write_lattice(c, start, n)) != PLQ_OK) return bail(rc);
+    *out = c;
+    return PLQ_OK;
+}
+
+int plq_chain_create_cluster(const plq_cluster_desc *d, const uint8_t *start, size_t n, plq_chain **out) {
+    if (!d || !out) return fail(PLQ_EINVAL, "descriptor or output handle is null");
+    *out = nullptr;
+    if (d->dim < 1 || d->dim > plq::MAX_D) return fail(PLQ_EINVAL, "the cluster device path supports dimensions 1..4");
+    if (d->q < 2 || d->q > PLQ_MAX_Q) return fail(PLQ_EINVAL, "a cluster redraw needs 2 <= q <= 16 states");
+    if (d->extent < 0 || d->extent > 1 || d->relabel < 0 || d->relabel > 1)
+        return fail(PLQ_EINVAL, "unknown extent or relabel rule");
+    if (d->q == 2 && d->extent == PLQ_EXTENT_ALL && d->relabel == PLQ_RELABEL_FORCED_CHANGE)
+        return fail(PLQ_EINVAL,
+                    "at q = 2 a forced change of every cluster is the deterministic global flip "
+                    "(updater.rs:447-478)");
+    if (!std::isfinite(d->gap) || d->gap < 0.0)
+        return fail(PLQ_EINVAL, "the bond probability needs a finite, non-negative gap");
+    if (!std::isfinite(d->beta)) return fail(PLQ_EINVAL, "beta must be finite");
+    if (d->profile < 0 || d->profile > 1) return fail(PLQ_EINVAL, "unknown threshold profile");
+    int64_t ns = 1;
+    for (int mu = 0; mu < d->dim; ++mu) {
+        if (d->shape[mu] < 1) return fail(PLQ_EINVAL, "every extent must be positive");
+        ns *= d->shape[mu];
+        if (ns > (int64_t(1) << 31) - 1) return fail(PLQ_ENOMEM, "lattice exceeds 2^31 - 1 sites");
+    }
+    if (ns * d->dim > (int64_t(1) << 32) - 1) return fail(PLQ_ENOMEM, "bond index exceeds 32 bits");
+    if (!start || n != (size_t)ns) return fail(PLQ_EINVAL, "start configuration and lattice disagree on site count");
+    PLQ_TRY(validate_states(start, n, d->q));
+    int dev = 0;
+    PLQ_TRY(check_device(&dev));
+
+    plq_chain *c = new plq_chain();
+    c->kind = KIND_CLUSTER;
+    c->device = dev;
+    c->D = d->dim;
+    for (int mu = 0; mu < d->dim; ++mu) c->shape[mu] = d->shape[mu];
+    c->n_sites = ns;
+    c->n_vars = ns;
+    c->model = PLQ_MODEL_POTTS;
+    c->q = d->q;
+    c->profile = d->profile;
+    c->beta = d->beta;
+    c->j = 1.0;
+    c->seed32 = plq::fold_seed(d->seed);
+    c->sweeps_between = d->sweeps_between;
+    c->seeded = d->extent == PLQ_EXTENT_SEEDED;
+    c->forced = d->relabel == PLQ_RELABEL_FORCED_CHANGE;
+    c->bond_t = plq::bond_threshold(d->beta, d->gap, d->profile);
+    c->cg = plq::make_ccgeom(d->dim, d->shape, max_box_sites(dev));
+    // The reference's cap counts propagation passes (gpu_cluster.rs:620-625);
+    // here every find and union walks strictly decreasing indices, so the
+    // site count bounds any walk.
+    c->pass_cap = (uint64_t)ns + 64;
+    int rc = PLQ_OK;
+    auto bail = [&](int code) {
+        delete c;
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(PLQ_EDEVICE, "hipStreamCreate failed"));
+    if ((rc = alloc_scratch(c)) != PLQ_OK) return bail(rc);
+    if ((rc = alloc((void **)&c->labels, (size_t)ns)) != PLQ_OK) return bail(rc);
+    if ((rc = alloc((void **)&c->parent, (size_t)ns * sizeof(int32_t))) != PLQ_OK) return bail(rc);
+    if ((rc = alloc((void **)&c->d_flag, sizeof(int))) != PLQ_OK) return bail(rc);
+    if (hipMemset(c->d_flag, 0, sizeof(int)) != hipSuccess) return bail(fail(PLQ_EDEVICE, "hipMemset failed"));
+    if ((rc = write_lattice(c, start, n)) != PLQ_OK) return bail(rc);
+    *out = c;
+    return PLQ_OK;
+}
+
+int plq_chain_destroy(plq_chain *c) {
+    if (!c) return fail(PLQ_EINVAL, "chain is null");
+    delete c;
+    return PLQ_OK;
+}
+
+int plq_chain_advance(plq_chain *c, uint64_t sweeps) {
+    if (!c) return fail(PLQ_EINVAL, "chain is null");
+    std::lock_guard<std::mutex> lk(c->mu);
+    PLQ_HIP(hipSetDevice(c->device));
+    for (uint64_t t = 0; t < sweeps; ++t) {
+        int rc = c->kind == KIND_CLUSTER ? enqueue_cluster_sweep(c) : enqueue_local_sweep(c);
+        if (rc != PLQ_OK) return rc;
+    }
+    if (c->kind == KIND_CLUSTER && sweeps > 0) return check_flag(c);
+    return PLQ_OK;
+}
+
+int plq_chain_next(plq_chain *c, uint8_t *out, size_t n) {
+    if (!c) return fail(PLQ_EINVAL, "chain is null");
+    PLQ_TRY(plq_chain_advance(c, c->sweeps_between));
+    std::lock_guard<std::mutex> lk(c->mu);
+    return read_lattice(c, out, n);
+}
+
+int plq_chain_read(plq_chain *c, uint8_t *out, size_t n) {
+    if (!c || !out) return fail(PLQ_EINVAL, "chain or buffer is null");
+    std::lock_guard<std::mutex> lk(c->mu);
+    PLQ_HIP(hipSetDevice(c->device));
+    return read_lattice(c, out, n);
+}
+
+int plq_chain_write(plq_chain *c, const uint8_t *in, size_t n) {
+    if (!c || !in) return fail(PLQ_EINVAL, "chain or buffer is null");
+    std::lock_guard<std::mutex> lk(c->mu);
+    PLQ_HIP(hipSetDevice(c->device));
+    return write_lattice(c, in, n);
+}
+
+int plq_chain_measure(plq_chain *c, plq_obs *out) {
+    if (!c || !out) return fail(PLQ_EINVAL, "chain or output is null");
+    std::lock_guard<std::mutex> lk(c->mu);
+    PLQ_HIP(hipSetDevice(c->device));
+    enqueue_measure(c, c->d_meas);
+    long long rec[plq::REC_K];
+    PLQ_HIP(hipMemcpyAsync(rec, c->d_meas, sizeof rec, hipMemcpyDeviceToHost, c->stream));
+    PLQ_HIP(hipStreamSynchronize(c->stream));
+    fill_obs(c, rec, c->sweeps_done, out);
+    return check_flag(c);
+}
+
+int plq_chain_record(plq_chain *c, uint64_t every, uint64_t capacity) {
+    if (!c) return fail(PLQ_EINVAL, "chain is null");
+    std::lock_guard<std::mutex> lk(c->mu);
+    PLQ_HIP(hipSetDevice(c->device));
+    PLQ_HIP(hipStreamSynchronize(c->stream));
+    if (c->d_rec) (void)hipFree(c->d_rec);
+    c->d_rec = nullptr;
+    c->rec_every = 0;
+    c->rec_cap = 0;
+    c->rec_count = 0;
+    c->rec_sweeps.clear();
+    if (every == 0) return PLQ_OK;
+    if (capacity == 0) return fail(PLQ_EINVAL, "recording needs a positive capacity");
+    PLQ_TRY(alloc((void **)&c->d_rec, sizeof(long long) * plq::REC_K * capacity));
+    c->rec_every = every;
+    c->rec_cap = capacity;
+    c->rec_sweeps.assign(capacity, 0);
+    return PLQ_OK;
+}
+
+int plq_chain_series(plq_chain *c, plq_obs *out, uint64_t capacity, uint64_t *n_out) {
+    if (!c || !n_out) return fail(PLQ_EINVAL, "chain or count is null");
+    std::lock_guard<std::mutex> lk(c->mu);
+    PLQ_HIP(hipSetDevice(c->device));
+    uint64_t have = c->rec_count < c->rec_cap ? c->rec_count : c->rec_cap;
+    *n_out = have;
+    if (!out || have == 0) return PLQ_OK;
+    if (capacity < have) return fail(PLQ_EINVAL, "series buffer too small");
+    std::vector<long long> host(have * plq::REC_K > 0 ? c->rec_cap * plq::REC_K : 1);
+    PLQ_HIP(hipMemcpyAsync(host.data(), c->d_rec, sizeof(long long) * plq::REC_K * c->rec_cap,
+                           hipMemcpyDeviceToHost, c->stream));
+    PLQ_HIP(hipStreamSynchronize(c->stream));
+    uint64_t first = c->rec_count - have;
+    for (uint64_t k = 0; k < have; ++k) {
+        uint64_t slot = (first + k) % c->rec_cap;
+        fill_obs(c, &host[slot * plq::REC_K], c->rec_sweeps[slot], &out[k]);
+    }
+    return check_flag(c);
+}
+
+int plq_chain_sweeps_done(plq_chain *c, uint64_t *out) {
+    if (!c || !out) return fail(PLQ_EINVAL, "chain or output is null");
+    *out = c->sweeps_done;
+    return PLQ_OK;
+}
+
+int plq_chain_set_sweep(plq_chain *c, uint64_t sweep) {
+    if (!c) return fail(PLQ_EINVAL, "chain is null");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->sweeps_done = sweep;
+    return PLQ_OK;
+}
+
+int plq_chain_sync(plq_chain *c) {
+    if (!c) return fail(PLQ_EINVAL, "chain is null");
+    std::lock_guard<std::mutex> lk(c->mu);
+    PLQ_HIP(hipSetDevice(c->device));
+    PLQ_HIP(hipStreamSynchronize(c->stream));
+    return check_flag(c);
+}
+
+int plq_chain_stream(plq_chain *c, void **stream) {
+    if (!c || !stream) return fail(PLQ_EINVAL, "chain or output is null");
+    *stream = (void *)c->stream;
+    return PLQ_OK;
+}
+
+int plq_chain_set_pass_cap(plq_chain *c, uint64_t cap) {
+    if (!c) return fail(PLQ_EINVAL, "chain is null");
+    if (c->kind != KIND_CLUSTER) return fail(PLQ_EINVAL, "only cluster chains have a labelling budget");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->pass_cap = cap;
+    return PLQ_OK;
+}
+
+int plq_chain_cluster_roots(plq_chain *c, int64_t *roots, size_t n) {
+    if (!c || !roots) return fail(PLQ_EINVAL, "chain or output is null");
+    if (c->kind != KIND_CLUSTER) return fail(PLQ_EINVAL, "not a cluster chain");
+    if (n != (size_t)c->n_sites) return fail(PLQ_EINVAL, "output length does not match the site count");
+    if (!c->have_partition) return fail(PLQ_EINVAL, "no cluster sweep has run since the last write");
+    std::lock_guard<std::mutex> lk(c->mu);
+    PLQ_HIP(hipSetDevice(c->device));
+    if (!c->roots) PLQ_TRY(alloc((void **)&c->roots, sizeof(int32_t) * (size_t)c->n_sites));
+    uint32_t cap = (uint32_t)(c->pass_cap > 0xffffffffull ? 0xffffffffull : c->pass_cap);
+    plq::launch_cluster_resolve(c->parent, c->roots, c->cg, cap, c->d_flag, nullptr, nullptr, c->stream);
+    std::vector<int32_t> host((size_t)c->n_sites);
+    PLQ_HIP(hipMemcpyAsync(host.data(), c->roots, sizeof(int32_t) * host.size(), hipMemcpyDeviceToHost, c->stream));
+    PLQ_HIP(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < host.size(); ++i) roots[i] = host[i];
+    return check_flag(c);
+}
+
+int plq_chain_cluster_stats(plq_chain *c, plq_cluster_stats *out) {
+    if (!c || !out) return fail(PLQ_EINVAL, "chain or output is null");
+    if (c->kind != KIND_CLUSTER) return fail(PLQ_EINVAL, "not a cluster chain");
+    if (!c->have_partition) return fail(PLQ_EINVAL, "no cluster sweep has run since the last write");
+    std::lock_guard<std::mutex> lk(c->mu);
+    PLQ_HIP(hipSetDevice(c->device));
+    if (!c->roots) PLQ_TRY(alloc((void **)&c->roots, sizeof(int32_t) * (size_t)c->n_sites));
+    if (!c->sizes) PLQ_TRY(alloc((void **)&c->sizes, sizeof(int32_t) * (size_t)c->n_sites));
+    if (!c->hist) PLQ_TRY(alloc((void **)&c->hist, sizeof(unsigned long long) * 42));
+    PLQ_HIP(hipMemsetAsync(c->sizes, 0, sizeof(int32_t) * (size_t)c->n_sites, c->stream));
+    PLQ_HIP(hipMemsetAsync(c->hist, 0, sizeof(unsigned long long) * 42, c->stream));
+    uint32_t cap = (uint32_t)(c->pass_cap > 0xffffffffull ? 0xffffffffull : c->pass_cap);
+    plq::launch_cluster_resolve(c->parent, c->roots, c->cg, cap, c->d_flag, c->sizes, c->hist, c->stream);
+    unsigned long long h[42];
+    PLQ_HIP(hipMemcpyAsync(h, c->hist, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    long long part[plq::REC_K];
+    std::vector<long long> parts((size_t)c->last_relabel_blocks * plq::REC_K);
+    PLQ_HIP(hipMemcpyAsync(parts.data(), c->d_partials + plq::MAX_PASS_BLOCKS * plq::REC_K,
+                           sizeof(long long) * parts.size(), hipMemcpyDeviceToHost, c->stream));
+    PLQ_HIP(hipStreamSynchronize(c->stream));
+    std::memset(part, 0, sizeof part);
+    for (int64_t b = 0; b < c->last_relabel_blocks; ++b)
+        for (int k = 0; k < plq::REC_K; ++k) part[k] += parts[(size_t)b * plq::REC_K + k];
+    std::memset(out, 0, sizeof *out);
+    out->sweep = c->sweeps_done;
+    for (int k = 0; k < 40; ++k) out->size_hist[k] = (int64_t)h[k];
+    out->n_clusters = (int64_t)h[40];
+    out->largest = (int64_t)h[41];
+    out->moved = part[1];
+    return check_flag(c);
+}
+
+}  // extern "C"

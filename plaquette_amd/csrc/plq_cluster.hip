@@ -1,0 +1,470 @@
+// plq_cluster.hip -- Swendsen-Wang / Wolff cluster sweep on gfx950.
+//
+// The reference builds clusters on the device by repeated min-label
+// propagation with a host read-back every four passes
+// (cluster_prelude.wgsl:61-147, gpu_cluster.rs:492-526).  Its fixed point
+// gives every site the minimum site index of its cluster, and the relabel
+// draw is keyed on that root (cluster_relabel.wgsl:39-90).  Any algorithm
+// reaching the same partition and the same root reproduces the reference's
+// labels bit for bit, so this file uses union-find instead:
+//
+//   1. cc_local   one block per box of the lattice: draw the box's internal
+//                 bonds (cluster_prelude.wgsl:61-80; bond = site*D + axis,
+//                 open iff the labels agree and m < T_p) and union their ends
+//                 in LDS, then store each site's box-local root as a global
+//                 site index.
+//   2. cc_boundary one lane per box-face site: draw the bonds that leave a
+//                 box and union their ends in the global forest with
+//                 compare-and-swap.
+//   3. cc_relabel one lane per site: find the root, draw the new label keyed
+//                 on it (Redraw or ForcedChange, optionally only the seed
+//                 site's cluster for Wolff), write the label.
+//
+// Every union links the larger root under the smaller one, so parent[x] <= x
+// always holds and the root of a tree is the minimum site index in it: the
+// same root the reference's propagation converges to (and the first-site
+// order the CPU Forest compacts by, cluster.rs:196-226).  Every loop here
+// walks strictly decreasing indices or retries only after another lane made
+// progress, so each terminates; an explicit hop budget (the pass cap of
+// gpu_cluster.rs:620-625) additionally lets the host force the
+// non-convergence error.
+#include <hip/hip_runtime.h>
+
+#include "plq_common.hpp"
+#include "plq_internal.hpp"
+
+namespace plq {
+
+__device__ __forceinline__ int32_t load_relaxed(const int32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_relaxed(int32_t *p, int32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------- local ---
+
+__device__ __forceinline__ int32_t lds_load(const int32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_store(int32_t *p, int32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ int32_t lds_find(int32_t *P, int32_t x) {
+    int32_t p = lds_load(&P[x]);
+    while (p != x) {
+        int32_t gp = lds_load(&P[p]);
+        if (gp != p) lds_store(&P[x], gp);  // path halving; gp is an ancestor of x
+        x = p;
+        p = lds_load(&P[x]);
+    }
+    return x;
+}
+
+__device__ __forceinline__ void lds_union(int32_t *P, int32_t a, int32_t b) {
+    for (;;) {
+        a = lds_find(P, a);
+        b = lds_find(P, b);
+        if (a == b) return;
+        int32_t hi = a > b ? a : b, lo = a > b ? b : a;
+        int32_t old = atomicCAS(&P[hi], hi, lo);
+        if (old == hi) return;
+        a = old;  // hi gained a parent meanwhile; continue from it
+        b = lo;
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(1024) void cc_local(const uint8_t *__restrict__ labels, int32_t *__restrict__ parent,
+                                                 CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t) {
+    extern __shared__ int32_t P[];
+    int32_t o[MAX_D], e[MAX_D], ls[MAX_D];
+    int64_t rest = blockIdx.x;
+    int32_t nloc = 1;
+#pragma unroll
+    for (int mu = 0; mu < D; ++mu) {
+        int32_t b = (int32_t)(rest % cg.nbox[mu]);
+        rest /= cg.nbox[mu];
+        o[mu] = b * cg.B[mu];
+        e[mu] = cg.L[mu] - o[mu] < cg.B[mu] ? cg.L[mu] - o[mu] : cg.B[mu];
+        ls[mu] = nloc;
+        nloc *= e[mu];
+    }
+    for (int32_t l = threadIdx.x; l < nloc; l += blockDim.x) P[l] = l;
+    __syncthreads();
+    for (int32_t l = threadIdx.x; l < nloc; l += blockDim.x) {
+        int32_t y[MAX_D];
+        int32_t r = l;
+        int64_t gsite = 0;
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) {
+            y[mu] = r % e[mu];
+            r /= e[mu];
+            gsite += (int64_t)(o[mu] + y[mu]) * cg.stride[mu];
+        }
+        uint8_t lab = labels[gsite];
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) {
+            int32_t lp;
+            int64_t gp;
+            if (y[mu] + 1 < e[mu]) {
+                lp = l + ls[mu];
+                gp = gsite + cg.stride[mu];
+            } else if (e[mu] == cg.L[mu]) {  // the box spans the axis: the wrap bond is internal
+                lp = l - (e[mu] - 1) * ls[mu];
+                gp = gsite - (int64_t)(cg.L[mu] - 1) * cg.stride[mu];
+            } else {
+                continue;  // leaves the box: cc_boundary draws it
+            }
+            if (labels[gp] != lab) continue;
+            uint32_t bond = (uint32_t)(gsite * D + mu);
+            if (keyed_bits_k(bond_seed, bond, skey) < bond_t) lds_union(P, l, lp);
+        }
+    }
+    __syncthreads();
+    for (int32_t l = threadIdx.x; l < nloc; l += blockDim.x) {
+        int32_t root = lds_find(P, l);
+        int32_t y[MAX_D];
+        int32_t r = l;
+        int64_t gsite = 0, groot = 0;
+        int32_t rr = root;
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) {
+            y[mu] = r % e[mu];
+            r /= e[mu];
+            gsite += (int64_t)(o[mu] + y[mu]) * cg.stride[mu];
+            groot += (int64_t)(o[mu] + rr % e[mu]) * cg.stride[mu];
+            rr /= e[mu];
+        }
+        parent[gsite] = (int32_t)groot;
+    }
+}
+
+// ------------------------------------------------------------- boundary ---
+
+__device__ __forceinline__ int32_t g_find(int32_t *P, int32_t x, uint32_t cap, uint32_t &hops) {
+    int32_t p = load_relaxed(&P[x]);
+    while (p != x) {
+        if (++hops > cap) return x;
+        int32_t gp = load_relaxed(&P[p]);
+        if (gp != p) store_relaxed(&P[x], gp);
+        x = p;
+        p = load_relaxed(&P[x]);
+    }
+    return x;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void cc_boundary(const uint8_t *__restrict__ labels, int32_t *__restrict__ parent,
+                                                   CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t,
+                                                   uint32_t cap, int *__restrict__ flag) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // Face layers: along axis mu, the last coordinate of each box (only when
+    // the box does not span the axis), each layer holding N / L[mu] sites.
+    int mu = 0;
+    int64_t layer_sites = 0, count = 0;
+    for (; mu < D; ++mu) {
+        if (cg.B[mu] >= cg.L[mu]) continue;
+        layer_sites = cg.N / cg.L[mu];
+        count = (int64_t)cg.nbox[mu] * layer_sites;
+        if (t < count) break;
+        t -= count;
+    }
+    if (mu >= D) return;
+    int32_t k = (int32_t)(t / layer_sites);
+    int64_t rem = t - (int64_t)k * layer_sites;  // index over the other axes
+    int32_t xm = (k + 1) * cg.B[mu];
+    if (xm > cg.L[mu]) xm = cg.L[mu];
+    xm -= 1;
+    // rebuild the site: rem enumerates the other axes in lattice order
+    int64_t gsite = 0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        if (a == mu) {
+            gsite += (int64_t)xm * cg.stride[a];
+        } else {
+            int32_t ya = (int32_t)(rem % cg.L[a]);
+            rem /= cg.L[a];
+            gsite += (int64_t)ya * cg.stride[a];
+        }
+    }
+    int64_t gp = (xm + 1 == cg.L[mu]) ? gsite - (int64_t)xm * cg.stride[mu] : gsite + cg.stride[mu];
+    if (labels[gsite] != labels[gp]) return;
+    uint32_t bond = (uint32_t)(gsite * D + mu);
+    if (keyed_bits_k(bond_seed, bond, skey) >= bond_t) return;
+    uint32_t hops = 0;
+    int32_t a = (int32_t)gsite, b = (int32_t)gp;
+    for (;;) {
+        a = g_find(parent, a, cap, hops);
+        b = g_find(parent, b, cap, hops);
+        if (hops > cap) {
+            atomicOr(flag, 1);
+            return;
+        }
+        if (a == b) return;
+        int32_t hi = a > b ? a : b, lo = a > b ? b : a;
+        int32_t old = atomicCAS(&parent[hi], hi, lo);
+        if (old == hi) return;
+        if (++hops > cap) {
+            atomicOr(flag, 1);
+            return;
+        }
+        a = old;
+        b = lo;
+    }
+}
+
+// -------------------------------------------------------------- relabel ---
+
+__device__ __forceinline__ long long blk_sum(long long v, long long *lds) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    long long t = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += lds[w];
+    return t;
+}
+
+__global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, int32_t *__restrict__ parent,
+                                                  int64_t N, uint32_t root_seed, uint32_t seed_seed, uint32_t skey,
+                                                  uint32_t sweep, int q, int seeded, int forced, uint32_t cap,
+                                                  int *__restrict__ flag, long long *__restrict__ partials,
+                                                  int64_t span) {
+    __shared__ long long lds[16];
+    __shared__ int32_t seed_root;
+    if (seeded) {
+        if (threadIdx.x == 0) {
+            // cluster_relabel.wgsl:59-62: the seed site from the keyed draw,
+            // scaled in f32 and clamped.
+            float u = keyed_uniform_f32(keyed_bits(seed_seed, 0u, sweep));
+            uint32_t ss = (uint32_t)(u * (float)(uint32_t)N);
+            if (ss >= (uint32_t)N) ss = (uint32_t)N - 1u;
+            uint32_t hops = 0;
+            int32_t r = g_find(parent, (int32_t)ss, cap, hops);
+            if (hops > cap) atomicOr(flag, 1);
+            seed_root = r;
+        }
+        __syncthreads();
+    }
+    long long moved = 0;
+    int counts[MAX_Q_DEV];
+    for (int a = 0; a < MAX_Q_DEV; ++a) counts[a] = 0;
+    int64_t begin = (int64_t)blockIdx.x * span;
+    int64_t end = begin + span;
+    if (end > N) end = N;
+    for (int64_t s = begin + threadIdx.x; s < end; s += blockDim.x) {
+        uint32_t hops = 0;
+        int32_t root = g_find(parent, (int32_t)s, cap, hops);
+        if (hops > cap) atomicOr(flag, 1);
+        uint32_t lab = labels[s];
+        if (!seeded || root == seed_root) {
+            float u = keyed_uniform_f32(keyed_bits_k(root_seed, (uint32_t)root, skey));
+            if (forced) {
+                uint32_t step = (uint32_t)(u * (float)(uint32_t)(q - 1));
+                if (step >= (uint32_t)(q - 1)) step = (uint32_t)(q - 2);
+                lab = (lab + 1u + step) % (uint32_t)q;
+            } else {
+                uint32_t draw = (uint32_t)(u * (float)(uint32_t)q);
+                if (draw >= (uint32_t)q) draw = (uint32_t)(q - 1);
+                lab = draw;
+            }
+            labels[s] = (uint8_t)lab;
+            moved += 1;
+        }
+        counts[lab] += 1;
+    }
+    long long *dst = partials + (int64_t)blockIdx.x * REC_K;
+    long long t = blk_sum(moved, lds);
+    if (threadIdx.x == 0) {
+        dst[0] = 0;
+        dst[1] = t;
+    }
+    for (int a = 0; a < MAX_Q_DEV; ++a) {
+        t = blk_sum((long long)(a < q ? counts[a] : 0), lds);
+        if (threadIdx.x == 0) dst[2 + a] = t;
+    }
+}
+
+// --------------------------------------------------------- measurement ---
+
+// Potts energy sums of a lattice-ordered label field: agreeing forward bonds
+// (potts/mod.rs:237-271) and label counts (potts/mod.rs:124-139).
+template <int D>
+__global__ __launch_bounds__(256) void potts_measure_kernel(const uint8_t *__restrict__ labels, CCGeom cg, int q,
+                                                            long long *__restrict__ partials, int64_t span) {
+    __shared__ long long lds[16];
+    long long agree = 0;
+    int counts[MAX_Q_DEV];
+    for (int a = 0; a < MAX_Q_DEV; ++a) counts[a] = 0;
+    int64_t begin = (int64_t)blockIdx.x * span;
+    int64_t end = begin + span;
+    if (end > cg.N) end = cg.N;
+    for (int64_t s = begin + threadIdx.x; s < end; s += blockDim.x) {
+        uint8_t lab = labels[s];
+        counts[lab] += 1;
+        int64_t rest = s;
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) {
+            int32_t x = (int32_t)(rest % cg.L[mu]);
+            rest /= cg.L[mu];
+            int64_t gp = (x + 1 == cg.L[mu]) ? s - (int64_t)x * cg.stride[mu] : s + cg.stride[mu];
+            agree += labels[gp] == lab;
+        }
+    }
+    long long *dst = partials + (int64_t)blockIdx.x * REC_K;
+    long long t = blk_sum(agree, lds);
+    if (threadIdx.x == 0) {
+        dst[0] = t;
+        dst[1] = 0;
+    }
+    for (int a = 0; a < MAX_Q_DEV; ++a) {
+        t = blk_sum((long long)(a < q ? counts[a] : 0), lds);
+        if (threadIdx.x == 0) dst[2 + a] = t;
+    }
+}
+
+// ---------------------------------------------------- roots and sizes ---
+
+__global__ void cc_resolve(int32_t *__restrict__ parent, int32_t *__restrict__ roots, int64_t N, uint32_t cap,
+                           int *__restrict__ flag) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N) return;
+    uint32_t hops = 0;
+    int32_t r = g_find(parent, (int32_t)s, cap, hops);
+    if (hops > cap) atomicOr(flag, 1);
+    roots[s] = r;
+}
+
+__global__ void cc_count(const int32_t *__restrict__ roots, int32_t *__restrict__ sizes, int64_t N) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N) return;
+    atomicAdd(&sizes[roots[s]], 1);
+}
+
+// hist[0..39] clusters by floor(log2 size); hist[40] clusters; hist[41] largest
+__global__ void cc_hist(const int32_t *__restrict__ roots, const int32_t *__restrict__ sizes, int64_t N,
+                        unsigned long long *__restrict__ hist) {
+    __shared__ unsigned int h[40];
+    __shared__ unsigned int nclu, largest;
+    if (threadIdx.x < 40) h[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        nclu = 0;
+        largest = 0;
+    }
+    __syncthreads();
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < N && roots[s] == (int32_t)s) {
+        uint32_t sz = (uint32_t)sizes[s];
+        atomicAdd(&h[31 - __clz(sz)], 1u);
+        atomicAdd(&nclu, 1u);
+        atomicMax(&largest, sz);
+    }
+    __syncthreads();
+    if (threadIdx.x < 40 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+    if (threadIdx.x == 0) {
+        atomicAdd(&hist[40], (unsigned long long)nclu);
+        atomicMax(&hist[41], (unsigned long long)largest);
+    }
+}
+
+// ------------------------------------------------------------ launchers ---
+
+// Boxes: start from the whole lattice and halve the longest box edge until a
+// box fits the LDS budget; a 2D 4096^2 lattice gets 128 x 128 boxes, so only
+// 2 x 128 of each box's 2 x 16384 forward bonds leave it.
+CCGeom make_ccgeom(int D, const int64_t *shape, int32_t max_box_sites) {
+    CCGeom cg{};
+    cg.D = D;
+    cg.N = 1;
+    for (int mu = 0; mu < MAX_D; ++mu) {
+        cg.L[mu] = mu < D ? (int32_t)shape[mu] : 1;
+        cg.stride[mu] = cg.N;
+        if (mu < D) cg.N *= shape[mu];
+        cg.B[mu] = cg.L[mu];
+    }
+    for (;;) {
+        int64_t vol = 1;
+        int big = 0;
+        for (int mu = 0; mu < D; ++mu) {
+            vol *= cg.B[mu];
+            if (cg.B[mu] > cg.B[big]) big = mu;
+        }
+        if (vol <= max_box_sites) break;
+        cg.B[big] = (cg.B[big] + 1) / 2;
+    }
+    cg.nboxes = 1;
+    cg.box_sites = 1;
+    for (int mu = 0; mu < MAX_D; ++mu) {
+        cg.nbox[mu] = (cg.L[mu] + cg.B[mu] - 1) / cg.B[mu];
+        cg.nboxes *= cg.nbox[mu];
+        cg.box_sites *= cg.B[mu];
+    }
+    return cg;
+}
+
+static int64_t face_sites(const CCGeom &cg) {
+    int64_t total = 0;
+    for (int mu = 0; mu < cg.D; ++mu)
+        if (cg.B[mu] < cg.L[mu]) total += (int64_t)cg.nbox[mu] * (cg.N / cg.L[mu]);
+    return total;
+}
+
+int64_t launch_cluster_sweep(const ClusterSweepArgs &a) {
+    const CCGeom &cg = a.cg;
+    uint32_t skey = a.sweep * GOLDEN;
+    uint32_t bond_seed = a.seed32 ^ BOND_KEY;
+    size_t lds = (size_t)cg.box_sites * sizeof(int32_t);
+    dim3 gl((uint32_t)cg.nboxes), bl(1024);
+    int64_t faces = face_sites(cg);
+    dim3 gb((uint32_t)((faces + 255) / 256)), bb(256);
+#define PLQ_CC(DD)                                                                                          \
+    cc_local<DD><<<gl, bl, lds, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t);            \
+    if (faces > 0)                                                                                          \
+        cc_boundary<DD><<<gb, bb, 0, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t, a.cap, \
+                                                 a.flag)
+    switch (cg.D) {
+        case 1: PLQ_CC(1); break;
+        case 2: PLQ_CC(2); break;
+        case 3: PLQ_CC(3); break;
+        default: PLQ_CC(4); break;
+    }
+#undef PLQ_CC
+    int64_t need = (cg.N + 255) / 256;
+    int64_t G = need < a.max_blocks ? need : a.max_blocks;
+    int64_t span = (cg.N + G - 1) / G;
+    cc_relabel<<<dim3((uint32_t)G), dim3(256), 0, a.stream>>>(
+        a.labels, a.parent, cg.N, a.seed32 ^ ROOT_SALT, a.seed32 ^ SEED_SALT, skey, a.sweep, a.q, a.seeded,
+        a.forced, a.cap, a.flag, a.partials, span);
+    return G;
+}
+
+void launch_potts_measure(const uint8_t *labels, const CCGeom &cg, int q, long long *partials, int64_t max_blocks,
+                          long long *rec, hipStream_t s) {
+    int64_t need = (cg.N + 255) / 256;
+    int64_t G = need < max_blocks ? need : max_blocks;
+    int64_t span = (cg.N + G - 1) / G;
+    dim3 g((uint32_t)G), b(256);
+    switch (cg.D) {
+        case 1: potts_measure_kernel<1><<<g, b, 0, s>>>(labels, cg, q, partials, span); break;
+        case 2: potts_measure_kernel<2><<<g, b, 0, s>>>(labels, cg, q, partials, span); break;
+        case 3: potts_measure_kernel<3><<<g, b, 0, s>>>(labels, cg, q, partials, span); break;
+        default: potts_measure_kernel<4><<<g, b, 0, s>>>(labels, cg, q, partials, span); break;
+    }
+    launch_finalize_record(partials, G, nullptr, 0, rec, s);
+}
+
+void launch_cluster_resolve(int32_t *parent, int32_t *roots, const CCGeom &cg, uint32_t cap, int *flag,
+                            int32_t *sizes, unsigned long long *hist, hipStream_t s) {
+    dim3 g((uint32_t)((cg.N + 255) / 256)), b(256);
+    cc_resolve<<<g, b, 0, s>>>(parent, roots, cg.N, cap, flag);
+    if (sizes && hist) {
+        cc_count<<<g, b, 0, s>>>(roots, sizes, cg.N);
+        cc_hist<<<g, b, 0, s>>>(roots, sizes, cg.N, hist);
+    }
+}
+
+}  // namespace plq

@@ -1,0 +1,98 @@
+// plq_common.hpp -- pieces shared by the host library and the gfx950 kernels:
+// the keyed counter-based stream, the colour-interleaved site layout, and the
+// launch-independent constants.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace plq {
+
+// Stream salts: metropolis.wgsl:30, cluster_prelude.wgsl:29,
+// cluster_relabel.wgsl:33 and :37.
+constexpr uint32_t PROPOSAL_KEY = 0x85ebca6bu;
+constexpr uint32_t BOND_KEY = 0x27d4eb2fu;
+constexpr uint32_t ROOT_SALT = 0x85ebca6bu;
+constexpr uint32_t SEED_SALT = 0xc2b2ae35u;
+constexpr uint32_t GOLDEN = 0x9e3779b9u;
+constexpr uint32_t DRAW_ONE = 1u << 24;  // 2^24: every 24-bit draw is below it
+
+// lowbias32 (rng.wgsl:34-42): two xor-shift-multiply rounds on u32.
+__host__ __device__ __forceinline__ uint32_t lowbias32(uint32_t v) {
+    v ^= v >> 16;
+    v *= 0x7feb352du;
+    v ^= v >> 15;
+    v *= 0x846ca68bu;
+    v ^= v >> 16;
+    return v;
+}
+
+// The 24-bit integer behind keyed_uniform (rng.wgsl:47-52):
+// keyed_uniform = m * 2^-24 exactly, so every comparison of it against a
+// threshold is an integer comparison of m.
+__host__ __device__ __forceinline__ uint32_t keyed_bits(uint32_t seed, uint32_t var, uint32_t sweep) {
+    uint32_t k = lowbias32(seed ^ var);
+    k = lowbias32(k ^ (sweep * GOLDEN));
+    return k >> 8;
+}
+
+// The same with the sweep term hoisted: sweep_key = sweep * GOLDEN.
+__host__ __device__ __forceinline__ uint32_t keyed_bits_k(uint32_t seed, uint32_t var, uint32_t sweep_key) {
+    uint32_t k = lowbias32(seed ^ var);
+    k = lowbias32(k ^ sweep_key);
+    return k >> 8;
+}
+
+// f32 uniform for the draws the reference scales in f32 (proposal, relabel,
+// Wolff seed): m * 2^-24, exact.
+__host__ __device__ __forceinline__ float keyed_uniform_f32(uint32_t m) {
+    return (float)m * (1.0f / 16777216.0f);
+}
+
+// fold_seed (device.rs:761-763).
+__host__ __device__ __forceinline__ uint32_t fold_seed(uint64_t seed) {
+    return (uint32_t)(seed ^ (seed >> 32));
+}
+
+constexpr int MAX_D = 4;
+
+// Colour-interleaved site layout.  A site (x0, x1, ..., x_{D-1}) lives in
+// row r = x1 + L1*(x2 + L2*x3) at position x0.  Its colour is
+// c = (x0 + x1 + ... ) & 1, and it is stored in colour array c at
+// r * H + (x0 >> 1), H = L0 / 2.  Every row of a colour array holds the H
+// sites of that colour in increasing x0, so a lane reading four consecutive
+// bytes reads four same-colour sites two lattice steps apart.
+struct Geom {
+    int32_t D;
+    int32_t L[MAX_D];      // extents; unused axes are 1
+    int32_t H;             // L[0] / 2
+    int32_t R;             // rows = N / L[0]
+    int32_t rstride[MAX_D];// row stride of axis mu >= 1 (rstride[1] = 1)
+    int64_t N;             // sites
+};
+
+// Decompose a row index into the coordinates of axes 1..D-1.
+__host__ __device__ __forceinline__ void row_coords(const Geom &g, int32_t r, int32_t *x) {
+    int32_t rest = r;
+    for (int mu = 1; mu < MAX_D; ++mu) {
+        if (mu < g.D) {
+            x[mu] = rest % g.L[mu];
+            rest /= g.L[mu];
+        } else {
+            x[mu] = 0;
+        }
+    }
+}
+
+// The row reached from row r (with coordinates x) by one step along axis
+// mu >= 1, sign +1 or -1, under periodic wrap.
+__host__ __device__ __forceinline__ int32_t row_step(const Geom &g, int32_t r, const int32_t *x, int mu, int sign) {
+    int32_t l = g.L[mu];
+    int32_t xn = x[mu] + sign;
+    int32_t delta = sign;
+    if (xn == l) delta = 1 - l;
+    if (xn < 0) delta = l - 1;
+    return r + delta * g.rstride[mu];
+}
+
+}  // namespace plq

@@ -1,0 +1,112 @@
+// plq_internal.hpp -- declarations shared between the host library and the
+// kernel translation units.  Nothing here is part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "plq_common.hpp"
+
+namespace plq {
+
+constexpr int MAX_Q_DEV = 16;
+// One measurement record on the device: [bond_sum, unused, counts[16]].
+constexpr int REC_K = 18;
+constexpr int MAX_PASS_BLOCKS = 2048;
+
+// Log2 form of a power-of-two geometry for the SWAR Ising pass.
+struct SwarGeom {
+    uint32_t lh4;   // log2(L0 / 8): dwords per colour row
+    uint32_t L1, L2, L3;
+    uint32_t ll1, ll2;
+};
+
+struct SitePassArgs {
+    const uint8_t *oth;  // the other colour's array (read)
+    uint8_t *own;        // this colour's array (updated)
+    Geom g;
+    SwarGeom sg;
+    bool swar;
+    int model;           // 0 Ising, 1 Potts
+    int mode;            // 0 Metropolis, 1 heat bath, 2 measure only
+    int color;
+    uint32_t seed;
+    uint32_t skey;       // sweep * 0x9e3779b9
+    int q;
+    const uint32_t *table;
+    long long *partials; // REC_K per block, written when meas
+    int meas;
+    int64_t max_blocks;
+    hipStream_t stream;
+};
+
+void launch_pack_sites(const uint8_t *lat, uint8_t *c0, uint8_t *c1, const Geom &g, hipStream_t s);
+void launch_unpack_sites(uint8_t *lat, const uint8_t *c0, const uint8_t *c1, const Geom &g, hipStream_t s);
+void launch_site_pass(const SitePassArgs &a);
+int site_pass_blocks(const SitePassArgs &a);
+void launch_finalize_record(const long long *pa, int64_t na, const long long *pb, int64_t nb, long long *rec,
+                            hipStream_t s);
+
+// ------------------------------------------------------------------ links ---
+
+struct LinkPassArgs {
+    uint8_t *const *arr;  // 2*D arrays, index dir*2 + colour (device pointers, in a device array)
+    Geom g;
+    int dir;
+    int color;
+    int mode;             // 0 Metropolis, 1 heat bath
+    uint32_t seed;
+    uint32_t skey;
+    const uint32_t *table;
+    hipStream_t stream;
+};
+void launch_pack_links(const uint8_t *lat, uint8_t *const *arr, const Geom &g, hipStream_t s);
+void launch_unpack_links(uint8_t *lat, uint8_t *const *arr, const Geom &g, hipStream_t s);
+void launch_link_pass(const LinkPassArgs &a);
+// Plaquette sum into rec[0] (partials reduced on the device).
+void launch_plaquette_sum(uint8_t *const *arr, const Geom &g, long long *partials, int64_t max_blocks,
+                          long long *rec, hipStream_t s);
+
+// ---------------------------------------------------------------- cluster ---
+
+// Plain lattice-order geometry cut into boxes for the LDS union-find.
+struct CCGeom {
+    int32_t D;
+    int32_t L[MAX_D];
+    int64_t stride[MAX_D];
+    int32_t B[MAX_D];     // box extents
+    int32_t nbox[MAX_D];  // boxes per axis
+    int64_t N;
+    int64_t nboxes;
+    int32_t box_sites;    // max sites in one box (LDS entries)
+};
+CCGeom make_ccgeom(int D, const int64_t *shape, int32_t max_box_sites);
+
+struct ClusterSweepArgs {
+    uint8_t *labels;      // lattice order
+    int32_t *parent;      // union-find forest, lattice order
+    CCGeom cg;
+    uint32_t seed32;
+    uint32_t sweep;
+    uint32_t bond_t;      // open iff agree && m < bond_t
+    int q;
+    int seeded;
+    int forced;
+    uint32_t cap;         // hop budget of every find / union
+    int *flag;            // set to 1 when a budget is exceeded
+    long long *partials;  // REC_K per relabel block: [0, moved, counts...]
+    int64_t max_blocks;
+    hipStream_t stream;
+};
+// Enqueue one cluster sweep; returns the number of relabel blocks whose
+// partials were written.
+int64_t launch_cluster_sweep(const ClusterSweepArgs &a);
+void launch_potts_measure(const uint8_t *labels, const CCGeom &cg, int q, long long *partials,
+                          int64_t max_blocks, long long *rec, hipStream_t s);
+// Canonical roots of the current forest, and (when sizes/hist are non-null)
+// cluster sizes and the size histogram (hist[0..39] by floor(log2 size),
+// hist[40] cluster count, hist[41] largest size).
+void launch_cluster_resolve(int32_t *parent, int32_t *roots, const CCGeom &cg, uint32_t cap, int *flag,
+                            int32_t *sizes, unsigned long long *hist, hipStream_t s);
+
+}  // namespace plq

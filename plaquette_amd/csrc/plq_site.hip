@@ -1,0 +1,420 @@
+// plq_site.hip -- checkerboard kernels for site fields (Ising, Potts) on gfx950.
+//
+// One launch is one colour pass of the reference's checkerboard
+// (site_schedule.wgsl:15-31, updater.rs:316-324): every site of colour c is
+// updated from its 2D neighbours, which all carry the other colour and are
+// not written during the pass.  Unlike the reference, a launch covers only the
+// sites of its colour: the two colours live in separate int8 arrays
+// (plq_common.hpp, Geom), so a lane never loads a site it will not update and
+// every neighbour index is computed from coordinates in registers instead of
+// read from the 2D-entry u32 neighbour table (device.rs:741-747).
+//
+// Decisions: the draw m = keyed_bits(seed32, site, sweep) (rng.wgsl:47-52) is
+// compared with an integer threshold of the site's class, built on the host
+// from the reference arithmetic (plq_thresholds.cpp).  The draw is computed
+// only when the threshold leaves the outcome open (0 < T < 2^24), which
+// matches the reference's "downhill takes no draw" rule without changing any
+// result, since keyed draws have no stream position.
+//
+// Observables are fused: on a recording sweep the colour-0 pass counts the
+// new colour-0 states, and the colour-1 pass counts the colour-1 states and
+// sums each colour-1 site's bond term against its (final) colour-0
+// neighbours.  Every bond has exactly one colour-1 end on an even lattice, so
+// that sum is the full bond sum of ising/mod.rs:97-117 and potts/mod.rs:237-271.
+#include <hip/hip_runtime.h>
+
+#include "plq_common.hpp"
+#include "plq_internal.hpp"
+
+namespace plq {
+
+// ---------------------------------------------------------------- layout ---
+
+__global__ void pack_sites_kernel(const uint8_t *__restrict__ lat, uint8_t *__restrict__ c0,
+                                  uint8_t *__restrict__ c1, Geom g) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.N) return;
+    int32_t r = (int32_t)(s / g.L[0]);
+    int32_t x0 = (int32_t)(s - (int64_t)r * g.L[0]);
+    int32_t x[MAX_D];
+    row_coords(g, r, x);
+    int32_t par = x0;
+    for (int mu = 1; mu < MAX_D; ++mu) par += x[mu];
+    int64_t idx = (int64_t)r * g.H + (x0 >> 1);
+    if (par & 1) c1[idx] = lat[s];
+    else c0[idx] = lat[s];
+}
+
+__global__ void unpack_sites_kernel(uint8_t *__restrict__ lat, const uint8_t *__restrict__ c0,
+                                    const uint8_t *__restrict__ c1, Geom g) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.N) return;
+    int32_t r = (int32_t)(s / g.L[0]);
+    int32_t x0 = (int32_t)(s - (int64_t)r * g.L[0]);
+    int32_t x[MAX_D];
+    row_coords(g, r, x);
+    int32_t par = x0;
+    for (int mu = 1; mu < MAX_D; ++mu) par += x[mu];
+    int64_t idx = (int64_t)r * g.H + (x0 >> 1);
+    lat[s] = (par & 1) ? c1[idx] : c0[idx];
+}
+
+// -------------------------------------------------------------- reduction ---
+
+// Sum `v` over the block and let thread 0 return it.  Block of up to 1024.
+__device__ __forceinline__ long long block_sum(long long v, long long *lds) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    long long t = 0;
+    if (threadIdx.x == 0) {
+        int nw = (blockDim.x + 63) >> 6;
+        for (int w = 0; w < nw; ++w) t += lds[w];
+    }
+    return t;
+}
+
+// Logical block index: hardware dispatches consecutive blocks round-robin over
+// the 8 XCDs; remapping gives each XCD a contiguous 1/8 of the rows so the
+// neighbour rows a block reads sit in the same XCD's L2.
+__device__ __forceinline__ int64_t logical_block() {
+    uint32_t b = blockIdx.x, G = gridDim.x;
+    if ((G & 7u) != 0) return b;
+    return (int64_t)(b & 7u) * (G >> 3) + (b >> 3);
+}
+
+// Write a record's partial sums: slot layout [bond, 0, counts[0..15]].
+__device__ __forceinline__ void write_partials(long long *partials, long long bond, const int *counts, int q,
+                                               long long *lds) {
+    long long *dst = partials + (int64_t)blockIdx.x * REC_K;
+    long long t = block_sum(bond, lds);
+    if (threadIdx.x == 0) dst[0] = t;
+    for (int a = 0; a < q; ++a) {
+        t = block_sum((long long)counts[a], lds);
+        if (threadIdx.x == 0) dst[2 + a] = t;
+    }
+}
+
+// Sum two runs of block partials (one per colour pass) into one record.
+__global__ void finalize_record_kernel(const long long *__restrict__ pa, int64_t na, const long long *__restrict__ pb,
+                                       int64_t nb, long long *__restrict__ rec) {
+    __shared__ long long lds[16];
+    for (int k = 0; k < REC_K; ++k) {
+        long long v = 0;
+        for (int64_t b = threadIdx.x; b < na; b += blockDim.x) v += pa[b * REC_K + k];
+        for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) v += pb[b * REC_K + k];
+        long long t = block_sum(v, lds);
+        if (threadIdx.x == 0) rec[k] = t;
+    }
+}
+
+// ------------------------------------------------------- generic site pass ---
+
+// One site per lane, any D <= 4, any even extents, Ising or Potts.
+// MODE: 0 Metropolis, 1 heat bath, 2 measure only (no update).
+template <int D, int MODEL, int MODE>
+__global__ __launch_bounds__(256) void site_pass_generic(const uint8_t *__restrict__ oth, uint8_t *__restrict__ own,
+                                                         Geom g, int c, uint32_t seed, uint32_t skey, int q,
+                                                         const uint32_t *__restrict__ table,
+                                                         long long *__restrict__ partials, int meas,
+                                                         int64_t work, int64_t span) {
+    __shared__ long long lds[16];
+    constexpr int NN = 2 * D;
+    const int64_t lb = logical_block();
+    const int64_t begin = lb * span;
+    int64_t end = begin + span;
+    if (end > work) end = work;
+    long long bond = 0;
+    int counts[MAX_Q_DEV];
+    for (int a = 0; a < MAX_Q_DEV; ++a) counts[a] = 0;
+    const int L0 = g.L[0];
+    for (int64_t w = begin + threadIdx.x; w < end; w += blockDim.x) {
+        int32_t r = (int32_t)(w / g.H);
+        int32_t i = (int32_t)(w - (int64_t)r * g.H);
+        int32_t x[MAX_D];
+        row_coords(g, r, x);
+        int32_t prs = 0;
+        for (int mu = 1; mu < D; ++mu) prs += x[mu];
+        int32_t p = (prs + c) & 1;
+        int32_t x0 = 2 * i + p;
+        uint32_t site = (uint32_t)((int64_t)r * L0 + x0);
+        int64_t rowbase = (int64_t)r * g.H;
+        uint8_t nb[NN];
+        {
+            int32_t xp = (x0 + 1 == L0) ? 0 : x0 + 1;
+            int32_t xm = (x0 == 0) ? L0 - 1 : x0 - 1;
+            nb[0] = oth[rowbase + (xp >> 1)];
+            nb[1] = oth[rowbase + (xm >> 1)];
+#pragma unroll
+            for (int mu = 1; mu < D; ++mu) {
+                nb[2 * mu] = oth[(int64_t)row_step(g, r, x, mu, +1) * g.H + i];
+                nb[2 * mu + 1] = oth[(int64_t)row_step(g, r, x, mu, -1) * g.H + i];
+            }
+        }
+        int cur = own[rowbase + i];
+        int nxt = cur;
+        if (MODEL == 0) {  // Ising: class (cur, d)
+            int d = 0;
+#pragma unroll
+            for (int k = 0; k < NN; ++k) d += nb[k];
+            if (MODE != 2) {
+                uint32_t t = table[cur * (NN + 1) + d];
+                bool hit;
+                if (t == 0u) hit = false;
+                else if (t >= DRAW_ONE) hit = true;
+                else hit = keyed_bits_k(seed, site, skey) < t;
+                if (MODE == 0) nxt = hit ? 1 - cur : cur;
+                else nxt = hit ? 0 : 1;
+                own[rowbase + i] = (uint8_t)nxt;
+            }
+            if (meas) {
+                counts[nxt] += 1;
+                if (c == 1) bond += (long long)((1 - 2 * nxt) * (NN - 2 * d));
+            }
+        } else {  // Potts
+            int n[MAX_Q_DEV];
+            for (int a = 0; a < q; ++a) n[a] = 0;
+#pragma unroll
+            for (int k = 0; k < NN; ++k) n[nb[k]] += 1;
+            if (MODE == 0) {
+                int prop;
+                if (q == 2) {
+                    prop = 1 - cur;
+                } else {
+                    uint32_t alternatives = (uint32_t)(q - 1);
+                    float u = keyed_uniform_f32(keyed_bits_k(seed ^ PROPOSAL_KEY, site, skey));
+                    uint32_t dr = (uint32_t)(u * (float)alternatives);
+                    if (dr >= alternatives) dr = alternatives - 1;
+                    prop = (int)dr + ((int)dr >= cur ? 1 : 0);
+                }
+                int change = n[prop] - n[cur];
+                uint32_t t = table[((uint32_t)cur * q + prop) * (2 * NN + 1) + (change + NN)];
+                bool hit;
+                if (t == 0u) hit = false;
+                else if (t >= DRAW_ONE) hit = true;
+                else hit = keyed_bits_k(seed, site, skey) < t;
+                nxt = hit ? prop : cur;
+                own[rowbase + i] = (uint8_t)nxt;
+            } else if (MODE == 1) {
+                uint32_t enc = 0, base = 1;
+                for (int a = 0; a < q - 1; ++a) {
+                    enc += (uint32_t)n[a] * base;
+                    base *= (uint32_t)(NN + 1);
+                }
+                uint32_t cls = (uint32_t)cur * base + enc;
+                const uint32_t *T = table + (int64_t)cls * (q - 1);
+                uint32_t m = keyed_bits_k(seed, site, skey);
+                nxt = q - 1;
+                for (int s = 0; s < q - 1; ++s)
+                    if (m < T[s]) {
+                        nxt = s;
+                        break;
+                    }
+                own[rowbase + i] = (uint8_t)nxt;
+            }
+            if (meas) {
+                counts[nxt] += 1;
+                if (c == 1) bond += (long long)n[nxt];
+            }
+        }
+    }
+    if (meas) write_partials(partials, bond, counts, q, lds);
+}
+
+// ------------------------------------------------------ SWAR Ising pass ---
+
+// Four same-colour sites per lane (one dword of a colour row).  Requires
+// power-of-two extents with L0 >= 8.  State bytes are 0/1, so the per-byte
+// count of "down" neighbours is the plain dword sum of the 2D neighbour words
+// (at most 8 per byte, no carries between bytes).
+template <int D, int MODE>
+__global__ __launch_bounds__(256) void ising_pass_swar(const uint32_t *__restrict__ oth, uint32_t *__restrict__ own,
+                                                       SwarGeom sg, int c, uint32_t seed, uint32_t skey,
+                                                       const uint32_t *__restrict__ table,
+                                                       long long *__restrict__ partials, int meas, int64_t work,
+                                                       int64_t span) {
+    constexpr int NN = 2 * D;
+    constexpr int K = NN + 1;
+    __shared__ uint32_t T[2 * K];
+    __shared__ long long lds[16];
+    if (threadIdx.x < 2 * K) T[threadIdx.x] = table[threadIdx.x];
+    __syncthreads();
+    const int64_t lb = logical_block();
+    const int64_t begin = lb * span;
+    int64_t end = begin + span;
+    if (end > work) end = work;
+    long long bond = 0;
+    int down = 0, total = 0;
+    const uint32_t h4mask = (1u << sg.lh4) - 1u;
+    for (int64_t w = begin + threadIdx.x; w < end; w += blockDim.x) {
+        uint32_t r = (uint32_t)(w >> sg.lh4);
+        uint32_t i4 = (uint32_t)w & h4mask;
+        uint32_t x1 = r & (sg.L1 - 1u);
+        uint32_t x2 = 0, x3 = 0;
+        if (D >= 3) x2 = (r >> sg.ll1) & (sg.L2 - 1u);
+        if (D >= 4) x3 = r >> (sg.ll1 + sg.ll2);
+        uint32_t prs = x1 + x2 + x3;
+        uint32_t p = (prs + (uint32_t)c) & 1u;
+        const uint32_t rb = r << sg.lh4;
+        uint32_t a0 = oth[rb + i4];
+        uint32_t sh;
+        if (p) {
+            uint32_t an = oth[rb + ((i4 + 1u) & h4mask)];
+            sh = (a0 >> 8) | (an << 24);
+        } else {
+            uint32_t ap = oth[rb + ((i4 - 1u) & h4mask)];
+            sh = (a0 << 8) | (ap >> 24);
+        }
+        uint32_t dsum = a0 + sh;
+        if (D >= 2) {
+            uint32_t rp = (x1 + 1u == sg.L1) ? r - (sg.L1 - 1u) : r + 1u;
+            uint32_t rm = (x1 == 0u) ? r + (sg.L1 - 1u) : r - 1u;
+            dsum += oth[(rp << sg.lh4) + i4] + oth[(rm << sg.lh4) + i4];
+        }
+        if (D >= 3) {
+            uint32_t st = sg.L1;
+            uint32_t span2 = (sg.L2 - 1u) * st;
+            uint32_t rp = (x2 + 1u == sg.L2) ? r - span2 : r + st;
+            uint32_t rm = (x2 == 0u) ? r + span2 : r - st;
+            dsum += oth[(rp << sg.lh4) + i4] + oth[(rm << sg.lh4) + i4];
+        }
+        if (D >= 4) {
+            uint32_t st = sg.L1 * sg.L2;
+            uint32_t span3 = (sg.L3 - 1u) * st;
+            uint32_t rp = (x3 + 1u == sg.L3) ? r - span3 : r + st;
+            uint32_t rm = (x3 == 0u) ? r + span3 : r - st;
+            dsum += oth[(rp << sg.lh4) + i4] + oth[(rm << sg.lh4) + i4];
+        }
+        uint32_t s4 = own[rb + i4];
+        uint32_t out = s4;
+        if (MODE != 2) {
+            out = 0;
+            const uint32_t site0 = (r << (sg.lh4 + 3)) + (i4 << 3) + p;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                uint32_t s = (s4 >> (8 * k)) & 1u;
+                uint32_t d = (dsum >> (8 * k)) & 0xffu;
+                uint32_t t = T[s * K + d];
+                bool hit;
+                if (t == 0u) hit = false;
+                else if (t >= DRAW_ONE) hit = true;
+                else hit = keyed_bits_k(seed, site0 + 2u * k, skey) < t;
+                uint32_t nxt = (MODE == 0) ? (s ^ (hit ? 1u : 0u)) : (hit ? 0u : 1u);
+                out |= nxt << (8 * k);
+            }
+            own[rb + i4] = out;
+        }
+        if (meas) {
+            down += __popc(out);
+            total += 4;
+            if (c == 1) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    int s = (int)((out >> (8 * k)) & 1u);
+                    int d = (int)((dsum >> (8 * k)) & 0xffu);
+                    bond += (long long)((1 - 2 * s) * (NN - 2 * d));
+                }
+            }
+        }
+    }
+    if (meas) {
+        int counts[2] = {total - down, down};
+        write_partials(partials, bond, counts, 2, lds);
+    }
+}
+
+// --------------------------------------------------------------- launchers ---
+
+static void grid_for_work(int64_t work, int64_t max_blocks, int *blocks, int64_t *span) {
+    int64_t need = (work + 255) / 256;
+    if (need < 1) need = 1;
+    int64_t G = need < max_blocks ? need : max_blocks;
+    if (G >= 8) G = (G / 8) * 8;
+    *blocks = (int)G;
+    *span = (work + G - 1) / G;
+}
+
+void launch_pack_sites(const uint8_t *lat, uint8_t *c0, uint8_t *c1, const Geom &g, hipStream_t s) {
+    int64_t blocks = (g.N + 255) / 256;
+    pack_sites_kernel<<<dim3((uint32_t)blocks), dim3(256), 0, s>>>(lat, c0, c1, g);
+}
+
+void launch_unpack_sites(uint8_t *lat, const uint8_t *c0, const uint8_t *c1, const Geom &g, hipStream_t s) {
+    int64_t blocks = (g.N + 255) / 256;
+    unpack_sites_kernel<<<dim3((uint32_t)blocks), dim3(256), 0, s>>>(lat, c0, c1, g);
+}
+
+void launch_finalize_record(const long long *pa, int64_t na, const long long *pb, int64_t nb, long long *rec,
+                            hipStream_t s) {
+    finalize_record_kernel<<<dim3(1), dim3(256), 0, s>>>(pa, na, pb, nb, rec);
+}
+
+int site_pass_blocks(const SitePassArgs &a) {
+    int blocks;
+    int64_t span;
+    int64_t work = a.swar ? (int64_t)a.g.R * a.g.H / 4 : (int64_t)a.g.R * a.g.H;
+    grid_for_work(work, a.max_blocks, &blocks, &span);
+    return blocks;
+}
+
+template <int D>
+static void dispatch_generic(const SitePassArgs &a, int blocks, int64_t work, int64_t span) {
+    dim3 grid((uint32_t)blocks), block(256);
+    const uint8_t *oth = a.oth;
+    uint8_t *own = a.own;
+#define PLQ_GEN(MODEL, MODE)                                                                                    \
+    site_pass_generic<D, MODEL, MODE><<<grid, block, 0, a.stream>>>(oth, own, a.g, a.color, a.seed, a.skey, a.q, \
+                                                                    a.table, a.partials, a.meas, work, span)
+    if (a.model == 0) {
+        if (a.mode == 0) PLQ_GEN(0, 0);
+        else if (a.mode == 1) PLQ_GEN(0, 1);
+        else PLQ_GEN(0, 2);
+    } else {
+        if (a.mode == 0) PLQ_GEN(1, 0);
+        else if (a.mode == 1) PLQ_GEN(1, 1);
+        else PLQ_GEN(1, 2);
+    }
+#undef PLQ_GEN
+}
+
+template <int D>
+static void dispatch_swar(const SitePassArgs &a, int blocks, int64_t work, int64_t span) {
+    dim3 grid((uint32_t)blocks), block(256);
+    const uint32_t *oth = (const uint32_t *)a.oth;
+    uint32_t *own = (uint32_t *)a.own;
+    if (a.mode == 0)
+        ising_pass_swar<D, 0><<<grid, block, 0, a.stream>>>(oth, own, a.sg, a.color, a.seed, a.skey, a.table,
+                                                            a.partials, a.meas, work, span);
+    else if (a.mode == 1)
+        ising_pass_swar<D, 1><<<grid, block, 0, a.stream>>>(oth, own, a.sg, a.color, a.seed, a.skey, a.table,
+                                                            a.partials, a.meas, work, span);
+    else
+        ising_pass_swar<D, 2><<<grid, block, 0, a.stream>>>(oth, own, a.sg, a.color, a.seed, a.skey, a.table,
+                                                            a.partials, a.meas, work, span);
+}
+
+void launch_site_pass(const SitePassArgs &a) {
+    int blocks;
+    int64_t span;
+    int64_t work = a.swar ? (int64_t)a.g.R * a.g.H / 4 : (int64_t)a.g.R * a.g.H;
+    grid_for_work(work, a.max_blocks, &blocks, &span);
+    if (a.swar) {
+        switch (a.g.D) {
+            case 1: dispatch_swar<1>(a, blocks, work, span); break;
+            case 2: dispatch_swar<2>(a, blocks, work, span); break;
+            case 3: dispatch_swar<3>(a, blocks, work, span); break;
+            default: dispatch_swar<4>(a, blocks, work, span); break;
+        }
+    } else {
+        switch (a.g.D) {
+            case 1: dispatch_generic<1>(a, blocks, work, span); break;
+            case 2: dispatch_generic<2>(a, blocks, work, span); break;
+            case 3: dispatch_generic<3>(a, blocks, work, span); break;
+            default: dispatch_generic<4>(a, blocks, work, span); break;
+        }
+    }
+}
+
+}  // namespace plq

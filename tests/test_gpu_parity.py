@@ -1,0 +1,260 @@
+"""Bit-exact parity of the HIP path against the CPU oracle (GPU).
+
+Every test calls the product through the C ABI (plaquette_amd ->
+libplaquette_hip.so) and compares with the oracle on the same seeded inputs:
+states must be identical byte for byte, integer observables equal, cluster
+partitions identical after canonical (minimum-site) relabelling.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+M, HB = 0, 1
+
+
+def start_config(n, q, hot, seed):
+    if not hot:
+        return np.zeros(n, np.uint8)
+    return np.random.default_rng(seed).integers(0, q, n).astype(np.uint8)
+
+
+def run_oracle_local(shape, model, q, kernel, profile, beta, j, h, offsets, seed, start, sweeps):
+    d = O.local_desc(shape, model=model, q=q, kernel=kernel, profile=profile, beta=beta, j=j, h=h,
+                     offsets=offsets, seed=seed)
+    return O.local_sweeps(d, start, 0, sweeps)
+
+
+# -- site fields: Ising ---------------------------------------------------------
+
+ISING_CASES = [
+    # shape, kernel, profile, beta, j, h, hot, sweeps
+    ([32, 32, 32], M, 0, 0.2216546, 1.0, 0.0, False, 100),
+    ([32, 32, 32], M, 0, 0.2216546, 1.0, 0.0, True, 100),
+    ([32, 32, 32], HB, 0, 0.2216546, 1.0, 0.0, False, 100),
+    ([32, 32, 32], HB, 0, 0.2216546, 1.0, 0.0, True, 100),
+    ([32, 32, 32], M, 1, 0.2216546, 1.0, 0.0, True, 60),
+    ([32, 32, 32], HB, 1, 0.2216546, 1.0, 0.0, True, 60),
+    ([64, 64], M, 0, 0.4406868, 1.0, 0.0, True, 200),
+    ([64, 64], HB, 1, 0.4406868, 1.0, 0.0, True, 200),
+    ([6, 10], M, 0, 0.4406868, 1.0, 0.0, True, 200),
+    ([4, 6, 8], HB, 0, 0.25, 1.0, 0.1, True, 150),
+    ([4, 6, 8], M, 1, 0.25, 0.7, -0.3, True, 150),
+    ([8, 4, 4, 2], M, 0, 0.15, 1.0, 0.0, True, 100),
+    ([6, 4, 2, 4], HB, 0, 0.15, 1.0, 0.2, True, 100),
+    ([16], M, 0, 0.8, 1.0, 0.1, True, 300),
+    ([10], HB, 1, 0.8, 1.0, 0.0, True, 300),
+    ([16, 8], M, 0, 0.0, 1.0, 0.0, True, 5),
+    ([16, 8], HB, 0, 0.0, 1.0, 0.0, True, 5),
+    ([8, 8, 8], M, 0, 5.0, 1.0, 0.0, True, 40),
+]
+
+
+def _id(case):
+    shape, kernel, profile, beta, j, h, hot, sweeps = case
+    return f"{'x'.join(map(str, shape))}-{'hb' if kernel else 'metro'}-{'f32' if profile else 'f64'}-b{beta}-{'hot' if hot else 'cold'}"
+
+
+@pytest.mark.parametrize("case", ISING_CASES, ids=[_id(c) for c in ISING_CASES])
+def test_ising_checkerboard_is_bit_exact(gpu, case):
+    shape, kernel, profile, beta, j, h, hot, sweeps = case
+    n = int(np.prod(shape))
+    start = start_config(n, 2, hot, 17)
+    seed = 20260728
+    chain = gpu.ising_gpu_chain(shape, j, h, beta, seed, start, sweeps_between=1, kernel=kernel, profile=profile)
+    # the first sweep alone, then the rest
+    got1 = next(chain)
+    want1 = run_oracle_local(shape, O.MODEL_ISING, 2, kernel, profile, beta, j, h, None, seed, start, 1)
+    assert np.array_equal(got1, want1)
+    chain.advance(sweeps - 1)
+    got = chain.read()
+    want = run_oracle_local(shape, O.MODEL_ISING, 2, kernel, profile, beta, j, h, None, seed, start, sweeps)
+    assert chain.sweeps_done == sweeps
+    assert np.array_equal(got, want), f"{(got != want).sum()} sites differ"
+    # device-reduced observables of the final state equal the oracle's sums
+    obs = chain.measure()
+    b, s = O.ising_sums(shape, want)
+    assert obs.bond_sum == b and obs.spin_sum == s
+    assert obs.energy == -j * b - h * s
+    chain.close()
+
+
+def test_upload_readback_identity(gpu):
+    # ising/gpu.rs:116-140, gpu_cluster.rs:651-664: zero sweeps is the identity
+    for shape in ([4, 4], [32, 32, 32], [6, 10]):
+        n = int(np.prod(shape))
+        start = start_config(n, 2, True, 1)
+        c = gpu.ising_gpu_chain(shape, 1.0, 0.0, 0.5, 7, start, sweeps_between=0)
+        assert np.array_equal(next(c), start)
+        c.close()
+    st = start_config(4 * 4 * 4 * 4 * 4, 2, True, 2)
+    c = gpu.gauge_gpu_chain([4, 4, 4, 4], 1.0, 0.5, 7, st, sweeps_between=0)
+    assert np.array_equal(next(c), st)
+    c.close()
+    st = start_config(35 * 3, 3, True, 3)
+    c = gpu.GpuClusterChain.swendsen_wang([5, 7, 3], 3, 1.0, 0.5, 7, st, sweeps_between=0)
+    assert np.array_equal(next(c), st)
+    c.close()
+
+
+def test_zero_coupling_negative_field_flips_every_site_once(gpu):
+    # ising/gpu.rs:142-192 at 2048^2 (4,194,304 sites): RNG-independent
+    n = 2048 * 2048
+    c = gpu.ising_gpu_chain([2048, 2048], 0.0, -1.0, 1.0, 7, np.zeros(n, np.uint8), sweeps_between=1)
+    after = next(c)
+    assert int(after.sum()) == n
+    assert c.measure().spin_sum == -n
+    c.close()
+
+
+def test_launch_shape_does_not_change_the_bits_at_256_cubed(gpu, monkeypatch):
+    # Full-size property: the four-sites-per-lane kernel and the
+    # one-site-per-lane kernel are independent implementations with different
+    # grids; with keyed draws they must agree bit for bit at 256^3.
+    shape = [256, 256, 256]
+    n = int(np.prod(shape))
+    start = start_config(n, 2, True, 5)
+    outs = []
+    for force in ("0", "1"):
+        monkeypatch.setenv("PLQ_FORCE_GENERIC", force)
+        for kernel in (M, HB):
+            c = gpu.ising_gpu_chain(shape, 1.0, 0.0, 0.2216546, 99, start, kernel=kernel)
+            c.advance(12)
+            outs.append((force, kernel, c.read(), c.measure().bond_sum))
+            c.close()
+    assert np.array_equal(outs[0][2], outs[2][2]) and outs[0][3] == outs[2][3]
+    assert np.array_equal(outs[1][2], outs[3][2]) and outs[1][3] == outs[3][3]
+
+
+def test_recorded_series_matches_per_sweep_measurement(gpu):
+    shape = [64, 64]
+    n = 4096
+    start = start_config(n, 2, True, 8)
+    c = gpu.ising_gpu_chain(shape, 1.0, 0.0, 0.4406868, 3, start)
+    c.record(1, 50)
+    c.advance(30)
+    series = c.series()
+    assert [o.sweep for o in series] == list(range(1, 31))
+    st = start.copy()
+    d = O.local_desc(shape, beta=0.4406868, seed=3)
+    for t, o in enumerate(series):
+        st = O.local_sweeps(d, st, t, 1)
+        b, s = O.ising_sums(shape, st)
+        assert (o.bond_sum, o.spin_sum) == (b, s)
+    c.close()
+
+
+# -- site fields: Potts ------------------------------------------------------------
+
+POTTS_CASES = [
+    ([8, 8], 3, M, 0, 1.0050525, None),
+    ([8, 8], 3, HB, 0, 1.0050525, None),
+    ([6, 4, 4], 4, M, 1, 0.6, [0.0, 0.1, -0.2, 0.05]),
+    ([6, 4, 4], 3, HB, 1, 0.6, [0.3, 0.0, -0.1]),
+    ([10, 6], 5, M, 0, 0.9, None),
+]
+
+
+@pytest.mark.parametrize("case", POTTS_CASES)
+def test_potts_checkerboard_is_bit_exact(gpu, case):
+    shape, q, kernel, profile, beta, offsets = case
+    n = int(np.prod(shape))
+    start = start_config(n, q, True, 4)
+    c = gpu.potts_gpu_chain(shape, q, 1.0, offsets, beta, 11, start, kernel=kernel, profile=profile)
+    c.advance(80)
+    got = c.read()
+    want = run_oracle_local(shape, O.MODEL_POTTS, q, kernel, profile, beta, 1.0, 0.0, offsets, 11, start, 80)
+    assert np.array_equal(got, want)
+    agree, counts = O.potts_sums(shape, q, want)
+    obs = c.measure()
+    assert obs.bond_sum == agree and list(obs.counts[:q]) == list(counts)
+    c.close()
+
+
+# -- link fields: Z2 -------------------------------------------------------------
+
+Z2_CASES = [
+    ([8, 8], M, 0, 0.5),
+    ([8, 8], HB, 0, 0.5),
+    ([4, 6, 4], M, 0, 0.75),
+    ([4, 6, 4], HB, 1, 0.75),
+    ([4, 4, 6, 4], HB, 0, 0.44),
+    ([4, 4, 6, 4], M, 1, 0.44),
+    ([8, 8, 8, 8], HB, 0, 0.44),
+]
+
+
+@pytest.mark.parametrize("case", Z2_CASES)
+def test_z2_link_checkerboard_is_bit_exact(gpu, case):
+    shape, kernel, profile, beta = case
+    n = int(np.prod(shape)) * len(shape)
+    start = start_config(n, 2, True, 6)
+    c = gpu.gauge_gpu_chain(shape, 1.0, beta, 21, start, kernel=kernel, profile=profile)
+    c.advance(40)
+    got = c.read()
+    want = run_oracle_local(shape, O.MODEL_Z2, 2, kernel, profile, beta, 1.0, 0.0, None, 21, start, 40)
+    assert np.array_equal(got, want)
+    assert c.measure().bond_sum == O.plaquette_sum(shape, want)
+    c.close()
+
+
+# -- cluster ---------------------------------------------------------------------
+
+CLUSTER_CASES = [
+    # shape, q, gap, beta, seeded, forced, profile, sweeps
+    ([64, 64], 3, 1.0, math.log(1 + math.sqrt(3)), False, False, 0, 20),
+    ([64, 64], 3, 1.0, math.log(1 + math.sqrt(3)), False, False, 1, 10),
+    ([300, 200], 3, 1.0, 1.0050525, False, False, 0, 5),
+    ([5, 7, 3], 3, 1.0, 0.5, False, False, 0, 20),
+    ([16, 16, 16], 2, 2.0, 0.2216546, False, False, 0, 10),
+    ([6, 5, 4, 3], 4, 1.0, 0.7, False, True, 0, 10),
+    ([32, 32], 3, 1.0, 1.2, True, True, 0, 30),
+    ([32, 32], 2, 2.0, 0.4406868, True, True, 1, 30),
+    ([1000], 3, 1.0, 2.0, False, False, 0, 10),
+]
+
+
+@pytest.mark.parametrize("case", CLUSTER_CASES)
+def test_cluster_sweeps_are_bit_exact(gpu, case):
+    shape, q, gap, beta, seeded, forced, profile, sweeps = case
+    n = int(np.prod(shape))
+    start = start_config(n, q, True, 9)
+    extent = gpu.Extent.SEEDED if seeded else gpu.Extent.ALL
+    relabel = gpu.Relabel.FORCED_CHANGE if forced else gpu.Relabel.REDRAW
+    c = gpu.GpuClusterChain.new(shape, q, gap, extent, relabel, beta, 31, start, profile=profile)
+    d = O.cluster_desc(shape, q=q, seeded=seeded, forced=forced, profile=profile, beta=beta, gap=gap, seed=31)
+    st = start
+    for t in range(sweeps):
+        c.advance(1)
+        st, roots = O.cluster_sweeps(d, st, t, 1)
+        got = c.read()
+        assert np.array_equal(got, st), f"sweep {t}: {(got != st).sum()} sites differ"
+        if t in (0, sweeps - 1):
+            assert np.array_equal(c.roots(), roots)
+    stats = c.stats()
+    sizes = np.bincount(roots)
+    sizes = sizes[sizes > 0]
+    assert stats.n_clusters == sizes.size and stats.largest == sizes.max()
+    hist = np.bincount(np.floor(np.log2(sizes)).astype(int), minlength=40)
+    assert list(stats.size_hist) == list(hist)
+    agree, counts = O.potts_sums(shape, q, st)
+    obs = c.measure()
+    assert obs.bond_sum == agree and list(obs.counts[:q]) == list(counts)
+    c.close()
+
+
+def test_cluster_budget_exhaustion_is_reported(gpu):
+    # gpu_cluster.rs:512-518: labelling past its cap is an error, forced from
+    # the host with a zero hop budget on an ordered (percolating) lattice
+    n = 64 * 64
+    c = gpu.GpuClusterChain.swendsen_wang([64, 64], 3, 1.0, 8.0, 1, np.zeros(n, np.uint8))
+    c.set_pass_cap(0)
+    with pytest.raises(gpu.PlaquetteError) as err:
+        c.advance(1)
+    assert err.value.code == gpu.PLQ_ENOCONV
+    c.close()
