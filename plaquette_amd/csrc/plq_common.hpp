@@ -54,6 +54,19 @@ __host__ __device__ __forceinline__ uint32_t fold_seed(uint64_t seed) {
     return (uint32_t)(seed ^ (seed >> 32));
 }
 
+// Element access through a 32-bit byte offset: the address is a uniform base
+// plus a zero-extended 32-bit offset, which the compiler issues as a global
+// load/store with an SGPR base and a VGPR offset (no 64-bit VALU address
+// arithmetic per access).  Callers guarantee idx * sizeof(T) < 2^32.
+template <class T>
+__device__ __forceinline__ T ld32(const T *base, uint32_t idx) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + idx * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void st32(T *base, uint32_t idx, T v) {
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + idx * (uint32_t)sizeof(T)) = v;
+}
+
 constexpr int MAX_D = 4;
 
 // Colour-interleaved site layout.  A site (x0, x1, ..., x_{D-1}) lives in

@@ -241,16 +241,18 @@ __global__ __launch_bounds__(256) void ising_pass_swar(const uint32_t *__restric
     __shared__ long long lds[16];
     if (threadIdx.x < 2 * K) T[threadIdx.x] = table[threadIdx.x];
     __syncthreads();
-    const int64_t lb = logical_block();
-    const int64_t begin = lb * span;
-    int64_t end = begin + span;
-    if (end > work) end = work;
-    long long bond = 0;
+    // 32-bit offsets throughout (N / 8 dwords < 2^32): every load is an SGPR
+    // base plus a 32-bit VGPR offset, with no 64-bit address arithmetic.
+    const uint32_t lb = (uint32_t)logical_block();
+    const uint32_t begin = lb * (uint32_t)span;
+    uint32_t end = begin + (uint32_t)span;
+    if (end > (uint32_t)work) end = (uint32_t)work;
+    int bond = 0;
     int down = 0, total = 0;
     const uint32_t h4mask = (1u << sg.lh4) - 1u;
-    for (int64_t w = begin + threadIdx.x; w < end; w += blockDim.x) {
-        uint32_t r = (uint32_t)(w >> sg.lh4);
-        uint32_t i4 = (uint32_t)w & h4mask;
+    for (uint32_t w = begin + threadIdx.x; w < end; w += blockDim.x) {
+        uint32_t r = w >> sg.lh4;
+        uint32_t i4 = w & h4mask;
         uint32_t x1 = r & (sg.L1 - 1u);
         uint32_t x2 = 0, x3 = 0;
         if (D >= 3) x2 = (r >> sg.ll1) & (sg.L2 - 1u);
@@ -258,36 +260,34 @@ __global__ __launch_bounds__(256) void ising_pass_swar(const uint32_t *__restric
         uint32_t prs = x1 + x2 + x3;
         uint32_t p = (prs + (uint32_t)c) & 1u;
         const uint32_t rb = r << sg.lh4;
-        uint32_t a0 = oth[rb + i4];
-        uint32_t sh;
-        if (p) {
-            uint32_t an = oth[rb + ((i4 + 1u) & h4mask)];
-            sh = (a0 >> 8) | (an << 24);
-        } else {
-            uint32_t ap = oth[rb + ((i4 - 1u) & h4mask)];
-            sh = (a0 << 8) | (ap >> 24);
-        }
+        // The along-axis-0 neighbours: the other colour's dword at the same
+        // position, and the same bytes shifted one site forward (p = 1) or
+        // back (p = 0), borrowing a byte from the adjacent dword of the row
+        // (wrapping at the row ends).  Branch-free: p alternates row by row.
+        uint32_t a0 = ld32(oth, rb + i4);
+        uint32_t adj = ld32(oth, rb + ((i4 + (p ? 1u : h4mask)) & h4mask));
+        uint32_t sh = p ? ((a0 >> 8) | (adj << 24)) : ((a0 << 8) | (adj >> 24));
         uint32_t dsum = a0 + sh;
         if (D >= 2) {
             uint32_t rp = (x1 + 1u == sg.L1) ? r - (sg.L1 - 1u) : r + 1u;
             uint32_t rm = (x1 == 0u) ? r + (sg.L1 - 1u) : r - 1u;
-            dsum += oth[(rp << sg.lh4) + i4] + oth[(rm << sg.lh4) + i4];
+            dsum += ld32(oth, (rp << sg.lh4) + i4) + ld32(oth, (rm << sg.lh4) + i4);
         }
         if (D >= 3) {
             uint32_t st = sg.L1;
             uint32_t span2 = (sg.L2 - 1u) * st;
             uint32_t rp = (x2 + 1u == sg.L2) ? r - span2 : r + st;
             uint32_t rm = (x2 == 0u) ? r + span2 : r - st;
-            dsum += oth[(rp << sg.lh4) + i4] + oth[(rm << sg.lh4) + i4];
+            dsum += ld32(oth, (rp << sg.lh4) + i4) + ld32(oth, (rm << sg.lh4) + i4);
         }
         if (D >= 4) {
             uint32_t st = sg.L1 * sg.L2;
             uint32_t span3 = (sg.L3 - 1u) * st;
             uint32_t rp = (x3 + 1u == sg.L3) ? r - span3 : r + st;
             uint32_t rm = (x3 == 0u) ? r + span3 : r - st;
-            dsum += oth[(rp << sg.lh4) + i4] + oth[(rm << sg.lh4) + i4];
+            dsum += ld32(oth, (rp << sg.lh4) + i4) + ld32(oth, (rm << sg.lh4) + i4);
         }
-        uint32_t s4 = own[rb + i4];
+        uint32_t s4 = ld32((const uint32_t *)own, rb + i4);
         uint32_t out = s4;
         if (MODE != 2) {
             out = 0;
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(256) void ising_pass_swar(const uint32_t *__restric
                 uint32_t nxt = (MODE == 0) ? (s ^ (hit ? 1u : 0u)) : (hit ? 0u : 1u);
                 out |= nxt << (8 * k);
             }
-            own[rb + i4] = out;
+            st32(own, rb + i4, out);
         }
         if (meas) {
             down += __popc(out);
@@ -314,14 +314,14 @@ __global__ __launch_bounds__(256) void ising_pass_swar(const uint32_t *__restric
                 for (int k = 0; k < 4; ++k) {
                     int s = (int)((out >> (8 * k)) & 1u);
                     int d = (int)((dsum >> (8 * k)) & 0xffu);
-                    bond += (long long)((1 - 2 * s) * (NN - 2 * d));
+                    bond += (1 - 2 * s) * (NN - 2 * d);
                 }
             }
         }
     }
     if (meas) {
         int counts[2] = {total - down, down};
-        write_partials(partials, bond, counts, 2, lds);
+        write_partials(partials, (long long)bond, counts, 2, lds);
     }
 }
 
