@@ -63,7 +63,7 @@ struct plq_chain {
     // local fields
     plq::Geom g{};
     plq::SwarGeom sg{};
-    bool swar = false;
+    int vec = 1;  // sites per lane of the site kernel
     uint8_t *col[2] = {nullptr, nullptr};  // site colours
     uint8_t *links[2 * plq::MAX_D] = {nullptr};
     uint8_t **d_links = nullptr;           // device copy of the link array pointers
@@ -196,7 +196,7 @@ plq::SitePassArgs site_args(plq_chain *c, int color, uint32_t sweep, int mode, i
     a.own = c->col[color];
     a.g = c->g;
     a.sg = c->sg;
-    a.swar = c->swar;
+    a.vec = c->vec;
     a.model = c->model == PLQ_MODEL_POTTS ? 1 : 0;
     a.mode = mode;
     a.color = color;
@@ -456,13 +456,19 @@ int plq_chain_create_local(const plq_local_desc *d, const uint8_t *start, size_t
     make_geom(c);
     bool all_pow2 = true;
     for (int mu = 0; mu < d->dim; ++mu) all_pow2 = all_pow2 && is_pow2(d->shape[mu]);
-    // PLQ_FORCE_GENERIC=1 selects the one-site-per-lane kernel even where the
-    // four-sites-per-lane one applies; tests use it to check that both launch
-    // shapes produce identical bits.
-    const char *force = getenv("PLQ_FORCE_GENERIC");
-    bool generic_only = force && force[0] == '1';
-    c->swar = !generic_only && c->kind == KIND_SITE && d->model == PLQ_MODEL_ISING && all_pow2 && d->shape[0] >= 8;
-    if (c->swar) {
+    // Sites per lane of the Ising kernel: 16 when L0 >= 32, 4 when L0 >= 8
+    // (power-of-two extents), else the one-site-per-lane kernel.
+    // PLQ_SITES_PER_LANE (1, 4 or 16) caps the choice; tests use it to check
+    // that every launch shape produces identical bits.
+    c->vec = 1;
+    if (c->kind == KIND_SITE && d->model == PLQ_MODEL_ISING && all_pow2)
+        c->vec = d->shape[0] >= 32 ? 16 : (d->shape[0] >= 8 ? 4 : 1);
+    if (const char *cap = getenv("PLQ_SITES_PER_LANE")) {
+        int v = atoi(cap);
+        if (v == 1 || v == 4 || v == 16)
+            while (c->vec > v) c->vec /= 4;
+    }
+    if (c->vec > 1) {
         c->sg.lh4 = ilog2(d->shape[0] / 8);
         c->sg.L1 = d->dim > 1 ? (uint32_t)d->shape[1] : 1;
         c->sg.L2 = d->dim > 2 ? (uint32_t)d->shape[2] : 1;

@@ -26,7 +26,7 @@ struct SitePassArgs {
     uint8_t *own;        // this colour's array (updated)
     Geom g;
     SwarGeom sg;
-    bool swar;
+    int vec;             // sites per lane: 1 (generic), 4 or 16 (power-of-two Ising)
     int model;           // 0 Ising, 1 Potts
     int mode;            // 0 Metropolis, 1 heat bath, 2 measure only
     int color;

@@ -97,16 +97,34 @@ __device__ __forceinline__ void write_partials(long long *partials, long long bo
     }
 }
 
-// Sum two runs of block partials (one per colour pass) into one record.
-__global__ void finalize_record_kernel(const long long *__restrict__ pa, int64_t na, const long long *__restrict__ pb,
-                                       int64_t nb, long long *__restrict__ rec) {
-    __shared__ long long lds[16];
+// Sum two runs of block partials (one per colour pass) into one record: each
+// lane accumulates whole rows, then one wave-and-block reduction of all REC_K
+// columns at once.
+__global__ __launch_bounds__(1024) void finalize_record_kernel(const long long *__restrict__ pa, int64_t na,
+                                                               const long long *__restrict__ pb, int64_t nb,
+                                                               long long *__restrict__ rec) {
+    __shared__ long long acc[16][REC_K];
+    long long v[REC_K];
+#pragma unroll
+    for (int k = 0; k < REC_K; ++k) v[k] = 0;
+    for (int64_t b = threadIdx.x; b < na + nb; b += blockDim.x) {
+        const long long *row = b < na ? pa + b * REC_K : pb + (b - na) * REC_K;
+#pragma unroll
+        for (int k = 0; k < REC_K; ++k) v[k] += row[k];
+    }
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
     for (int k = 0; k < REC_K; ++k) {
-        long long v = 0;
-        for (int64_t b = threadIdx.x; b < na; b += blockDim.x) v += pa[b * REC_K + k];
-        for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) v += pb[b * REC_K + k];
-        long long t = block_sum(v, lds);
-        if (threadIdx.x == 0) rec[k] = t;
+        long long t = v[k];
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        if (lane == 0) acc[wave][k] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < REC_K) {
+        long long t = 0;
+        int nw = (int)(blockDim.x >> 6);
+        for (int w = 0; w < nw; ++w) t += acc[w][threadIdx.x];
+        rec[threadIdx.x] = t;
     }
 }
 
@@ -325,6 +343,160 @@ __global__ __launch_bounds__(256) void ising_pass_swar(const uint32_t *__restric
     }
 }
 
+// ----------------------------------------------------- 16-site Ising pass ---
+
+// lowbias32 after its first xor-shift (the caller supplies v ^ (v >> 16)).
+__device__ __forceinline__ uint32_t lowbias32_tail(uint32_t v) {
+    v *= 0x7feb352du;
+    v ^= v >> 15;
+    v *= 0x846ca68bu;
+    v ^= v >> 16;
+    return v;
+}
+
+// Spread the low four bits of x to bit 0 of each byte: bit k of x lands at
+// bit 8k of x * (1 + 2^7 + 2^14 + 2^21), and no other partial product falls
+// on a multiple of 8 or collides with another.
+__device__ __forceinline__ uint32_t spread4(uint32_t x) { return (x * 0x00204081u) & 0x01010101u; }
+
+// Sixteen same-colour sites per lane: one 16-byte chunk of a colour row.
+// Requires power-of-two extents with L0 >= 32.  Per site the work is one
+// LDS threshold read, the keyed draw and one compare; everything else is
+// done per dword (4 sites) or per chunk (16 sites):
+//   - the "down" neighbour counts of 4 sites are one dword sum;
+//   - the class byte of each site, scaled to an LDS byte address, is built
+//     for 4 sites at once ((dsum << 3) | (state << 2) indexes a copy of the
+//     table reordered to [d][state]);
+//   - the site indices of a chunk are site0 | 2k (site0 has bits 1..4 clear),
+//     so the first xor-shift of the first hash round, (x ^ x >> 16) with
+//     x = seed ^ site, is computed once per chunk and per site only the
+//     constant 2k is xored in;
+//   - the 16 decisions are collected as bits and spread back to bytes with
+//     one multiply per dword.
+template <int D, int MODE>
+__global__ __launch_bounds__(256) void ising_pass_x16(const uint4 *__restrict__ oth, uint4 *__restrict__ own,
+                                                      SwarGeom sg, int c, uint32_t seed, uint32_t skey,
+                                                      const uint32_t *__restrict__ table,
+                                                      long long *__restrict__ partials, int meas, int64_t work,
+                                                      int64_t span) {
+    constexpr int NN = 2 * D;
+    constexpr int K = NN + 1;
+    __shared__ uint32_t T[2 * 16];  // T[d * 2 + s], byte address (d << 3) | (s << 2)
+    __shared__ long long lds[16];
+    if (threadIdx.x < 2 * K) {
+        int s = threadIdx.x / K, d = threadIdx.x % K;
+        T[d * 2 + s] = table[threadIdx.x];
+    }
+    __syncthreads();
+    const uint32_t lb = (uint32_t)logical_block();
+    const uint32_t begin = lb * (uint32_t)span;
+    uint32_t end = begin + (uint32_t)span;
+    if (end > (uint32_t)work) end = (uint32_t)work;
+    int bond = 0;
+    int down = 0, total = 0;
+    const uint32_t lh16 = sg.lh4 - 2u;        // log2(chunks per colour row)
+    const uint32_t cmask = (1u << lh16) - 1u;
+    const char *Tb = reinterpret_cast<const char *>(T);
+    for (uint32_t w = begin + threadIdx.x; w < end; w += blockDim.x) {
+        uint32_t r = w >> lh16;
+        uint32_t ch = w & cmask;
+        uint32_t x1 = r & (sg.L1 - 1u);
+        uint32_t x2 = 0, x3 = 0;
+        if (D >= 3) x2 = (r >> sg.ll1) & (sg.L2 - 1u);
+        if (D >= 4) x3 = r >> (sg.ll1 + sg.ll2);
+        uint32_t p = (x1 + x2 + x3 + (uint32_t)c) & 1u;
+        const uint32_t rc = r << lh16;  // chunk index of the row start
+        uint4 a = ld32(oth, w);
+        // the byte beyond the chunk on the side the shift pulls from
+        uint32_t adj_dw = p ? ((rc + ((ch + 1u) & cmask)) << 2) : (((rc + ((ch - 1u) & cmask)) << 2) + 3u);
+        uint32_t adj = ld32(reinterpret_cast<const uint32_t *>(oth), adj_dw);
+        uint4 sh;
+        if (p) {
+            sh.x = __builtin_amdgcn_alignbyte(a.y, a.x, 1);
+            sh.y = __builtin_amdgcn_alignbyte(a.z, a.y, 1);
+            sh.z = __builtin_amdgcn_alignbyte(a.w, a.z, 1);
+            sh.w = __builtin_amdgcn_alignbyte(adj, a.w, 1);
+        } else {
+            sh.x = __builtin_amdgcn_alignbyte(a.x, adj, 3);
+            sh.y = __builtin_amdgcn_alignbyte(a.y, a.x, 3);
+            sh.z = __builtin_amdgcn_alignbyte(a.z, a.y, 3);
+            sh.w = __builtin_amdgcn_alignbyte(a.w, a.z, 3);
+        }
+        uint4 ds;
+        ds.x = a.x + sh.x;
+        ds.y = a.y + sh.y;
+        ds.z = a.z + sh.z;
+        ds.w = a.w + sh.w;
+        auto add_row = [&](uint32_t rr) {
+            uint4 b = ld32(oth, (rr << lh16) + ch);
+            ds.x += b.x;
+            ds.y += b.y;
+            ds.z += b.z;
+            ds.w += b.w;
+        };
+        if (D >= 2) {
+            add_row((x1 + 1u == sg.L1) ? r - (sg.L1 - 1u) : r + 1u);
+            add_row((x1 == 0u) ? r + (sg.L1 - 1u) : r - 1u);
+        }
+        if (D >= 3) {
+            uint32_t st = sg.L1, span2 = (sg.L2 - 1u) * sg.L1;
+            add_row((x2 + 1u == sg.L2) ? r - span2 : r + st);
+            add_row((x2 == 0u) ? r + span2 : r - st);
+        }
+        if (D >= 4) {
+            uint32_t st = sg.L1 * sg.L2, span3 = (sg.L3 - 1u) * st;
+            add_row((x3 + 1u == sg.L3) ? r - span3 : r + st);
+            add_row((x3 == 0u) ? r + span3 : r - st);
+        }
+        uint4 s = ld32(reinterpret_cast<const uint4 *>(own), w);
+        uint4 out = s;
+        if (MODE != 2) {
+            const uint32_t site0 = (r << (sg.lh4 + 3u)) + (ch << 5) + p;
+            const uint32_t bx = seed ^ site0;
+            const uint32_t v0 = bx ^ (bx >> 16);
+            auto decide4 = [&](uint32_t sw, uint32_t dw, uint32_t kbase) -> uint32_t {
+                const uint32_t addr4 = (dw << 3) | (sw << 2);
+                uint32_t hb = 0;
+#pragma unroll
+                for (int k = 3; k >= 0; --k) {
+                    uint32_t addr = (addr4 >> (8 * k)) & 0xffu;
+                    uint32_t t = *reinterpret_cast<const uint32_t *>(Tb + addr);
+                    uint32_t h1 = lowbias32_tail(v0 ^ (2u * (kbase + (uint32_t)k)));
+                    uint32_t m = lowbias32(h1 ^ skey) >> 8;
+                    hb = hb + hb + (m < t ? 1u : 0u);
+                }
+                uint32_t bits = spread4(hb);
+                return MODE == 0 ? (sw ^ bits) : (bits ^ 0x01010101u);
+            };
+            out.x = decide4(s.x, ds.x, 0);
+            out.y = decide4(s.y, ds.y, 4);
+            out.z = decide4(s.z, ds.z, 8);
+            out.w = decide4(s.w, ds.w, 12);
+            st32(own, w, out);
+        }
+        if (meas) {
+            down += __popc(out.x) + __popc(out.y) + __popc(out.z) + __popc(out.w);
+            total += 16;
+            if (c == 1) {
+                uint32_t o4[4] = {out.x, out.y, out.z, out.w};
+                uint32_t d4[4] = {ds.x, ds.y, ds.z, ds.w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        int sn = (int)((o4[j] >> (8 * k)) & 1u);
+                        int dn = (int)((d4[j] >> (8 * k)) & 0xffu);
+                        bond += (1 - 2 * sn) * (NN - 2 * dn);
+                    }
+            }
+        }
+    }
+    if (meas) {
+        int counts[2] = {total - down, down};
+        write_partials(partials, (long long)bond, counts, 2, lds);
+    }
+}
+
 // --------------------------------------------------------------- launchers ---
 
 static void grid_for_work(int64_t work, int64_t max_blocks, int *blocks, int64_t *span) {
@@ -348,14 +520,15 @@ void launch_unpack_sites(uint8_t *lat, const uint8_t *c0, const uint8_t *c1, con
 
 void launch_finalize_record(const long long *pa, int64_t na, const long long *pb, int64_t nb, long long *rec,
                             hipStream_t s) {
-    finalize_record_kernel<<<dim3(1), dim3(256), 0, s>>>(pa, na, pb, nb, rec);
+    finalize_record_kernel<<<dim3(1), dim3(1024), 0, s>>>(pa, na, pb, nb, rec);
 }
+
+static int64_t pass_work(const SitePassArgs &a) { return (int64_t)a.g.R * a.g.H / a.vec; }
 
 int site_pass_blocks(const SitePassArgs &a) {
     int blocks;
     int64_t span;
-    int64_t work = a.swar ? (int64_t)a.g.R * a.g.H / 4 : (int64_t)a.g.R * a.g.H;
-    grid_for_work(work, a.max_blocks, &blocks, &span);
+    grid_for_work(pass_work(a), a.max_blocks, &blocks, &span);
     return blocks;
 }
 
@@ -380,41 +553,53 @@ static void dispatch_generic(const SitePassArgs &a, int blocks, int64_t work, in
 }
 
 template <int D>
-static void dispatch_swar(const SitePassArgs &a, int blocks, int64_t work, int64_t span) {
+static void dispatch_x4(const SitePassArgs &a, int blocks, int64_t work, int64_t span) {
     dim3 grid((uint32_t)blocks), block(256);
     const uint32_t *oth = (const uint32_t *)a.oth;
     uint32_t *own = (uint32_t *)a.own;
-    if (a.mode == 0)
-        ising_pass_swar<D, 0><<<grid, block, 0, a.stream>>>(oth, own, a.sg, a.color, a.seed, a.skey, a.table,
-                                                            a.partials, a.meas, work, span);
-    else if (a.mode == 1)
-        ising_pass_swar<D, 1><<<grid, block, 0, a.stream>>>(oth, own, a.sg, a.color, a.seed, a.skey, a.table,
-                                                            a.partials, a.meas, work, span);
-    else
-        ising_pass_swar<D, 2><<<grid, block, 0, a.stream>>>(oth, own, a.sg, a.color, a.seed, a.skey, a.table,
-                                                            a.partials, a.meas, work, span);
+#define PLQ_X4(MODE)                                                                                         \
+    ising_pass_swar<D, MODE><<<grid, block, 0, a.stream>>>(oth, own, a.sg, a.color, a.seed, a.skey, a.table, \
+                                                           a.partials, a.meas, work, span)
+    if (a.mode == 0) PLQ_X4(0);
+    else if (a.mode == 1) PLQ_X4(1);
+    else PLQ_X4(2);
+#undef PLQ_X4
+}
+
+template <int D>
+static void dispatch_x16(const SitePassArgs &a, int blocks, int64_t work, int64_t span) {
+    dim3 grid((uint32_t)blocks), block(256);
+    const uint4 *oth = (const uint4 *)a.oth;
+    uint4 *own = (uint4 *)a.own;
+#define PLQ_X16(MODE)                                                                                       \
+    ising_pass_x16<D, MODE><<<grid, block, 0, a.stream>>>(oth, own, a.sg, a.color, a.seed, a.skey, a.table, \
+                                                          a.partials, a.meas, work, span)
+    if (a.mode == 0) PLQ_X16(0);
+    else if (a.mode == 1) PLQ_X16(1);
+    else PLQ_X16(2);
+#undef PLQ_X16
 }
 
 void launch_site_pass(const SitePassArgs &a) {
     int blocks;
     int64_t span;
-    int64_t work = a.swar ? (int64_t)a.g.R * a.g.H / 4 : (int64_t)a.g.R * a.g.H;
+    int64_t work = pass_work(a);
     grid_for_work(work, a.max_blocks, &blocks, &span);
-    if (a.swar) {
-        switch (a.g.D) {
-            case 1: dispatch_swar<1>(a, blocks, work, span); break;
-            case 2: dispatch_swar<2>(a, blocks, work, span); break;
-            case 3: dispatch_swar<3>(a, blocks, work, span); break;
-            default: dispatch_swar<4>(a, blocks, work, span); break;
-        }
-    } else {
-        switch (a.g.D) {
-            case 1: dispatch_generic<1>(a, blocks, work, span); break;
-            case 2: dispatch_generic<2>(a, blocks, work, span); break;
-            case 3: dispatch_generic<3>(a, blocks, work, span); break;
-            default: dispatch_generic<4>(a, blocks, work, span); break;
-        }
+#define PLQ_BY_D(FN)                                   \
+    switch (a.g.D) {                                   \
+        case 1: FN<1>(a, blocks, work, span); break;   \
+        case 2: FN<2>(a, blocks, work, span); break;   \
+        case 3: FN<3>(a, blocks, work, span); break;   \
+        default: FN<4>(a, blocks, work, span); break;  \
     }
+    if (a.vec == 16) {
+        PLQ_BY_D(dispatch_x16)
+    } else if (a.vec == 4) {
+        PLQ_BY_D(dispatch_x4)
+    } else {
+        PLQ_BY_D(dispatch_generic)
+    }
+#undef PLQ_BY_D
 }
 
 }  // namespace plq

@@ -112,23 +112,46 @@ def test_zero_coupling_negative_field_flips_every_site_once(gpu):
     c.close()
 
 
-def test_launch_shape_does_not_change_the_bits_at_256_cubed(gpu, monkeypatch):
-    # Full-size property: the four-sites-per-lane kernel and the
-    # one-site-per-lane kernel are independent implementations with different
-    # grids; with keyed draws they must agree bit for bit at 256^3.
+@pytest.mark.parametrize("kernel", [M, HB])
+def test_launch_shape_does_not_change_the_bits_at_256_cubed(gpu, monkeypatch, kernel):
+    # Full-size property: the 16-, 4- and 1-site-per-lane kernels are
+    # independent implementations with different grids; with keyed draws they
+    # must agree bit for bit at 256^3, and so must their fused observables.
     shape = [256, 256, 256]
     n = int(np.prod(shape))
     start = start_config(n, 2, True, 5)
     outs = []
-    for force in ("0", "1"):
-        monkeypatch.setenv("PLQ_FORCE_GENERIC", force)
-        for kernel in (M, HB):
-            c = gpu.ising_gpu_chain(shape, 1.0, 0.0, 0.2216546, 99, start, kernel=kernel)
-            c.advance(12)
-            outs.append((force, kernel, c.read(), c.measure().bond_sum))
-            c.close()
-    assert np.array_equal(outs[0][2], outs[2][2]) and outs[0][3] == outs[2][3]
-    assert np.array_equal(outs[1][2], outs[3][2]) and outs[1][3] == outs[3][3]
+    for vec in ("16", "4", "1"):
+        monkeypatch.setenv("PLQ_SITES_PER_LANE", vec)
+        c = gpu.ising_gpu_chain(shape, 1.0, 0.0, 0.2216546, 99, start, kernel=kernel)
+        c.record(4, 8)
+        c.advance(12)
+        outs.append((c.read(), [(o.bond_sum, o.spin_sum) for o in c.series()], c.measure().bond_sum))
+        c.close()
+    for other in outs[1:]:
+        assert np.array_equal(outs[0][0], other[0])
+        assert outs[0][1] == other[1] and outs[0][2] == other[2]
+    assert outs[0][1][-1][0] == outs[0][2]
+
+
+@pytest.mark.parametrize("vec", ["16", "4", "1"])
+def test_every_launch_shape_is_bit_exact_against_the_oracle(gpu, monkeypatch, vec):
+    monkeypatch.setenv("PLQ_SITES_PER_LANE", vec)
+    for shape, kernel in (([64, 32, 32], M), ([32, 64], HB), ([32, 8, 16, 8], M)):
+        n = int(np.prod(shape))
+        start = start_config(n, 2, True, 12)
+        c = gpu.ising_gpu_chain(shape, 1.0, 0.05, 0.3, 5, start, kernel=kernel)
+        c.record(1, 30)
+        c.advance(30)
+        got = c.read()
+        series = c.series()
+        c.close()
+        d = O.local_desc(shape, kernel=kernel, beta=0.3, h=0.05, seed=5)
+        st = start
+        for t in range(30):
+            st = O.local_sweeps(d, st, t, 1)
+            assert (series[t].bond_sum, series[t].spin_sum) == O.ising_sums(shape, st)
+        assert np.array_equal(got, st)
 
 
 def test_recorded_series_matches_per_sweep_measurement(gpu):
