@@ -150,8 +150,43 @@ def read_traffic():
         return None
 
 
+def profile_passes(sweeps: int, copy_mib: int) -> None:
+    """A short fixed workload for hardware-counter runs: `sweeps` Metropolis
+    sweeps of the 256^3 lattice (2 colour-pass launches each, no measurement
+    launches), then one device-to-device copy of `copy_mib` MiB whose
+    FETCH_SIZE / WRITE_SIZE calibrate the counters against known bytes."""
+    import numpy as np
+
+    import plaquette_amd as P
+
+    n = int(np.prod(SHAPE))
+    chain = P.ising_gpu_chain(SHAPE, 1.0, 0.0, BETA, SEED, np.zeros(n, np.uint8))
+    chain.advance(2)
+    chain.sync()
+    chain.advance(sweeps)
+    chain.sync()
+    chain.close()
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipDeviceSynchronize.argtypes = []
+    hip.hipFree.argtypes = [C.c_void_p]
+    nbytes = copy_mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), nbytes) or hip.hipMalloc(C.byref(b), nbytes):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemcpy(b, a, nbytes, 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"profile": "passes", "sweeps": sweeps, "pass_launches": 2 * sweeps,
+                      "copy_bytes": nbytes, "site_updates_per_launch": n // 2}))
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--profile", action="store_true",
+                    help="run the fixed counter-collection workload instead of the benchmark")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
@@ -159,6 +194,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
+    if args.profile:
+        profile_passes(10, 256)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on these single-GPU boxes "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
@@ -208,7 +246,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "ising_pass_swar<3,0> (one colour pass)",
+            "kernel": "ising_pass_x16<3,0> (one colour pass)",
             "avg_launch_us": round(metro["pass_ms"] * 1e3, 3),
             "site_updates_per_ns_kernel": round((n // 2) / (metro["pass_ms"] * 1e6), 1),
         },

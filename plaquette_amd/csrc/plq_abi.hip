@@ -190,6 +190,8 @@ int alloc_scratch(plq_chain *c) {
 
 // ---------------------------------------------------------- local sweeps ---
 
+// The arguments of one colour pass.  When `meas` is set the pass leaves its
+// column-major partial sums in its half of d_partials (colour 0 first).
 plq::SitePassArgs site_args(plq_chain *c, int color, uint32_t sweep, int mode, int meas) {
     plq::SitePassArgs a{};
     a.oth = c->col[1 - color];
@@ -204,22 +206,27 @@ plq::SitePassArgs site_args(plq_chain *c, int color, uint32_t sweep, int mode, i
     a.skey = sweep * plq::GOLDEN;
     a.q = c->q;
     a.table = c->d_table;
-    a.partials = c->d_partials + (color ? plq::MAX_PASS_BLOCKS * plq::REC_K : 0);
-    a.meas = meas;
     a.max_blocks = plq::MAX_PASS_BLOCKS;
     a.stream = c->stream;
+    a.m.part = c->d_partials + (color ? plq::MAX_PASS_BLOCKS * plq::REC_K : 0);
+    a.m.on = meas;
     return a;
 }
 
-// Enqueue a measurement of the current site field into rec (REC_K entries).
-void enqueue_site_measure(plq_chain *c, long long *rec) {
-    plq::SitePassArgs a0 = site_args(c, 0, 0, 2, 1);
-    plq::SitePassArgs a1 = site_args(c, 1, 0, 2, 1);
+// Enqueue the two colour passes of one sweep (mode 2: measure only, no
+// update); with rec non-null, fold their partial sums into rec.
+void enqueue_site_passes(plq_chain *c, uint32_t sweep, int mode, long long *rec) {
+    plq::SitePassArgs a0 = site_args(c, 0, sweep, mode, rec != nullptr);
+    plq::SitePassArgs a1 = site_args(c, 1, sweep, mode, rec != nullptr);
     plq::launch_site_pass(a0);
     plq::launch_site_pass(a1);
-    plq::launch_finalize_record(a0.partials, plq::site_pass_blocks(a0), a1.partials, plq::site_pass_blocks(a1), rec,
-                                c->stream);
+    if (rec)
+        plq::launch_finalize_columns(a0.m.part, plq::site_pass_blocks(a0), a1.m.part, plq::site_pass_blocks(a1),
+                                     2 + c->q, rec, c->stream);
 }
+
+// Enqueue a measurement of the current site field into rec (REC_K entries).
+void enqueue_site_measure(plq_chain *c, long long *rec) { enqueue_site_passes(c, 0, 2, rec); }
 
 void enqueue_measure(plq_chain *c, long long *rec) {
     if (c->kind == KIND_SITE) {
@@ -243,14 +250,8 @@ int enqueue_local_sweep(plq_chain *c) {
     bool record = c->rec_every && ((c->sweeps_done + 1) % c->rec_every == 0);
     int mode = c->kernel == PLQ_KERNEL_METROPOLIS ? 0 : 1;
     if (c->kind == KIND_SITE) {
-        plq::SitePassArgs a0 = site_args(c, 0, sweep, mode, record);
-        plq::SitePassArgs a1 = site_args(c, 1, sweep, mode, record);
-        plq::launch_site_pass(a0);
-        plq::launch_site_pass(a1);
         c->sweeps_done += 1;
-        if (record)
-            plq::launch_finalize_record(a0.partials, plq::site_pass_blocks(a0), a1.partials,
-                                        plq::site_pass_blocks(a1), rec_slot(c), c->stream);
+        enqueue_site_passes(c, sweep, mode, record ? rec_slot(c) : nullptr);
     } else {
         for (int dir = 0; dir < c->D; ++dir)
             for (int color = 0; color < 2; ++color) {

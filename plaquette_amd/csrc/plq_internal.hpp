@@ -21,6 +21,14 @@ struct SwarGeom {
     uint32_t ll1, ll2;
 };
 
+// Fused measurement of a colour pass: partial sums per block, column-major
+// (column k of block b at part[k * blocks + b]), folded into a record by
+// launch_finalize_columns after the colour-1 pass.
+struct MeasArgs {
+    long long *part;       // this pass's partials
+    int on;                // this pass measures
+};
+
 struct SitePassArgs {
     const uint8_t *oth;  // the other colour's array (read)
     uint8_t *own;        // this colour's array (updated)
@@ -34,8 +42,7 @@ struct SitePassArgs {
     uint32_t skey;       // sweep * 0x9e3779b9
     int q;
     const uint32_t *table;
-    long long *partials; // REC_K per block, written when meas
-    int meas;
+    MeasArgs m;
     int64_t max_blocks;
     hipStream_t stream;
 };
@@ -46,6 +53,8 @@ void launch_site_pass(const SitePassArgs &a);
 int site_pass_blocks(const SitePassArgs &a);
 void launch_finalize_record(const long long *pa, int64_t na, const long long *pb, int64_t nb, long long *rec,
                             hipStream_t s);
+void launch_finalize_columns(const long long *pa, int64_t na, const long long *pb, int64_t nb, int nk, long long *rec,
+                             hipStream_t s);
 
 // ------------------------------------------------------------------ links ---
 

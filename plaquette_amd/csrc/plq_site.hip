@@ -85,15 +85,37 @@ __device__ __forceinline__ int64_t logical_block() {
     return (int64_t)(b & 7u) * (G >> 3) + (b >> 3);
 }
 
-// Write a record's partial sums: slot layout [bond, 0, counts[0..15]].
-__device__ __forceinline__ void write_partials(long long *partials, long long bond, const int *counts, int q,
-                                               long long *lds) {
-    long long *dst = partials + (int64_t)blockIdx.x * REC_K;
+// Write this block's partial sums of a record, column-major: column k of
+// block b at part[k * gridDim.x + b], columns [bond, 0, counts[0..q)].
+// finalize_columns_kernel folds both colour passes' columns into the record.
+__device__ __forceinline__ void emit_partials(const MeasArgs &m, long long bond, const int *counts, int q,
+                                              long long *lds) {
+    const int64_t G = gridDim.x;
     long long t = block_sum(bond, lds);
-    if (threadIdx.x == 0) dst[0] = t;
+    if (threadIdx.x == 0) {
+        m.part[blockIdx.x] = t;
+        m.part[G + blockIdx.x] = 0;
+    }
     for (int a = 0; a < q; ++a) {
         t = block_sum((long long)counts[a], lds);
-        if (threadIdx.x == 0) dst[2 + a] = t;
+        if (threadIdx.x == 0) m.part[(2 + a) * G + blockIdx.x] = t;
+    }
+}
+
+// Fold the column-major partials of the two colour passes (na and nb blocks)
+// into a record: nk columns, each a contiguous run, read coalesced.
+__global__ __launch_bounds__(1024) void finalize_columns_kernel(const long long *__restrict__ pa, int64_t na,
+                                                                const long long *__restrict__ pb, int64_t nb, int nk,
+                                                                long long *__restrict__ rec) {
+    __shared__ long long lds[16];
+    for (int k = 0; k < REC_K; ++k) {
+        long long v = 0;
+        if (k < nk) {
+            for (int64_t b = threadIdx.x; b < na; b += blockDim.x) v += pa[k * na + b];
+            for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) v += pb[k * nb + b];
+        }
+        long long t = block_sum(v, lds);
+        if (threadIdx.x == 0) rec[k] = t;
     }
 }
 
@@ -136,7 +158,7 @@ template <int D, int MODEL, int MODE>
 __global__ __launch_bounds__(256) void site_pass_generic(const uint8_t *__restrict__ oth, uint8_t *__restrict__ own,
                                                          Geom g, int c, uint32_t seed, uint32_t skey, int q,
                                                          const uint32_t *__restrict__ table,
-                                                         long long *__restrict__ partials, int meas,
+                                                         MeasArgs m,
                                                          int64_t work, int64_t span) {
     __shared__ long long lds[16];
     constexpr int NN = 2 * D;
@@ -187,7 +209,7 @@ __global__ __launch_bounds__(256) void site_pass_generic(const uint8_t *__restri
                 else nxt = hit ? 0 : 1;
                 own[rowbase + i] = (uint8_t)nxt;
             }
-            if (meas) {
+            if (m.on) {
                 counts[nxt] += 1;
                 if (c == 1) bond += (long long)((1 - 2 * nxt) * (NN - 2 * d));
             }
@@ -232,13 +254,13 @@ __global__ __launch_bounds__(256) void site_pass_generic(const uint8_t *__restri
                     }
                 own[rowbase + i] = (uint8_t)nxt;
             }
-            if (meas) {
+            if (m.on) {
                 counts[nxt] += 1;
                 if (c == 1) bond += (long long)n[nxt];
             }
         }
     }
-    if (meas) write_partials(partials, bond, counts, q, lds);
+    if (m.on) emit_partials(m, bond, counts, q, lds);
 }
 
 // ------------------------------------------------------ SWAR Ising pass ---
@@ -251,7 +273,7 @@ template <int D, int MODE>
 __global__ __launch_bounds__(256) void ising_pass_swar(const uint32_t *__restrict__ oth, uint32_t *__restrict__ own,
                                                        SwarGeom sg, int c, uint32_t seed, uint32_t skey,
                                                        const uint32_t *__restrict__ table,
-                                                       long long *__restrict__ partials, int meas, int64_t work,
+                                                       MeasArgs m, int64_t work,
                                                        int64_t span) {
     constexpr int NN = 2 * D;
     constexpr int K = NN + 1;
@@ -324,7 +346,7 @@ __global__ __launch_bounds__(256) void ising_pass_swar(const uint32_t *__restric
             }
             st32(own, rb + i4, out);
         }
-        if (meas) {
+        if (m.on) {
             down += __popc(out);
             total += 4;
             if (c == 1) {
@@ -337,9 +359,9 @@ __global__ __launch_bounds__(256) void ising_pass_swar(const uint32_t *__restric
             }
         }
     }
-    if (meas) {
+    if (m.on) {
         int counts[2] = {total - down, down};
-        write_partials(partials, (long long)bond, counts, 2, lds);
+        emit_partials(m, (long long)bond, counts, 2, lds);
     }
 }
 
@@ -377,7 +399,7 @@ template <int D, int MODE>
 __global__ __launch_bounds__(256) void ising_pass_x16(const uint4 *__restrict__ oth, uint4 *__restrict__ own,
                                                       SwarGeom sg, int c, uint32_t seed, uint32_t skey,
                                                       const uint32_t *__restrict__ table,
-                                                      long long *__restrict__ partials, int meas, int64_t work,
+                                                      MeasArgs m, int64_t work,
                                                       int64_t span) {
     constexpr int NN = 2 * D;
     constexpr int K = NN + 1;
@@ -474,7 +496,7 @@ __global__ __launch_bounds__(256) void ising_pass_x16(const uint4 *__restrict__ 
             out.w = decide4(s.w, ds.w, 12);
             st32(own, w, out);
         }
-        if (meas) {
+        if (m.on) {
             down += __popc(out.x) + __popc(out.y) + __popc(out.z) + __popc(out.w);
             total += 16;
             if (c == 1) {
@@ -491,9 +513,9 @@ __global__ __launch_bounds__(256) void ising_pass_x16(const uint4 *__restrict__ 
             }
         }
     }
-    if (meas) {
+    if (m.on) {
         int counts[2] = {total - down, down};
-        write_partials(partials, (long long)bond, counts, 2, lds);
+        emit_partials(m, (long long)bond, counts, 2, lds);
     }
 }
 
@@ -523,6 +545,11 @@ void launch_finalize_record(const long long *pa, int64_t na, const long long *pb
     finalize_record_kernel<<<dim3(1), dim3(1024), 0, s>>>(pa, na, pb, nb, rec);
 }
 
+void launch_finalize_columns(const long long *pa, int64_t na, const long long *pb, int64_t nb, int nk, long long *rec,
+                             hipStream_t s) {
+    finalize_columns_kernel<<<dim3(1), dim3(1024), 0, s>>>(pa, na, pb, nb, nk, rec);
+}
+
 static int64_t pass_work(const SitePassArgs &a) { return (int64_t)a.g.R * a.g.H / a.vec; }
 
 int site_pass_blocks(const SitePassArgs &a) {
@@ -539,7 +566,7 @@ static void dispatch_generic(const SitePassArgs &a, int blocks, int64_t work, in
     uint8_t *own = a.own;
 #define PLQ_GEN(MODEL, MODE)                                                                                    \
     site_pass_generic<D, MODEL, MODE><<<grid, block, 0, a.stream>>>(oth, own, a.g, a.color, a.seed, a.skey, a.q, \
-                                                                    a.table, a.partials, a.meas, work, span)
+                                                                    a.table, a.m, work, span)
     if (a.model == 0) {
         if (a.mode == 0) PLQ_GEN(0, 0);
         else if (a.mode == 1) PLQ_GEN(0, 1);
@@ -559,7 +586,7 @@ static void dispatch_x4(const SitePassArgs &a, int blocks, int64_t work, int64_t
     uint32_t *own = (uint32_t *)a.own;
 #define PLQ_X4(MODE)                                                                                         \
     ising_pass_swar<D, MODE><<<grid, block, 0, a.stream>>>(oth, own, a.sg, a.color, a.seed, a.skey, a.table, \
-                                                           a.partials, a.meas, work, span)
+                                                           a.m, work, span)
     if (a.mode == 0) PLQ_X4(0);
     else if (a.mode == 1) PLQ_X4(1);
     else PLQ_X4(2);
@@ -573,7 +600,7 @@ static void dispatch_x16(const SitePassArgs &a, int blocks, int64_t work, int64_
     uint4 *own = (uint4 *)a.own;
 #define PLQ_X16(MODE)                                                                                       \
     ising_pass_x16<D, MODE><<<grid, block, 0, a.stream>>>(oth, own, a.sg, a.color, a.seed, a.skey, a.table, \
-                                                          a.partials, a.meas, work, span)
+                                                          a.m, work, span)
     if (a.mode == 0) PLQ_X16(0);
     else if (a.mode == 1) PLQ_X16(1);
     else PLQ_X16(2);
