@@ -63,7 +63,8 @@ struct plq_chain {
     // local fields
     plq::Geom g{};
     plq::SwarGeom sg{};
-    int vec = 1;  // sites per lane of the site kernel
+    int vec = 1;                           // sites per lane of the site kernel
+    int64_t pass_blocks = plq::MAX_PASS_BLOCKS;  // grid cap of a colour pass
     uint8_t *col[2] = {nullptr, nullptr};  // site colours
     uint8_t *links[2 * plq::MAX_D] = {nullptr};
     uint8_t **d_links = nullptr;           // device copy of the link array pointers
@@ -206,7 +207,7 @@ plq::SitePassArgs site_args(plq_chain *c, int color, uint32_t sweep, int mode, i
     a.skey = sweep * plq::GOLDEN;
     a.q = c->q;
     a.table = c->d_table;
-    a.max_blocks = plq::MAX_PASS_BLOCKS;
+    a.max_blocks = c->pass_blocks;
     a.stream = c->stream;
     a.m.part = c->d_partials + (color ? plq::MAX_PASS_BLOCKS * plq::REC_K : 0);
     a.m.on = meas;
@@ -468,6 +469,13 @@ int plq_chain_create_local(const plq_local_desc *d, const uint8_t *start, size_t
         int v = atoi(cap);
         if (v == 1 || v == 4 || v == 16)
             while (c->vec > v) c->vec /= 4;
+    }
+    // Grid cap of a colour pass (8..2048 blocks of 256 lanes); lanes walk
+    // their block's span, so a smaller grid gives each lane several chunks
+    // to pipeline.  PLQ_PASS_BLOCKS overrides the default.
+    if (const char *pb = getenv("PLQ_PASS_BLOCKS")) {
+        long v = atol(pb);
+        if (v >= 8 && v <= plq::MAX_PASS_BLOCKS) c->pass_blocks = v;
     }
     if (c->vec > 1) {
         c->sg.lh4 = ilog2(d->shape[0] / 8);

@@ -395,11 +395,76 @@ __device__ __forceinline__ uint32_t spread4(uint32_t x) { return (x * 0x00204081
 //     constant 2k is xored in;
 //   - the 16 decisions are collected as bits and spread back to bytes with
 //     one multiply per dword.
+// Everything a 16-site chunk reads, loaded ahead of its use so that a lane
+// computes one chunk while the next chunk's loads are in flight.
+template <int D>
+struct X16Chunk {
+    uint4 a;                        // the other colour's 16 bytes at this position
+    uint4 s;                        // this colour's 16 states
+    uint4 rows[D > 1 ? 2 * (D - 1) : 1];  // the other colour's bytes in the neighbour rows
+    uint32_t adj;                   // the other colour's dword the axis-0 shift pulls a byte from
+    uint32_t site0;                 // lattice index of the chunk's first site
+    uint32_t p;                     // x0 parity of this colour in the row
+};
+
+template <int D>
+__device__ __forceinline__ void x16_load(const uint4 *__restrict__ oth, const uint4 *__restrict__ own,
+                                         const SwarGeom &sg, uint32_t c, uint32_t lh16, uint32_t cmask, uint32_t w,
+                                         X16Chunk<D> &k) {
+    const uint32_t r = w >> lh16;
+    const uint32_t ch = w & cmask;
+    const uint32_t x1 = r & (sg.L1 - 1u);
+    uint32_t x2 = 0, x3 = 0;
+    if (D >= 3) x2 = (r >> sg.ll1) & (sg.L2 - 1u);
+    if (D >= 4) x3 = r >> (sg.ll1 + sg.ll2);
+    const uint32_t p = (x1 + x2 + x3 + c) & 1u;
+    const uint32_t rc = r << lh16;  // chunk index of the row start
+    k.p = p;
+    k.site0 = (r << (sg.lh4 + 3u)) + (ch << 5) + p;
+    k.a = ld32(oth, w);
+    // p = 1 needs the first byte of the next chunk, p = 0 the last byte of the
+    // previous one (wrapping at the row ends)
+    const uint32_t adj_dw = p ? ((rc + ((ch + 1u) & cmask)) << 2) : (((rc + ((ch - 1u) & cmask)) << 2) + 3u);
+    k.adj = ld32(reinterpret_cast<const uint32_t *>(oth), adj_dw);
+    if (D >= 2) {
+        k.rows[0] = ld32(oth, ((((x1 + 1u == sg.L1) ? r - (sg.L1 - 1u) : r + 1u)) << lh16) + ch);
+        k.rows[1] = ld32(oth, ((((x1 == 0u) ? r + (sg.L1 - 1u) : r - 1u)) << lh16) + ch);
+    }
+    if (D >= 3) {
+        const uint32_t st = sg.L1, span2 = (sg.L2 - 1u) * sg.L1;
+        k.rows[2] = ld32(oth, (((x2 + 1u == sg.L2) ? r - span2 : r + st) << lh16) + ch);
+        k.rows[3] = ld32(oth, (((x2 == 0u) ? r + span2 : r - st) << lh16) + ch);
+    }
+    if (D >= 4) {
+        const uint32_t st = sg.L1 * sg.L2, span3 = (sg.L3 - 1u) * st;
+        k.rows[4] = ld32(oth, (((x3 + 1u == sg.L3) ? r - span3 : r + st) << lh16) + ch);
+        k.rows[5] = ld32(oth, (((x3 == 0u) ? r + span3 : r - st) << lh16) + ch);
+    }
+    k.s = ld32(own, w);
+}
+
+// Sixteen same-colour sites per lane: one 16-byte chunk of a colour row.
+// Requires power-of-two extents with L0 >= 32.  Per site the work is one
+// LDS threshold read, the keyed draw and one compare; everything else is
+// done per dword (4 sites) or per chunk (16 sites):
+//   - the "down" neighbour counts of 4 sites are one dword sum;
+//   - the class byte of each site, scaled to an LDS byte address, is built
+//     for 4 sites at once ((dsum << 3) | (state << 2) indexes a copy of the
+//     table reordered to [d][state]);
+//   - the site indices of a chunk are site0 | 2k (site0 has bits 1..4 clear),
+//     so the first xor-shift of the first hash round, (x ^ x >> 16) with
+//     x = seed ^ site, is computed once per chunk and per site only the
+//     constant 2k is xored in;
+//   - the 16 decisions are collected as bits and spread back to bytes with
+//     one multiply per dword.
+// A lane walks its block's chunks with a stride of blockDim.x.  (Loading the
+// next chunk ahead of the current one's compute was measured slower: it
+// raises the register count from ~50 to 83 and occupancy from 8 to 5 waves,
+// and the pass is bound by VALU issue, not by load latency.)
 template <int D, int MODE>
 __global__ __launch_bounds__(256) void ising_pass_x16(const uint4 *__restrict__ oth, uint4 *__restrict__ own,
                                                       SwarGeom sg, int c, uint32_t seed, uint32_t skey,
-                                                      const uint32_t *__restrict__ table,
-                                                      MeasArgs m, int64_t work,
+                                                      const uint32_t *__restrict__ table, MeasArgs m, int64_t work,
                                                       int64_t span) {
     constexpr int NN = 2 * D;
     constexpr int K = NN + 1;
@@ -416,78 +481,54 @@ __global__ __launch_bounds__(256) void ising_pass_x16(const uint4 *__restrict__ 
     if (end > (uint32_t)work) end = (uint32_t)work;
     int bond = 0;
     int down = 0, total = 0;
-    const uint32_t lh16 = sg.lh4 - 2u;        // log2(chunks per colour row)
+    const uint32_t lh16 = sg.lh4 - 2u;  // log2(chunks per colour row)
     const uint32_t cmask = (1u << lh16) - 1u;
     const char *Tb = reinterpret_cast<const char *>(T);
+    const uint32_t uc = (uint32_t)c;
     for (uint32_t w = begin + threadIdx.x; w < end; w += blockDim.x) {
-        uint32_t r = w >> lh16;
-        uint32_t ch = w & cmask;
-        uint32_t x1 = r & (sg.L1 - 1u);
-        uint32_t x2 = 0, x3 = 0;
-        if (D >= 3) x2 = (r >> sg.ll1) & (sg.L2 - 1u);
-        if (D >= 4) x3 = r >> (sg.ll1 + sg.ll2);
-        uint32_t p = (x1 + x2 + x3 + (uint32_t)c) & 1u;
-        const uint32_t rc = r << lh16;  // chunk index of the row start
-        uint4 a = ld32(oth, w);
-        // the byte beyond the chunk on the side the shift pulls from
-        uint32_t adj_dw = p ? ((rc + ((ch + 1u) & cmask)) << 2) : (((rc + ((ch - 1u) & cmask)) << 2) + 3u);
-        uint32_t adj = ld32(reinterpret_cast<const uint32_t *>(oth), adj_dw);
-        uint4 sh;
-        if (p) {
-            sh.x = __builtin_amdgcn_alignbyte(a.y, a.x, 1);
-            sh.y = __builtin_amdgcn_alignbyte(a.z, a.y, 1);
-            sh.z = __builtin_amdgcn_alignbyte(a.w, a.z, 1);
-            sh.w = __builtin_amdgcn_alignbyte(adj, a.w, 1);
-        } else {
-            sh.x = __builtin_amdgcn_alignbyte(a.x, adj, 3);
-            sh.y = __builtin_amdgcn_alignbyte(a.y, a.x, 3);
-            sh.z = __builtin_amdgcn_alignbyte(a.z, a.y, 3);
-            sh.w = __builtin_amdgcn_alignbyte(a.w, a.z, 3);
-        }
+        X16Chunk<D> cur;
+        x16_load<D>(oth, own, sg, uc, lh16, cmask, w, cur);
+        const uint4 a = cur.a;
+        const uint32_t p = cur.p;
+        // axis-0 neighbours: the same bytes shifted one site forward (p = 1)
+        // or back (p = 0); selected without branching since p alternates
+        // from row to row inside a wave
+        const uint32_t sel = p ? 1u : 3u;
         uint4 ds;
-        ds.x = a.x + sh.x;
-        ds.y = a.y + sh.y;
-        ds.z = a.z + sh.z;
-        ds.w = a.w + sh.w;
-        auto add_row = [&](uint32_t rr) {
-            uint4 b = ld32(oth, (rr << lh16) + ch);
-            ds.x += b.x;
-            ds.y += b.y;
-            ds.z += b.z;
-            ds.w += b.w;
-        };
-        if (D >= 2) {
-            add_row((x1 + 1u == sg.L1) ? r - (sg.L1 - 1u) : r + 1u);
-            add_row((x1 == 0u) ? r + (sg.L1 - 1u) : r - 1u);
+        ds.x = a.x + __builtin_amdgcn_alignbyte(p ? a.y : a.x, p ? a.x : cur.adj, sel);
+        ds.y = a.y + __builtin_amdgcn_alignbyte(p ? a.z : a.y, p ? a.y : a.x, sel);
+        ds.z = a.z + __builtin_amdgcn_alignbyte(p ? a.w : a.z, p ? a.z : a.y, sel);
+        ds.w = a.w + __builtin_amdgcn_alignbyte(p ? cur.adj : a.w, p ? a.w : a.z, sel);
+#pragma unroll
+        for (int j = 0; j < (D > 1 ? 2 * (D - 1) : 0); ++j) {
+            ds.x += cur.rows[j].x;
+            ds.y += cur.rows[j].y;
+            ds.z += cur.rows[j].z;
+            ds.w += cur.rows[j].w;
         }
-        if (D >= 3) {
-            uint32_t st = sg.L1, span2 = (sg.L2 - 1u) * sg.L1;
-            add_row((x2 + 1u == sg.L2) ? r - span2 : r + st);
-            add_row((x2 == 0u) ? r + span2 : r - st);
-        }
-        if (D >= 4) {
-            uint32_t st = sg.L1 * sg.L2, span3 = (sg.L3 - 1u) * st;
-            add_row((x3 + 1u == sg.L3) ? r - span3 : r + st);
-            add_row((x3 == 0u) ? r + span3 : r - st);
-        }
-        uint4 s = ld32(reinterpret_cast<const uint4 *>(own), w);
+        const uint4 s = cur.s;
         uint4 out = s;
         if (MODE != 2) {
-            const uint32_t site0 = (r << (sg.lh4 + 3u)) + (ch << 5) + p;
-            const uint32_t bx = seed ^ site0;
+            const uint32_t bx = seed ^ cur.site0;
             const uint32_t v0 = bx ^ (bx >> 16);
+            // m < t <=> m - t is negative (both below 2^31); the four sign
+            // bytes are gathered into one dword with two byte permutes, and
+            // their sign bits become the per-site hit bytes.
             auto decide4 = [&](uint32_t sw, uint32_t dw, uint32_t kbase) -> uint32_t {
                 const uint32_t addr4 = (dw << 3) | (sw << 2);
-                uint32_t hb = 0;
+                uint32_t diff[4];
 #pragma unroll
-                for (int k = 3; k >= 0; --k) {
+                for (int k = 0; k < 4; ++k) {
                     uint32_t addr = (addr4 >> (8 * k)) & 0xffu;
                     uint32_t t = *reinterpret_cast<const uint32_t *>(Tb + addr);
                     uint32_t h1 = lowbias32_tail(v0 ^ (2u * (kbase + (uint32_t)k)));
-                    uint32_t m = lowbias32(h1 ^ skey) >> 8;
-                    hb = hb + hb + (m < t ? 1u : 0u);
+                    uint32_t mm = lowbias32(h1 ^ skey) >> 8;
+                    diff[k] = mm - t;
                 }
-                uint32_t bits = spread4(hb);
+                // byte 3 of diff[0..3] -> bytes 0..3
+                uint32_t lo = __builtin_amdgcn_perm(diff[1], diff[0], 0x0c0c0703u);
+                uint32_t hi = __builtin_amdgcn_perm(diff[3], diff[2], 0x07030c0cu);
+                uint32_t bits = ((lo | hi) >> 7) & 0x01010101u;
                 return MODE == 0 ? (sw ^ bits) : (bits ^ 0x01010101u);
             };
             out.x = decide4(s.x, ds.x, 0);
