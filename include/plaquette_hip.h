@@ -153,6 +153,65 @@ int plq_chain_cluster_stats(plq_chain *chain, plq_cluster_stats *out);
 int plq_chain_set_pass_cap(plq_chain *chain, uint64_t cap);
 int plq_chain_destroy(plq_chain *chain);
 
+/* ---------------------------------------------------------------------------
+ * Batched Z2 gauge ensembles (no reference counterpart: the reference builds
+ * one GpuChain per lattice, gauge/gpu.rs:54-110).  Many replicas of uneven
+ * shapes advance together, one launch per colour pass for all of them; every
+ * replica keeps the reference's link checkerboard and keyed draws
+ * (link_schedule.wgsl:23-40, gauge/model.wgsl:136-147), keyed by its own seed.
+ * Every `measure_every` sweeps the plaquette sums of all replicas are recorded
+ * and, with `exchange` set, one replica-exchange round swaps inverse
+ * temperatures between replicas of equal shape whose betas are neighbours:
+ * round n pairs the k-th and (k+1)-th smallest beta for k = n mod 2,
+ * n mod 2 + 2, ...; the pair swaps iff delta = (beta_k+1 - beta_k) * J * (P_a -
+ * P_b) >= 0 or m < T, m the keyed draw (fold_seed(exchange_seed) ^ 0x68e31da4,
+ * global pair index, round) and T the f64 threshold of exp(delta).  A block of
+ * measure_every sweeps (with its measurement and exchange) can be replayed as
+ * one captured hipGraph.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    int32_t dim;                 /* 2..4 */
+    int32_t n_replicas;
+    int32_t kernel;              /* PLQ_KERNEL_* */
+    int32_t profile;             /* PLQ_PROFILE_* */
+    const int64_t *shapes;       /* n_replicas x dim, every extent even */
+    const double *betas;         /* replica r starts at betas[r] */
+    const uint64_t *seeds;       /* per replica */
+    double j;
+    uint64_t measure_every;      /* sweeps between recorded measurements; 0 = none */
+    int32_t exchange;            /* 1 = an exchange round after every measurement */
+    uint64_t exchange_seed;
+    uint64_t record_capacity;    /* measurements kept (ring) */
+} plq_ensemble_desc;
+
+typedef struct plq_ensemble plq_ensemble;
+
+/* `start` concatenates the replicas' lattice-order link states
+ * (replica r: n_sites_r * dim bytes, link = site * dim + dir). */
+int plq_ensemble_create(const plq_ensemble_desc *desc, const uint8_t *start, size_t n, plq_ensemble **out);
+int plq_ensemble_advance(plq_ensemble *e, uint64_t sweeps);
+int plq_ensemble_read(plq_ensemble *e, uint8_t *out, size_t n);
+/* Current signed plaquette sum of every replica (synchronises). */
+int plq_ensemble_plaquette_sums(plq_ensemble *e, int64_t *sums, size_t n);
+/* Current inverse temperature of every replica. */
+int plq_ensemble_betas(plq_ensemble *e, double *betas, size_t n);
+/* Recorded measurements, oldest first: sums[k * n_replicas + r] and the beta
+ * slot (index into the descriptor's betas) replica r held when measured. */
+int plq_ensemble_series(plq_ensemble *e, int64_t *sums, uint32_t *slots, uint64_t capacity, uint64_t *n_out);
+/* Accepted swaps and attempts per exchange pair (pairs in group order). */
+int plq_ensemble_exchange_counts(plq_ensemble *e, uint64_t *accepted, uint64_t *attempted, size_t npairs,
+                                 size_t *npairs_out);
+/* Launch shape: lanes per block, links per lane, hipGraph replay on/off. */
+int plq_ensemble_set_launch(plq_ensemble *e, int32_t block_size, int32_t per_lane, int32_t use_graph);
+/* Time the candidate launch shapes on a scratch copy of the state and keep
+ * the fastest; the chain's state and counters are left as they were. */
+int plq_ensemble_tune(plq_ensemble *e, int32_t *block_size, int32_t *per_lane, int32_t *use_graph,
+                      double *best_us_per_sweep);
+int plq_ensemble_sweeps_done(plq_ensemble *e, uint64_t *out);
+int plq_ensemble_stream(plq_ensemble *e, void **stream);
+int plq_ensemble_sync(plq_ensemble *e);
+int plq_ensemble_destroy(plq_ensemble *e);
+
 /* Host-side helpers shared with the device path, exposed for tests. */
 uint32_t plq_fold_seed(uint64_t seed);
 /* The integer acceptance table the local kernels compare the 24-bit draw

@@ -68,6 +68,22 @@ class Derived(C.Structure):
     _fields_ = [("value", C.c_double), ("stderr", C.c_double), ("n_blocks", C.c_int64)]
 
 
+class EnsembleDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("n_replicas", C.c_int32),
+        ("kernel", C.c_int32),
+        ("profile", C.c_int32),
+        ("shapes", C.POINTER(C.c_int64)),
+        ("betas", C.POINTER(C.c_double)),
+        ("seed32s", C.POINTER(C.c_uint32)),
+        ("j", C.c_double),
+        ("measure_every", C.c_uint64),
+        ("exchange", C.c_int32),
+        ("ex_seed32", C.c_uint32),
+    ]
+
+
 class Pcg64(C.Structure):
     _fields_ = [("state_lo", C.c_uint64), ("state_hi", C.c_uint64), ("inc_lo", C.c_uint64), ("inc_hi", C.c_uint64)]
 
@@ -108,6 +124,10 @@ def lib():
             "orc_pcg64_seed_from_u64": (None, [P(Pcg64), C.c_uint64]),
             "orc_pcg64_next_u64": (C.c_uint64, [P(Pcg64)]),
             "orc_pcg64_next_f64": (C.c_double, [P(Pcg64)]),
+            "orc_ensemble_init": (C.c_int, [P(EnsembleDesc), P(C.c_uint32), P(C.c_uint32)]),
+            "orc_ensemble_sweeps": (C.c_int, [P(EnsembleDesc), P(C.c_uint8), P(C.c_uint32), P(C.c_uint32),
+                                              P(C.c_uint64), P(C.c_uint64), C.c_uint64, P(C.c_int64),
+                                              P(C.c_uint32), C.c_uint64, P(C.c_uint64), P(C.c_uint64)]),
             "orc_pmode_ising_sweeps": (C.c_double, [C.c_int32, P(C.c_int64), C.c_int32, C.c_double, C.c_double,
                                                     C.c_double, C.c_uint64, P(C.c_uint8), C.c_uint32]),
         }
@@ -319,3 +339,76 @@ def pmode_ising_sweeps(shape, kernel, beta, j, h, seed, state, n_sweeps):
     st = np.ascontiguousarray(state, dtype=np.uint8).copy()
     secs = lib().orc_pmode_ising_sweeps(len(shape), _shape(shape), kernel, beta, j, h, seed, _u8(st), n_sweeps)
     return st, secs
+
+
+# ------------------------------------------------------------- ensemble ---
+
+EXCHANGE_SALT = 0x68E31DA4
+
+
+class Ensemble:
+    """CPU restatement of a batched Z2 ensemble with replica exchange."""
+
+    def __init__(self, dim, shapes, betas, seeds, kernel=KERNEL_HEAT_BATH, profile=PROFILE_F64, j=1.0,
+                 measure_every=10, exchange=True, exchange_seed=0, state=None):
+        self.R = len(shapes)
+        self._shapes = np.ascontiguousarray(np.array(shapes, dtype=np.int64).reshape(self.R, dim))
+        self._betas = np.ascontiguousarray(betas, dtype=np.float64)
+        self._seeds = np.array([fold_seed(s) for s in seeds], dtype=np.uint32)
+        d = EnsembleDesc()
+        d.dim = dim
+        d.n_replicas = self.R
+        d.kernel = kernel
+        d.profile = profile
+        d.shapes = self._shapes.ctypes.data_as(C.POINTER(C.c_int64))
+        d.betas = self._betas.ctypes.data_as(C.POINTER(C.c_double))
+        d.seed32s = self._seeds.ctypes.data_as(C.POINTER(C.c_uint32))
+        d.j = j
+        d.measure_every = measure_every
+        d.exchange = int(exchange)
+        d.ex_seed32 = fold_seed(exchange_seed) ^ EXCHANGE_SALT
+        self.desc = d
+        self.dim = dim
+        self.sizes = [int(np.prod(s)) * dim for s in self._shapes]
+        self.state = np.ascontiguousarray(state, dtype=np.uint8).copy() if state is not None else \
+            np.zeros(sum(self.sizes), np.uint8)
+        self.slot_of = np.zeros(self.R, np.uint32)
+        self.rep_at = np.zeros(self.R, np.uint32)
+        lib().orc_ensemble_init(C.byref(d), self.slot_of.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                self.rep_at.ctypes.data_as(C.POINTER(C.c_uint32)))
+        self.sweeps_done = C.c_uint64(0)
+        self.rounds_done = C.c_uint64(0)
+        self.records = []
+        self.slots = []
+        self.n_pairs = sum(max(0, c - 1) for c in self._group_sizes())
+        self.accepted = np.zeros(max(1, self.n_pairs), np.uint64)
+
+    def _group_sizes(self):
+        seen = []
+        counts = []
+        for s in self._shapes.tolist():
+            if s in seen:
+                counts[seen.index(s)] += 1
+            else:
+                seen.append(s)
+                counts.append(1)
+        return counts
+
+    def advance(self, sweeps):
+        cap = sweeps + 1
+        rec = np.zeros(cap * self.R, np.int64)
+        slots = np.zeros(cap * self.R, np.uint32)
+        n = C.c_uint64(0)
+        P = C.POINTER
+        lib().orc_ensemble_sweeps(C.byref(self.desc), _u8(self.state), self.slot_of.ctypes.data_as(P(C.c_uint32)),
+                                  self.rep_at.ctypes.data_as(P(C.c_uint32)), C.byref(self.sweeps_done),
+                                  C.byref(self.rounds_done), sweeps, rec.ctypes.data_as(P(C.c_int64)),
+                                  slots.ctypes.data_as(P(C.c_uint32)), cap, C.byref(n),
+                                  self.accepted.ctypes.data_as(P(C.c_uint64)))
+        for k in range(n.value):
+            self.records.append(rec[k * self.R:(k + 1) * self.R].copy())
+            self.slots.append(slots[k * self.R:(k + 1) * self.R].copy())
+        return self
+
+    def betas_now(self):
+        return self._betas[self.slot_of]

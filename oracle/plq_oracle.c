@@ -828,3 +828,132 @@ double orc_pmode_ising_sweeps(int32_t dim, const int64_t *shape, int32_t kernel,
     free(parity);
     return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
 }
+
+/* ------------------------------------------------------------ ensemble --- */
+
+typedef struct {
+    int n_groups;
+    int group_of[4096];
+    int g_first[4096];   /* offset of group g in rep_at */
+    int g_size[4096];
+} ens_groups;
+
+static int same_shape(const orc_ensemble_desc *d, int a, int b) {
+    for (int k = 0; k < d->dim; ++k)
+        if (d->shapes[a * d->dim + k] != d->shapes[b * d->dim + k]) return 0;
+    return 1;
+}
+
+/* groups by first appearance of a shape; members ordered by ascending beta
+ * (stable), giving the pair order every exchange round walks */
+static void ens_build(const orc_ensemble_desc *d, ens_groups *G, uint32_t *rep_at) {
+    int R = d->n_replicas;
+    G->n_groups = 0;
+    int placed = 0;
+    for (int r = 0; r < R; ++r) {
+        int g = -1;
+        for (int s = 0; s < r; ++s)
+            if (same_shape(d, r, s)) { g = G->group_of[s]; break; }
+        if (g < 0) { g = G->n_groups++; G->g_size[g] = 0; }
+        G->group_of[r] = g;
+        G->g_size[g] += 1;
+    }
+    for (int g = 0; g < G->n_groups; ++g) {
+        G->g_first[g] = placed;
+        int k = 0;
+        for (int r = 0; r < R; ++r)
+            if (G->group_of[r] == g) rep_at[placed + k++] = (uint32_t)r;
+        /* stable insertion sort by beta */
+        for (int i = 1; i < k; ++i) {
+            uint32_t v = rep_at[placed + i];
+            int jx = i - 1;
+            while (jx >= 0 && d->betas[rep_at[placed + jx]] > d->betas[v]) {
+                rep_at[placed + jx + 1] = rep_at[placed + jx];
+                --jx;
+            }
+            rep_at[placed + jx + 1] = v;
+        }
+        placed += k;
+    }
+}
+
+int orc_ensemble_init(const orc_ensemble_desc *d, uint32_t *slot_of, uint32_t *rep_at) {
+    if (d->n_replicas < 1 || d->n_replicas > 4096) return -1;
+    ens_groups G;
+    ens_build(d, &G, rep_at);
+    for (int r = 0; r < d->n_replicas; ++r) slot_of[r] = (uint32_t)r;
+    return 0;
+}
+
+int orc_ensemble_sweeps(const orc_ensemble_desc *d, uint8_t *state, uint32_t *slot_of, uint32_t *rep_at,
+                        uint64_t *sweeps_done, uint64_t *rounds_done, uint64_t n_sweeps, int64_t *rec_sums,
+                        uint32_t *rec_slots, uint64_t rec_cap, uint64_t *n_rec, uint64_t *accepted) {
+    int R = d->n_replicas;
+    if (R < 1 || R > 4096) return -1;
+    ens_groups G;
+    uint32_t order0[4096];
+    ens_build(d, &G, order0); /* group layout only; rep_at carries the current order */
+    size_t *off = (size_t *)malloc(sizeof(size_t) * (size_t)R);
+    size_t o = 0;
+    for (int r = 0; r < R; ++r) {
+        off[r] = o;
+        o += (size_t)orc_n_sites(d->dim, d->shapes + r * d->dim) * (size_t)d->dim;
+    }
+    int64_t *sums = (int64_t *)malloc(sizeof(int64_t) * (size_t)R);
+    for (uint64_t t = 0; t < n_sweeps; ++t) {
+        uint32_t sweep = (uint32_t)*sweeps_done;
+        for (int r = 0; r < R; ++r) {
+            orc_local_desc ld;
+            memset(&ld, 0, sizeof ld);
+            ld.dim = d->dim;
+            for (int k = 0; k < d->dim; ++k) ld.shape[k] = d->shapes[r * d->dim + k];
+            ld.model = ORC_MODEL_Z2;
+            ld.q = 2;
+            ld.kernel = d->kernel;
+            ld.profile = d->profile;
+            ld.beta = d->betas[slot_of[r]];
+            ld.j = d->j;
+            ld.seed32 = d->seed32s[r];
+            orc_local_sweeps(&ld, state + off[r], sweep, 1);
+        }
+        *sweeps_done += 1;
+        if (!d->measure_every || *sweeps_done % d->measure_every != 0) continue;
+        for (int r = 0; r < R; ++r) sums[r] = orc_plaquette_sum(d->dim, d->shapes + r * d->dim, state + off[r]);
+        if (rec_sums && n_rec && *n_rec < rec_cap) {
+            for (int r = 0; r < R; ++r) {
+                rec_sums[*n_rec * R + r] = sums[r];
+                if (rec_slots) rec_slots[*n_rec * R + r] = slot_of[r];
+            }
+            *n_rec += 1;
+        }
+        if (!d->exchange) continue;
+        uint32_t round = (uint32_t)*rounds_done;
+        uint32_t p = 0;
+        for (int g = 0; g < G.n_groups; ++g) {
+            int first = G.g_first[g];
+            for (int k = 0; k + 1 < G.g_size[g]; ++k, ++p) {
+                if ((uint32_t)(k % 2) != round % 2) continue;
+                uint32_t a = rep_at[first + k], b = rep_at[first + k + 1];
+                /* betas of the k-th and (k+1)-th smallest of the group */
+                double bk = d->betas[order0[first + k]], bk1 = d->betas[order0[first + k + 1]];
+                double s = (bk1 - bk) * d->j;
+                int64_t x = sums[a] - sums[b];
+                double delta = s * (double)x;
+                int swap = delta >= 0.0;
+                if (!swap) swap = u64_of(orc_keyed_bits(d->ex_seed32, p, round)) < exp(delta);
+                if (swap) {
+                    rep_at[first + k] = b;
+                    rep_at[first + k + 1] = a;
+                    uint32_t sa = slot_of[a];
+                    slot_of[a] = slot_of[b];
+                    slot_of[b] = sa;
+                    if (accepted) accepted[p] += 1;
+                }
+            }
+        }
+        *rounds_done += 1;
+    }
+    free(off);
+    free(sums);
+    return 0;
+}

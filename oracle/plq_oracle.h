@@ -114,6 +114,26 @@ int orc_cluster_bonds(const orc_cluster_desc *d, const uint8_t *state, uint32_t 
 /* Connected components of the open bonds, canonical: root = min site index. */
 int orc_components(int32_t dim, const int64_t *shape, const uint8_t *bonds, int64_t *roots);
 
+/* Batched Z2 replicas with replica exchange (no reference counterpart; the
+ * semantics are those documented for plq_ensemble_* in plaquette_hip.h).
+ * state concatenates the replicas' lattice-order links; slot_of / rep_at /
+ * counters carry the exchange state between calls (rep_at in group order:
+ * groups by first appearance of a shape, members by ascending beta). */
+typedef struct {
+    int32_t dim, n_replicas, kernel, profile;
+    const int64_t *shapes;
+    const double *betas;
+    const uint32_t *seed32s;
+    double j;
+    uint64_t measure_every;
+    int32_t exchange;
+    uint32_t ex_seed32;   /* fold_seed(exchange_seed) ^ 0x68e31da4 */
+} orc_ensemble_desc;
+int orc_ensemble_init(const orc_ensemble_desc *d, uint32_t *slot_of, uint32_t *rep_at);
+int orc_ensemble_sweeps(const orc_ensemble_desc *d, uint8_t *state, uint32_t *slot_of, uint32_t *rep_at,
+                        uint64_t *sweeps_done, uint64_t *rounds_done, uint64_t n_sweeps, int64_t *rec_sums,
+                        uint32_t *rec_slots, uint64_t rec_cap, uint64_t *n_rec, uint64_t *accepted);
+
 /* observables (integer sums) */
 void orc_ising_sums(int32_t dim, const int64_t *shape, const uint8_t *state,
                     int64_t *bond_sum, int64_t *spin_sum);

@@ -138,6 +138,23 @@ class ClusterStats(C.Structure):
     ]
 
 
+class EnsembleDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("n_replicas", C.c_int32),
+        ("kernel", C.c_int32),
+        ("profile", C.c_int32),
+        ("shapes", C.POINTER(C.c_int64)),
+        ("betas", C.POINTER(C.c_double)),
+        ("seeds", C.POINTER(C.c_uint64)),
+        ("j", C.c_double),
+        ("measure_every", C.c_uint64),
+        ("exchange", C.c_int32),
+        ("exchange_seed", C.c_uint64),
+        ("record_capacity", C.c_uint64),
+    ]
+
+
 # Every symbol the header declares, with its ctypes signature.
 _P = C.POINTER
 _HANDLE = C.c_void_p
@@ -161,6 +178,20 @@ SIGNATURES = {
     "plq_chain_cluster_stats": (C.c_int, [_HANDLE, _P(ClusterStats)]),
     "plq_chain_set_pass_cap": (C.c_int, [_HANDLE, C.c_uint64]),
     "plq_chain_destroy": (C.c_int, [_HANDLE]),
+    "plq_ensemble_create": (C.c_int, [_P(EnsembleDesc), _P(C.c_uint8), C.c_size_t, _P(_HANDLE)]),
+    "plq_ensemble_advance": (C.c_int, [_HANDLE, C.c_uint64]),
+    "plq_ensemble_read": (C.c_int, [_HANDLE, _P(C.c_uint8), C.c_size_t]),
+    "plq_ensemble_plaquette_sums": (C.c_int, [_HANDLE, _P(C.c_int64), C.c_size_t]),
+    "plq_ensemble_betas": (C.c_int, [_HANDLE, _P(C.c_double), C.c_size_t]),
+    "plq_ensemble_series": (C.c_int, [_HANDLE, _P(C.c_int64), _P(C.c_uint32), C.c_uint64, _P(C.c_uint64)]),
+    "plq_ensemble_exchange_counts": (C.c_int, [_HANDLE, _P(C.c_uint64), _P(C.c_uint64), C.c_size_t,
+                                               _P(C.c_size_t)]),
+    "plq_ensemble_set_launch": (C.c_int, [_HANDLE, C.c_int32, C.c_int32, C.c_int32]),
+    "plq_ensemble_tune": (C.c_int, [_HANDLE, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32), _P(C.c_double)]),
+    "plq_ensemble_sweeps_done": (C.c_int, [_HANDLE, _P(C.c_uint64)]),
+    "plq_ensemble_stream": (C.c_int, [_HANDLE, _P(C.c_void_p)]),
+    "plq_ensemble_sync": (C.c_int, [_HANDLE]),
+    "plq_ensemble_destroy": (C.c_int, [_HANDLE]),
     "plq_fold_seed": (C.c_uint32, [C.c_uint64]),
     "plq_threshold_table": (C.c_int, [_P(LocalDesc), _P(C.c_uint32), C.c_size_t]),
 }
@@ -395,3 +426,108 @@ def gauge_gpu_chain(shape, j, beta, seed, start, sweeps_between=1, kernel=Kernel
     """models/gauge/gpu.rs:54-110: links, link = site * D + dir."""
     d = local_desc(shape, Model.Z2, 2, kernel, profile, beta, j, 0.0, None, seed, sweeps_between)
     return GpuChain.build(d, start)
+
+
+class Ensemble:
+    """A batched Z2 gauge ensemble: many replicas of uneven shapes advanced
+    together on one device, with optional replica exchange between
+    neighbouring inverse temperatures of equal-shape replicas
+    (plq_ensemble_* in include/plaquette_hip.h; no reference counterpart)."""
+
+    def __init__(self, dim, shapes, betas, seeds, start=None, kernel=Kernel.HEAT_BATH, profile=Profile.F64,
+                 j=1.0, measure_every=10, exchange=True, exchange_seed=0, record_capacity=4096):
+        self.R = len(shapes)
+        self.dim = dim
+        self._shapes = np.ascontiguousarray(np.array(shapes, dtype=np.int64).reshape(self.R, dim))
+        self._betas = np.ascontiguousarray(betas, dtype=np.float64)
+        self._seeds = np.ascontiguousarray(np.array(seeds, dtype=np.uint64))
+        self.sizes = [int(np.prod(s)) * dim for s in self._shapes]
+        d = EnsembleDesc()
+        d.dim = dim
+        d.n_replicas = self.R
+        d.kernel = int(kernel)
+        d.profile = int(profile)
+        d.shapes = self._shapes.ctypes.data_as(_P(C.c_int64))
+        d.betas = self._betas.ctypes.data_as(_P(C.c_double))
+        d.seeds = self._seeds.ctypes.data_as(_P(C.c_uint64))
+        d.j = float(j)
+        d.measure_every = int(measure_every)
+        d.exchange = int(bool(exchange))
+        d.exchange_seed = int(exchange_seed) & 0xFFFFFFFFFFFFFFFF
+        d.record_capacity = int(record_capacity)
+        st = np.zeros(sum(self.sizes), np.uint8) if start is None else np.ascontiguousarray(start, dtype=np.uint8)
+        h = C.c_void_p(0)
+        _check(lib().plq_ensemble_create(C.byref(d), st.ctypes.data_as(_P(C.c_uint8)), st.size, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().plq_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def advance(self, sweeps: int) -> None:
+        _check(lib().plq_ensemble_advance(self._h, int(sweeps)))
+
+    def read(self) -> np.ndarray:
+        out = np.empty(sum(self.sizes), np.uint8)
+        _check(lib().plq_ensemble_read(self._h, out.ctypes.data_as(_P(C.c_uint8)), out.size))
+        return out
+
+    def plaquette_sums(self) -> np.ndarray:
+        out = np.empty(self.R, np.int64)
+        _check(lib().plq_ensemble_plaquette_sums(self._h, out.ctypes.data_as(_P(C.c_int64)), self.R))
+        return out
+
+    def betas(self) -> np.ndarray:
+        out = np.empty(self.R, np.float64)
+        _check(lib().plq_ensemble_betas(self._h, out.ctypes.data_as(_P(C.c_double)), self.R))
+        return out
+
+    def series(self):
+        n = C.c_uint64(0)
+        _check(lib().plq_ensemble_series(self._h, None, None, 0, C.byref(n)))
+        sums = np.zeros(max(1, n.value) * self.R, np.int64)
+        slots = np.zeros(max(1, n.value) * self.R, np.uint32)
+        _check(lib().plq_ensemble_series(self._h, sums.ctypes.data_as(_P(C.c_int64)),
+                                         slots.ctypes.data_as(_P(C.c_uint32)), n.value, C.byref(n)))
+        k = n.value
+        return sums[:k * self.R].reshape(k, self.R), slots[:k * self.R].reshape(k, self.R)
+
+    def exchange_counts(self):
+        n = C.c_size_t(0)
+        _check(lib().plq_ensemble_exchange_counts(self._h, None, None, 0, C.byref(n)))
+        acc = np.zeros(max(1, n.value), np.uint64)
+        att = np.zeros(max(1, n.value), np.uint64)
+        _check(lib().plq_ensemble_exchange_counts(self._h, acc.ctypes.data_as(_P(C.c_uint64)),
+                                                  att.ctypes.data_as(_P(C.c_uint64)), n.value, C.byref(n)))
+        return acc[:n.value], att[:n.value]
+
+    def set_launch(self, block_size=256, per_lane=1, use_graph=True):
+        _check(lib().plq_ensemble_set_launch(self._h, int(block_size), int(per_lane), int(bool(use_graph))))
+
+    def tune(self) -> dict:
+        bs, pl, g = C.c_int32(0), C.c_int32(0), C.c_int32(0)
+        us = C.c_double(0)
+        _check(lib().plq_ensemble_tune(self._h, C.byref(bs), C.byref(pl), C.byref(g), C.byref(us)))
+        return {"block_size": bs.value, "per_lane": pl.value, "use_graph": bool(g.value),
+                "us_per_sweep": us.value}
+
+    @property
+    def sweeps_done(self) -> int:
+        v = C.c_uint64(0)
+        _check(lib().plq_ensemble_sweeps_done(self._h, C.byref(v)))
+        return v.value
+
+    def stream(self) -> int:
+        p = C.c_void_p(0)
+        _check(lib().plq_ensemble_stream(self._h, C.byref(p)))
+        return p.value or 0
+
+    def sync(self) -> None:
+        _check(lib().plq_ensemble_sync(self._h))

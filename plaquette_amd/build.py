@@ -24,6 +24,8 @@ SOURCES = [
     "plq_site.hip",
     "plq_link.hip",
     "plq_cluster.hip",
+    "plq_ensemble.hip",
+    "plq_ensemble_abi.hip",
     "plq_thresholds.cpp",
 ]
 

@@ -36,6 +36,14 @@ int fail(int code, const std::string &msg) {
             return fail(PLQ_EDEVICE, std::string(#expr " failed: ") + hipGetErrorString(e_)); \
     } while (0)
 
+}  // namespace
+
+namespace plq {
+int host_fail(int code, const std::string &msg) { return fail(code, msg); }
+}  // namespace plq
+
+namespace {
+
 enum Kind { KIND_SITE = 0, KIND_LINK = 1, KIND_CLUSTER = 2 };
 
 bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }

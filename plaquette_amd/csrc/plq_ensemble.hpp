@@ -1,0 +1,58 @@
+// plq_ensemble.hpp -- device-side descriptors of a batched replica ensemble.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace plq {
+
+struct EnsReplica {
+    int32_t L[4];         // extents (unused axes 1)
+    int32_t H;            // L0 / 2 words per colour row
+    int32_t rstride[4];   // row stride of axis a >= 1
+    uint32_t nsites;
+    uint32_t off[2];      // word offsets of the two colour arrays
+    uint64_t mH;          // ceil(2^40 / H)
+    uint64_t mL[4];       // ceil(2^40 / L[a])
+    uint32_t block0;      // first block of this replica in a pass launch
+    uint32_t seed32;      // fold_seed of the replica's seed
+};
+
+struct EnsGroupPair {
+    uint32_t k;           // the pair (k, k+1) of its group's sorted betas
+    uint32_t rep_at0;     // offset of the group's rep_at list
+    uint32_t positive;    // 1 if (beta_k+1 - beta_k) J >= 0
+    uint32_t table0;      // offset of its threshold table (indexed by |P_a - P_b|)
+    uint32_t table_len;
+};
+
+struct EnsPassArgs {
+    uint32_t *words;
+    const EnsReplica *reps;
+    int nrep;
+    int D;
+    const uint32_t *tables;      // per beta slot: 2 * (2D - 1) thresholds
+    const uint32_t *beta_slot;   // per replica
+    const uint32_t *sweep_base;  // device sweep counter
+    uint32_t sweep_off;
+    int mu, c;
+    int mode;                    // 0 Metropolis, 1 heat bath
+    uint32_t blocks;
+    int block_size;
+    int per_lane;
+    hipStream_t stream;
+};
+
+void launch_ens_pack(const uint8_t *lat, uint32_t *words, const EnsReplica &R, int D, hipStream_t s);
+void launch_ens_unpack(uint8_t *lat, const uint32_t *words, const EnsReplica &R, int D, hipStream_t s);
+void launch_ens_pass(const EnsPassArgs &a);
+void launch_ens_plaquette(const uint32_t *words, const EnsReplica *reps, int nrep, int D, uint32_t blocks,
+                          int block_size, int per_lane, unsigned long long *sums, hipStream_t s);
+void launch_ens_exchange(const EnsGroupPair *pairs, int npairs, uint32_t *slot_of, uint32_t *rep_at,
+                         const unsigned long long *sums, const uint32_t *ex_tables, const uint32_t *round_base,
+                         uint32_t round_off, uint32_t ex_seed, unsigned long long *accepted, hipStream_t s);
+void launch_ens_tick(uint32_t *counters, uint32_t sweeps, uint32_t rounds, hipStream_t s);
+void launch_ens_record(const unsigned long long *sums, const uint32_t *slot_of, int nrep, long long *ring_sums,
+                       uint32_t *ring_slots, uint32_t cap, uint32_t *counters, hipStream_t s);
+
+}  // namespace plq

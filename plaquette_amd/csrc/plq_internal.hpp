@@ -5,9 +5,14 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "plq_common.hpp"
 
 namespace plq {
+
+// Record `msg` as the calling thread's last error and return `code`.
+int host_fail(int code, const std::string &msg);
 
 constexpr int MAX_Q_DEV = 16;
 // One measurement record on the device: [bond_sum, unused, counts[16]].
