@@ -35,20 +35,23 @@ struct LinkSite {
     uint32_t x[4];
 };
 
-// One step of a site along axis a (sign +1/-1) with periodic wrap.
-__device__ __forceinline__ LinkSite lstep(const EnsReplica &R, const LinkSite &s, int a, int sign) {
+// One step of a site along axis A (SIGN +1/-1) with periodic wrap.  The
+// axis is a template parameter so every coordinate lives in a register (a
+// runtime index into the coordinate array would put it in scratch memory).
+template <int A, int SIGN>
+__device__ __forceinline__ LinkSite lstep(const EnsReplica &R, const LinkSite &s) {
     LinkSite t = s;
-    uint32_t l = (uint32_t)R.L[a];
-    if (a == 0) {
-        t.x0 = sign > 0 ? (s.x0 + 1u == l ? 0u : s.x0 + 1u) : (s.x0 == 0u ? l - 1u : s.x0 - 1u);
+    const uint32_t l = (uint32_t)R.L[A];
+    if (A == 0) {
+        t.x0 = SIGN > 0 ? (s.x0 + 1u == l ? 0u : s.x0 + 1u) : (s.x0 == 0u ? l - 1u : s.x0 - 1u);
     } else {
-        uint32_t xa = s.x[a];
-        uint32_t st = (uint32_t)R.rstride[a];
-        if (sign > 0) {
-            t.x[a] = xa + 1u == l ? 0u : xa + 1u;
+        const uint32_t xa = s.x[A];
+        const uint32_t st = (uint32_t)R.rstride[A];
+        if (SIGN > 0) {
+            t.x[A] = xa + 1u == l ? 0u : xa + 1u;
             t.r = xa + 1u == l ? s.r - (l - 1u) * st : s.r + st;
         } else {
-            t.x[a] = xa == 0u ? l - 1u : xa - 1u;
+            t.x[A] = xa == 0u ? l - 1u : xa - 1u;
             t.r = xa == 0u ? s.r + (l - 1u) * st : s.r - st;
         }
     }
@@ -58,32 +61,54 @@ __device__ __forceinline__ LinkSite lstep(const EnsReplica &R, const LinkSite &s
 // The word of a site: colour (x0 + row parity) & 1, index r * H + x0 / 2.
 __device__ __forceinline__ uint32_t lword(const uint32_t *__restrict__ words, const EnsReplica &R, const LinkSite &s,
                                           uint32_t rowpar) {
-    uint32_t col = (s.x0 + rowpar) & 1u;
-    return ld32(words, R.off[col] + s.r * (uint32_t)R.H + (s.x0 >> 1));
+    const uint32_t off = ((s.x0 + rowpar) & 1u) ? R.off[1] : R.off[0];
+    return ld32(words, off + s.r * (uint32_t)R.H + (s.x0 >> 1));
 }
 
 __device__ __forceinline__ uint32_t row_parity(const LinkSite &s) { return (s.x[1] + s.x[2] + s.x[3]) & 1u; }
 
 __device__ __forceinline__ uint32_t byte_of(uint32_t w, int d) { return (w >> (8 * d)) & 1u; }
 
-// Block -> replica: the replicas' block ranges are consecutive; a uniform
-// scan over the (at most a few hundred) replicas.
-__device__ __forceinline__ int block_replica(const EnsReplica *__restrict__ reps, int nrep, uint32_t b) {
-    int r = 0;
-    while (r + 1 < nrep && reps[r + 1].block0 <= b) ++r;
-    return r;
+// The two staples of link (x, MU) in the (MU, NU) plane, as the number of
+// negative products among them:
+//   forward  U_NU(x+MU) U_MU(x+NU) U_NU(x)
+//   backward U_MU(x-NU) U_NU(x-NU+MU) U_NU(x-NU)
+template <int MU, int NU>
+__device__ __forceinline__ int staple_negs(const uint32_t *__restrict__ words, const EnsReplica &R, const LinkSite &s,
+                                           uint32_t wx, uint32_t wxm) {
+    const LinkSite xpn = lstep<NU, +1>(R, s);
+    const LinkSite xmn = lstep<NU, -1>(R, s);
+    const LinkSite xmnp = lstep<MU, +1>(R, xmn);
+    const uint32_t wpn = lword(words, R, xpn, row_parity(xpn));
+    const uint32_t wmn = lword(words, R, xmn, row_parity(xmn));
+    const uint32_t wmnp = lword(words, R, xmnp, row_parity(xmnp));
+    int neg = (int)(byte_of(wxm, NU) ^ byte_of(wpn, MU) ^ byte_of(wx, NU));
+    neg += (int)(byte_of(wmn, MU) ^ byte_of(wmnp, NU) ^ byte_of(wmn, NU));
+    return neg;
 }
 
-// One (mu, c) pass of every replica.  sweep = *sweep_base + sweep_off.
-template <int D, int MODE>
+template <int D, int MU>
+__device__ __forceinline__ int link_negs(const uint32_t *__restrict__ words, const EnsReplica &R, const LinkSite &s,
+                                         uint32_t wx, uint32_t wxm) {
+    int neg = 0;
+    if (MU != 0) neg += staple_negs<MU, 0>(words, R, s, wx, wxm);
+    if (D > 1 && MU != 1) neg += staple_negs<MU, 1>(words, R, s, wx, wxm);
+    if (D > 2 && MU != 2) neg += staple_negs<MU, (D > 2 ? 2 : 0)>(words, R, s, wx, wxm);
+    if (D > 3 && MU != 3) neg += staple_negs<MU, (D > 3 ? 3 : 0)>(words, R, s, wx, wxm);
+    return neg;
+}
+
+// One (MU, c) pass of every replica.  sweep = *sweep_base + sweep_off.
+template <int D, int MODE, int MU>
 __global__ __launch_bounds__(256) void ens_link_pass(uint32_t *__restrict__ words, const EnsReplica *__restrict__ reps,
-                                                     int nrep, const uint32_t *__restrict__ tables,
+                                                     const uint16_t *__restrict__ block_rep,
+                                                     const uint32_t *__restrict__ tables,
                                                      const uint32_t *__restrict__ beta_slot,
                                                      const uint32_t *__restrict__ sweep_base, uint32_t sweep_off,
-                                                     int mu, int c, int per_lane) {
+                                                     int c, int per_lane) {
     constexpr int K = 2 * (D - 1);
     __shared__ uint32_t T[2 * (K + 1)];
-    const int ri = block_replica(reps, nrep, blockIdx.x);
+    const int ri = block_rep[blockIdx.x];
     const EnsReplica R = reps[ri];
     if (threadIdx.x < 2 * (K + 1)) T[threadIdx.x] = tables[beta_slot[ri] * (2 * (K + 1)) + threadIdx.x];
     __syncthreads();
@@ -115,43 +140,42 @@ __global__ __launch_bounds__(256) void ens_link_pass(uint32_t *__restrict__ word
         const uint32_t rp = row_parity(s);
         s.x0 = 2u * i + ((rp + (uint32_t)c) & 1u);
         const uint32_t wx = lword(words, R, s, rp);
-        const LinkSite xm = lstep(R, s, mu, +1);
+        const LinkSite xm = lstep<MU, +1>(R, s);
         const uint32_t wxm = lword(words, R, xm, row_parity(xm));
-        int neg = 0;
-#pragma unroll
-        for (int nu = 0; nu < D; ++nu) {
-            if (nu == mu) continue;
-            const LinkSite xpn = lstep(R, s, nu, +1);
-            const LinkSite xmn = lstep(R, s, nu, -1);
-            const LinkSite xmnp = lstep(R, xmn, mu, +1);
-            const uint32_t wpn = lword(words, R, xpn, row_parity(xpn));
-            const uint32_t wmn = lword(words, R, xmn, row_parity(xmn));
-            const uint32_t wmnp = lword(words, R, xmnp, row_parity(xmnp));
-            // forward: U_nu(x+mu) U_mu(x+nu) U_nu(x); backward: U_mu(x-nu) U_nu(x-nu+mu) U_nu(x-nu)
-            neg += (int)(byte_of(wxm, nu) ^ byte_of(wpn, mu) ^ byte_of(wx, nu));
-            neg += (int)(byte_of(wmn, mu) ^ byte_of(wmnp, nu) ^ byte_of(wmn, nu));
-        }
-        const uint32_t cur = byte_of(wx, mu);
+        const int neg = link_negs<D, MU>(words, R, s, wx, wxm);
+        const uint32_t cur = byte_of(wx, MU);
         const uint32_t t = T[cur * (K + 1) + (uint32_t)neg];
         const uint32_t site = s.r * (uint32_t)R.L[0] + s.x0;
-        const uint32_t mm = keyed_bits_k(R.seed32, site * (uint32_t)D + (uint32_t)mu, skey);
+        const uint32_t mm = keyed_bits_k(R.seed32, site * (uint32_t)D + (uint32_t)MU, skey);
         const bool hit = mm < t;
         const uint32_t nxt = MODE == 0 ? (cur ^ (hit ? 1u : 0u)) : (hit ? 0u : 1u);
         if (nxt != cur) {
             uint8_t *bytes = reinterpret_cast<uint8_t *>(words);
-            const uint32_t widx = R.off[c] + s.r * (uint32_t)R.H + i;
-            bytes[(size_t)widx * 4u + (uint32_t)mu] = (uint8_t)nxt;
+            const uint32_t widx = (c ? R.off[1] : R.off[0]) + s.r * (uint32_t)R.H + i;
+            bytes[(size_t)widx * 4u + (uint32_t)MU] = (uint8_t)nxt;
         }
     }
+}
+
+// U_A(x) U_B(x+A) U_A(x+B) U_B(x) (lattice.rs:420-430) as 0 (+1) / 1 (-1)
+template <int A, int B>
+__device__ __forceinline__ uint32_t plaq_bit(const uint32_t *__restrict__ words, const EnsReplica &R,
+                                             const LinkSite &s, uint32_t wx) {
+    const LinkSite sa = lstep<A, +1>(R, s);
+    const LinkSite sb = lstep<B, +1>(R, s);
+    const uint32_t wa = lword(words, R, sa, row_parity(sa));
+    const uint32_t wb = lword(words, R, sb, row_parity(sb));
+    return byte_of(wx, A) ^ byte_of(wa, B) ^ byte_of(wb, A) ^ byte_of(wx, B);
 }
 
 // Signed plaquette sum of every replica into sums[r] (zeroed by the caller).
 template <int D>
 __global__ __launch_bounds__(256) void ens_plaquette(const uint32_t *__restrict__ words,
-                                                     const EnsReplica *__restrict__ reps, int nrep,
+                                                     const EnsReplica *__restrict__ reps,
+                                                     const uint16_t *__restrict__ block_rep,
                                                      unsigned long long *__restrict__ sums, int per_lane) {
     __shared__ long long lds[8];
-    const int ri = block_replica(reps, nrep, blockIdx.x);
+    const int ri = block_rep[blockIdx.x];
     const EnsReplica R = reps[ri];
     long long acc = 0;
     const uint32_t base = (blockIdx.x - R.block0) * blockDim.x * (uint32_t)per_lane + threadIdx.x;
@@ -179,19 +203,12 @@ __global__ __launch_bounds__(256) void ens_plaquette(const uint32_t *__restrict_
         const uint32_t rp = row_parity(s);
         s.x0 = 2u * i + ((rp + (uint32_t)(k & 1)) & 1u);
         const uint32_t wx = lword(words, R, s, rp);
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-            const LinkSite sa = lstep(R, s, a, +1);
-            const uint32_t wa = lword(words, R, sa, row_parity(sa));
-#pragma unroll
-            for (int b = a + 1; b < D; ++b) {
-                const LinkSite sb = lstep(R, s, b, +1);
-                const uint32_t wb = lword(words, R, sb, row_parity(sb));
-                // U_a(x) U_b(x+a) U_a(x+b) U_b(x)  (lattice.rs:420-430)
-                uint32_t x = byte_of(wx, a) ^ byte_of(wa, b) ^ byte_of(wb, a) ^ byte_of(wx, b);
-                acc += 1 - 2 * (int)x;
-            }
-        }
+        uint32_t neg = plaq_bit<0, 1>(words, R, s, wx);
+        if (D >= 3) neg += plaq_bit<0, (D >= 3 ? 2 : 1)>(words, R, s, wx) + plaq_bit<1, (D >= 3 ? 2 : 1)>(words, R, s, wx);
+        if (D >= 4)
+            neg += plaq_bit<0, (D >= 4 ? 3 : 1)>(words, R, s, wx) + plaq_bit<1, (D >= 4 ? 3 : 1)>(words, R, s, wx) +
+                   plaq_bit<2, (D >= 4 ? 3 : 1)>(words, R, s, wx);
+        acc += (long long)(D * (D - 1) / 2) - 2 * (long long)neg;
     }
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
     if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = acc;
@@ -306,31 +323,40 @@ void launch_ens_unpack(uint8_t *lat, const uint32_t *words, const EnsReplica &R,
     ens_unpack<<<dim3((R.nsites + 255) / 256), dim3(256), 0, s>>>(lat, words, R, D);
 }
 
-void launch_ens_pass(const EnsPassArgs &a) {
+template <int D, int MODE>
+static void ens_pass_mu(const EnsPassArgs &a) {
     dim3 grid(a.blocks), block(a.block_size);
-#define PLQ_ENS(DD)                                                                                          \
-    if (a.mode == 0)                                                                                         \
-        ens_link_pass<DD, 0><<<grid, block, 0, a.stream>>>(a.words, a.reps, a.nrep, a.tables, a.beta_slot,   \
-                                                           a.sweep_base, a.sweep_off, a.mu, a.c, a.per_lane); \
-    else                                                                                                     \
-        ens_link_pass<DD, 1><<<grid, block, 0, a.stream>>>(a.words, a.reps, a.nrep, a.tables, a.beta_slot,   \
-                                                           a.sweep_base, a.sweep_off, a.mu, a.c, a.per_lane)
-    switch (a.D) {
-        case 2: PLQ_ENS(2); break;
-        case 3: PLQ_ENS(3); break;
-        default: PLQ_ENS(4); break;
+#define PLQ_ENS_MU(M)                                                                                              \
+    ens_link_pass<D, MODE, M><<<grid, block, 0, a.stream>>>(a.words, a.reps, a.block_rep, a.tables, a.beta_slot, \
+                                                            a.sweep_base, a.sweep_off, a.c, a.per_lane)
+    switch (a.mu) {
+        case 0: PLQ_ENS_MU(0); break;
+        case 1: PLQ_ENS_MU(1); break;
+        case 2: PLQ_ENS_MU(2 < D ? 2 : 0); break;
+        default: PLQ_ENS_MU(3 < D ? 3 : 0); break;
     }
-#undef PLQ_ENS
+#undef PLQ_ENS_MU
 }
 
-void launch_ens_plaquette(const uint32_t *words, const EnsReplica *reps, int nrep, int D, uint32_t blocks,
-                          int block_size, int per_lane, unsigned long long *sums, hipStream_t s) {
+void launch_ens_pass(const EnsPassArgs &a) {
+    switch (a.D * 2 + a.mode) {
+        case 4: ens_pass_mu<2, 0>(a); break;
+        case 5: ens_pass_mu<2, 1>(a); break;
+        case 6: ens_pass_mu<3, 0>(a); break;
+        case 7: ens_pass_mu<3, 1>(a); break;
+        case 8: ens_pass_mu<4, 0>(a); break;
+        default: ens_pass_mu<4, 1>(a); break;
+    }
+}
+
+void launch_ens_plaquette(const uint32_t *words, const EnsReplica *reps, const uint16_t *block_rep, int nrep, int D,
+                          uint32_t blocks, int block_size, int per_lane, unsigned long long *sums, hipStream_t s) {
     ens_zero<<<dim3((nrep + 255) / 256), dim3(256), 0, s>>>(sums, nrep);
     dim3 grid(blocks), block(block_size);
     switch (D) {
-        case 2: ens_plaquette<2><<<grid, block, 0, s>>>(words, reps, nrep, sums, per_lane); break;
-        case 3: ens_plaquette<3><<<grid, block, 0, s>>>(words, reps, nrep, sums, per_lane); break;
-        default: ens_plaquette<4><<<grid, block, 0, s>>>(words, reps, nrep, sums, per_lane); break;
+        case 2: ens_plaquette<2><<<grid, block, 0, s>>>(words, reps, block_rep, sums, per_lane); break;
+        case 3: ens_plaquette<3><<<grid, block, 0, s>>>(words, reps, block_rep, sums, per_lane); break;
+        default: ens_plaquette<4><<<grid, block, 0, s>>>(words, reps, block_rep, sums, per_lane); break;
     }
 }
 

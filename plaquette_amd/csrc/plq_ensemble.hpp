@@ -29,6 +29,7 @@ struct EnsGroupPair {
 struct EnsPassArgs {
     uint32_t *words;
     const EnsReplica *reps;
+    const uint16_t *block_rep;   // replica of each block
     int nrep;
     int D;
     const uint32_t *tables;      // per beta slot: 2 * (2D - 1) thresholds
@@ -46,8 +47,8 @@ struct EnsPassArgs {
 void launch_ens_pack(const uint8_t *lat, uint32_t *words, const EnsReplica &R, int D, hipStream_t s);
 void launch_ens_unpack(uint8_t *lat, const uint32_t *words, const EnsReplica &R, int D, hipStream_t s);
 void launch_ens_pass(const EnsPassArgs &a);
-void launch_ens_plaquette(const uint32_t *words, const EnsReplica *reps, int nrep, int D, uint32_t blocks,
-                          int block_size, int per_lane, unsigned long long *sums, hipStream_t s);
+void launch_ens_plaquette(const uint32_t *words, const EnsReplica *reps, const uint16_t *block_rep, int nrep, int D,
+                          uint32_t blocks, int block_size, int per_lane, unsigned long long *sums, hipStream_t s);
 void launch_ens_exchange(const EnsGroupPair *pairs, int npairs, uint32_t *slot_of, uint32_t *rep_at,
                          const unsigned long long *sums, const uint32_t *ex_tables, const uint32_t *round_base,
                          uint32_t round_off, uint32_t ex_seed, unsigned long long *accepted, hipStream_t s);

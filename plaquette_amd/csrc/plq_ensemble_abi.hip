@@ -61,6 +61,8 @@ struct plq_ensemble {
     std::vector<size_t> lat_off;  // per replica offset in the lattice-order concatenation
     size_t lat_bytes = 0;
     plq::EnsReplica *d_reps = nullptr;
+    uint16_t *d_block_rep = nullptr;  // replica of each pass block
+    size_t block_rep_cap = 0;
     uint32_t *d_words = nullptr;
     size_t nwords = 0;
     uint8_t *d_lat = nullptr;
@@ -90,7 +92,7 @@ struct plq_ensemble {
         (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        for (void *p : {(void *)d_reps, (void *)d_words, (void *)d_lat, (void *)d_tables, (void *)d_slot_of,
+        for (void *p : {(void *)d_reps, (void *)d_block_rep, (void *)d_words, (void *)d_lat, (void *)d_tables, (void *)d_slot_of,
                         (void *)d_rep_at, (void *)d_pairs, (void *)d_ex_tables, (void *)d_sums, (void *)d_accepted,
                         (void *)d_counters, (void *)d_ring_sums, (void *)d_ring_slots})
             if (p) (void)hipFree(p);
@@ -110,6 +112,19 @@ int layout_blocks(plq_ensemble *e) {
         b += (half + per_block - 1) / per_block;
     }
     e->total_blocks = b;
+    std::vector<uint16_t> block_rep(b);
+    for (int r = 0; r < e->nrep; ++r) {
+        uint32_t end = r + 1 < e->nrep ? e->reps[r + 1].block0 : b;
+        for (uint32_t k = e->reps[r].block0; k < end; ++k) block_rep[k] = (uint16_t)r;
+    }
+    if (e->block_rep_cap < b) {
+        if (e->d_block_rep) (void)hipFree(e->d_block_rep);
+        e->d_block_rep = nullptr;
+        e->block_rep_cap = 0;
+        ENS_TRY(dalloc((void **)&e->d_block_rep, sizeof(uint16_t) * b));
+        e->block_rep_cap = b;
+    }
+    ENS_HIP(hipMemcpy(e->d_block_rep, block_rep.data(), sizeof(uint16_t) * b, hipMemcpyHostToDevice));
     ENS_HIP(hipMemcpy(e->d_reps, e->reps.data(), sizeof(plq::EnsReplica) * e->reps.size(), hipMemcpyHostToDevice));
     if (e->graph_exec) {
         (void)hipGraphExecDestroy(e->graph_exec);
@@ -122,6 +137,7 @@ void enqueue_block(plq_ensemble *e, uint32_t nsweeps, bool measure) {
     plq::EnsPassArgs a{};
     a.words = e->d_words;
     a.reps = e->d_reps;
+    a.block_rep = e->d_block_rep;
     a.nrep = e->nrep;
     a.D = e->D;
     a.tables = e->d_tables;
@@ -143,7 +159,7 @@ void enqueue_block(plq_ensemble *e, uint32_t nsweeps, bool measure) {
     }
     uint32_t rounds = 0;
     if (measure) {
-        plq::launch_ens_plaquette(e->d_words, e->d_reps, e->nrep, e->D, e->total_blocks, e->block_size,
+        plq::launch_ens_plaquette(e->d_words, e->d_reps, e->d_block_rep, e->nrep, e->D, e->total_blocks, e->block_size,
                                   e->per_lane, e->d_sums, e->stream);
         plq::launch_ens_record(e->d_sums, e->d_slot_of, e->nrep, e->d_ring_sums, e->d_ring_slots,
                                (uint32_t)e->rec_cap, e->d_counters, e->stream);
@@ -418,7 +434,7 @@ int plq_ensemble_plaquette_sums(plq_ensemble *e, int64_t *sums, size_t n) {
     if (n != (size_t)e->nrep) return plq::host_fail(PLQ_EINVAL, "one sum per replica");
     std::lock_guard<std::mutex> lk(e->mu);
     ENS_HIP(hipSetDevice(e->device));
-    plq::launch_ens_plaquette(e->d_words, e->d_reps, e->nrep, e->D, e->total_blocks, e->block_size, e->per_lane,
+    plq::launch_ens_plaquette(e->d_words, e->d_reps, e->d_block_rep, e->nrep, e->D, e->total_blocks, e->block_size, e->per_lane,
                               e->d_sums, e->stream);
     std::vector<unsigned long long> h(e->nrep);
     ENS_HIP(hipMemcpyAsync(h.data(), e->d_sums, sizeof(unsigned long long) * e->nrep, hipMemcpyDeviceToHost,
