@@ -75,68 +75,72 @@ __device__ __forceinline__ void lds_union(int32_t *P, int32_t a, int32_t b) {
     }
 }
 
+// n / d for n < 2^24 through a 64-bit reciprocal: q = (n * M) >> 40 with
+// M = ceil(2^40 / d) is exact while n * d < 2^40.
+__device__ __forceinline__ uint32_t cdiv(uint32_t n, uint64_t M) { return (uint32_t)(((uint64_t)n * M) >> 40); }
+
 template <int D>
 __global__ __launch_bounds__(1024) void cc_local(const uint8_t *__restrict__ labels, int32_t *__restrict__ parent,
                                                  CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t) {
     extern __shared__ int32_t P[];
-    int32_t o[MAX_D], e[MAX_D], ls[MAX_D];
-    int64_t rest = blockIdx.x;
-    int32_t nloc = 1;
+    uint32_t o[MAX_D], e[MAX_D], ls[MAX_D];
+    uint64_t me[MAX_D];
+    uint32_t rest = blockIdx.x;
+    uint32_t nloc = 1;
 #pragma unroll
     for (int mu = 0; mu < D; ++mu) {
-        int32_t b = (int32_t)(rest % cg.nbox[mu]);
-        rest /= cg.nbox[mu];
-        o[mu] = b * cg.B[mu];
-        e[mu] = cg.L[mu] - o[mu] < cg.B[mu] ? cg.L[mu] - o[mu] : cg.B[mu];
+        const uint32_t nb = (uint32_t)cg.nbox[mu];
+        const uint32_t q = rest / nb;  // once per block
+        const uint32_t b = rest - q * nb;
+        rest = q;
+        o[mu] = b * (uint32_t)cg.B[mu];
+        const uint32_t left = (uint32_t)cg.L[mu] - o[mu];
+        e[mu] = left < (uint32_t)cg.B[mu] ? left : (uint32_t)cg.B[mu];
+        me[mu] = ((uint64_t(1) << 40) + e[mu] - 1) / e[mu];
         ls[mu] = nloc;
         nloc *= e[mu];
     }
-    for (int32_t l = threadIdx.x; l < nloc; l += blockDim.x) P[l] = l;
-    __syncthreads();
-    for (int32_t l = threadIdx.x; l < nloc; l += blockDim.x) {
-        int32_t y[MAX_D];
-        int32_t r = l;
-        int64_t gsite = 0;
+    // local index -> (box coordinates, global site)
+    auto locate = [&](uint32_t l, uint32_t *y) -> uint32_t {
+        uint32_t r = l, g = 0;
 #pragma unroll
         for (int mu = 0; mu < D; ++mu) {
-            y[mu] = r % e[mu];
-            r /= e[mu];
-            gsite += (int64_t)(o[mu] + y[mu]) * cg.stride[mu];
+            uint32_t q = mu + 1 < D ? cdiv(r, me[mu]) : 0u;
+            y[mu] = mu + 1 < D ? r - q * e[mu] : r;
+            r = q;
+            g += (o[mu] + y[mu]) * (uint32_t)cg.stride[mu];
         }
-        uint8_t lab = labels[gsite];
+        return g;
+    };
+    for (uint32_t l = threadIdx.x; l < nloc; l += blockDim.x) P[l] = (int32_t)l;
+    __syncthreads();
+    for (uint32_t l = threadIdx.x; l < nloc; l += blockDim.x) {
+        uint32_t y[MAX_D];
+        const uint32_t gsite = locate(l, y);
+        const uint8_t lab = labels[gsite];
 #pragma unroll
         for (int mu = 0; mu < D; ++mu) {
-            int32_t lp;
-            int64_t gp;
-            if (y[mu] + 1 < e[mu]) {
+            uint32_t lp, gp;
+            if (y[mu] + 1u < e[mu]) {
                 lp = l + ls[mu];
-                gp = gsite + cg.stride[mu];
-            } else if (e[mu] == cg.L[mu]) {  // the box spans the axis: the wrap bond is internal
-                lp = l - (e[mu] - 1) * ls[mu];
-                gp = gsite - (int64_t)(cg.L[mu] - 1) * cg.stride[mu];
+                gp = gsite + (uint32_t)cg.stride[mu];
+            } else if (e[mu] == (uint32_t)cg.L[mu]) {  // the box spans the axis: the wrap bond is internal
+                lp = l - (e[mu] - 1u) * ls[mu];
+                gp = gsite - (uint32_t)(cg.L[mu] - 1) * (uint32_t)cg.stride[mu];
             } else {
                 continue;  // leaves the box: cc_boundary draws it
             }
             if (labels[gp] != lab) continue;
-            uint32_t bond = (uint32_t)(gsite * D + mu);
-            if (keyed_bits_k(bond_seed, bond, skey) < bond_t) lds_union(P, l, lp);
+            const uint32_t bond = gsite * (uint32_t)D + (uint32_t)mu;
+            if (keyed_bits_k(bond_seed, bond, skey) < bond_t) lds_union(P, (int32_t)l, (int32_t)lp);
         }
     }
     __syncthreads();
-    for (int32_t l = threadIdx.x; l < nloc; l += blockDim.x) {
-        int32_t root = lds_find(P, l);
-        int32_t y[MAX_D];
-        int32_t r = l;
-        int64_t gsite = 0, groot = 0;
-        int32_t rr = root;
-#pragma unroll
-        for (int mu = 0; mu < D; ++mu) {
-            y[mu] = r % e[mu];
-            r /= e[mu];
-            gsite += (int64_t)(o[mu] + y[mu]) * cg.stride[mu];
-            groot += (int64_t)(o[mu] + rr % e[mu]) * cg.stride[mu];
-            rr /= e[mu];
-        }
+    for (uint32_t l = threadIdx.x; l < nloc; l += blockDim.x) {
+        const uint32_t root = (uint32_t)lds_find(P, (int32_t)l);
+        uint32_t y[MAX_D], yr[MAX_D];
+        const uint32_t gsite = locate(l, y);
+        const uint32_t groot = root == l ? gsite : locate(root, yr);
         parent[gsite] = (int32_t)groot;
     }
 }
@@ -151,6 +155,18 @@ __device__ __forceinline__ int32_t g_find(int32_t *P, int32_t x, uint32_t cap, u
         if (gp != p) store_relaxed(&P[x], gp);
         x = p;
         p = load_relaxed(&P[x]);
+    }
+    return x;
+}
+
+// Root of x in a forest no lane modifies during the kernel: plain (cached)
+// loads, no path compression.
+__device__ __forceinline__ int32_t find_ro(const int32_t *__restrict__ P, int32_t x, uint32_t cap, uint32_t &hops) {
+    int32_t p = P[x];
+    while (p != x) {
+        if (++hops > cap) return x;
+        x = p;
+        p = P[x];
     }
     return x;
 }
@@ -229,7 +245,7 @@ __device__ __forceinline__ long long blk_sum(long long v, long long *lds) {
     return t;
 }
 
-__global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, int32_t *__restrict__ parent,
+__global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, const int32_t *__restrict__ parent,
                                                   int64_t N, uint32_t root_seed, uint32_t seed_seed, uint32_t skey,
                                                   uint32_t sweep, int q, int seeded, int forced, uint32_t cap,
                                                   int *__restrict__ flag, long long *__restrict__ partials,
@@ -244,7 +260,7 @@ __global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, 
             uint32_t ss = (uint32_t)(u * (float)(uint32_t)N);
             if (ss >= (uint32_t)N) ss = (uint32_t)N - 1u;
             uint32_t hops = 0;
-            int32_t r = g_find(parent, (int32_t)ss, cap, hops);
+            int32_t r = find_ro(parent, (int32_t)ss, cap, hops);
             if (hops > cap) atomicOr(flag, 1);
             seed_root = r;
         }
@@ -258,7 +274,7 @@ __global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, 
     if (end > N) end = N;
     for (int64_t s = begin + threadIdx.x; s < end; s += blockDim.x) {
         uint32_t hops = 0;
-        int32_t root = g_find(parent, (int32_t)s, cap, hops);
+        int32_t root = find_ro(parent, (int32_t)s, cap, hops);
         if (hops > cap) atomicOr(flag, 1);
         uint32_t lab = labels[s];
         if (!seeded || root == seed_root) {
@@ -303,15 +319,18 @@ __global__ __launch_bounds__(256) void potts_measure_kernel(const uint8_t *__res
     int64_t begin = (int64_t)blockIdx.x * span;
     int64_t end = begin + span;
     if (end > cg.N) end = cg.N;
-    for (int64_t s = begin + threadIdx.x; s < end; s += blockDim.x) {
-        uint8_t lab = labels[s];
+    for (int64_t s64 = begin + threadIdx.x; s64 < end; s64 += blockDim.x) {
+        const uint32_t s = (uint32_t)s64;
+        const uint8_t lab = labels[s];
         counts[lab] += 1;
-        int64_t rest = s;
+        uint32_t rest = s;
 #pragma unroll
         for (int mu = 0; mu < D; ++mu) {
-            int32_t x = (int32_t)(rest % cg.L[mu]);
-            rest /= cg.L[mu];
-            int64_t gp = (x + 1 == cg.L[mu]) ? s - (int64_t)x * cg.stride[mu] : s + cg.stride[mu];
+            const uint32_t q = mu + 1 < D ? (cg.mul_div ? cdiv(rest, cg.mL[mu]) : rest / (uint32_t)cg.L[mu]) : 0u;
+            const uint32_t x = mu + 1 < D ? rest - q * (uint32_t)cg.L[mu] : rest;
+            rest = q;
+            const uint32_t st = (uint32_t)cg.stride[mu];
+            const uint32_t gp = (x + 1u == (uint32_t)cg.L[mu]) ? s - x * st : s + st;
             agree += labels[gp] == lab;
         }
     }
@@ -334,7 +353,7 @@ __global__ void cc_resolve(int32_t *__restrict__ parent, int32_t *__restrict__ r
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= N) return;
     uint32_t hops = 0;
-    int32_t r = g_find(parent, (int32_t)s, cap, hops);
+    int32_t r = find_ro(parent, (int32_t)s, cap, hops);
     if (hops > cap) atomicOr(flag, 1);
     roots[s] = r;
 }
@@ -396,9 +415,13 @@ CCGeom make_ccgeom(int D, const int64_t *shape, int32_t max_box_sites) {
         if (vol <= max_box_sites) break;
         cg.B[big] = (cg.B[big] + 1) / 2;
     }
+    int64_t maxL = 1;
+    for (int mu = 0; mu < D; ++mu) maxL = cg.L[mu] > maxL ? cg.L[mu] : maxL;
+    cg.mul_div = (double)cg.N * (double)maxL < 1099511627776.0 ? 1 : 0;  // 2^40
     cg.nboxes = 1;
     cg.box_sites = 1;
     for (int mu = 0; mu < MAX_D; ++mu) {
+        cg.mL[mu] = ((uint64_t(1) << 40) + (uint64_t)cg.L[mu] - 1) / (uint64_t)cg.L[mu];
         cg.nbox[mu] = (cg.L[mu] + cg.B[mu] - 1) / cg.B[mu];
         cg.nboxes *= cg.nbox[mu];
         cg.box_sites *= cg.B[mu];

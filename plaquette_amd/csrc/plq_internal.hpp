@@ -88,6 +88,8 @@ struct CCGeom {
     int32_t D;
     int32_t L[MAX_D];
     int64_t stride[MAX_D];
+    uint64_t mL[MAX_D];   // ceil(2^40 / L[mu]) for division by multiplication
+    int32_t mul_div;      // 1 when N * max(L) < 2^40, so the reciprocal division is exact
     int32_t B[MAX_D];     // box extents
     int32_t nbox[MAX_D];  // boxes per axis
     int64_t N;

@@ -108,12 +108,12 @@ __global__ __launch_bounds__(1024) void finalize_columns_kernel(const long long 
                                                                 const long long *__restrict__ pb, int64_t nb, int nk,
                                                                 long long *__restrict__ rec) {
     __shared__ long long lds[16];
-    for (int k = 0; k < REC_K; ++k) {
+    if (threadIdx.x == 0)
+        for (int k = nk; k < REC_K; ++k) rec[k] = 0;
+    for (int k = 0; k < nk; ++k) {
         long long v = 0;
-        if (k < nk) {
-            for (int64_t b = threadIdx.x; b < na; b += blockDim.x) v += pa[k * na + b];
-            for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) v += pb[k * nb + b];
-        }
+        for (int64_t b = threadIdx.x; b < na; b += blockDim.x) v += pa[k * na + b];
+        for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) v += pb[k * nb + b];
         long long t = block_sum(v, lds);
         if (threadIdx.x == 0) rec[k] = t;
     }
@@ -588,7 +588,7 @@ void launch_finalize_record(const long long *pa, int64_t na, const long long *pb
 
 void launch_finalize_columns(const long long *pa, int64_t na, const long long *pb, int64_t nb, int nk, long long *rec,
                              hipStream_t s) {
-    finalize_columns_kernel<<<dim3(1), dim3(1024), 0, s>>>(pa, na, pb, nb, nk, rec);
+    finalize_columns_kernel<<<dim3(1), dim3(256), 0, s>>>(pa, na, pb, nb, nk, rec);
 }
 
 static int64_t pass_work(const SitePassArgs &a) { return (int64_t)a.g.R * a.g.H / a.vec; }
