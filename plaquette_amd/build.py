@@ -26,6 +26,7 @@ SOURCES = [
     "plq_cluster.hip",
     "plq_ensemble.hip",
     "plq_ensemble_abi.hip",
+    "plq_small.hip",
     "plq_thresholds.cpp",
 ]
 

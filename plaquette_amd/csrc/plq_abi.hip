@@ -72,6 +72,7 @@ struct plq_chain {
     plq::Geom g{};
     plq::SwarGeom sg{};
     int vec = 1;                           // sites per lane of the site kernel
+    bool small = false;                    // whole sweeps in one workgroup (plq_small.hip)
     int64_t pass_blocks = plq::MAX_PASS_BLOCKS;  // grid cap of a colour pass
     uint8_t *col[2] = {nullptr, nullptr};  // site colours
     uint8_t *links[2 * plq::MAX_D] = {nullptr};
@@ -284,6 +285,43 @@ int enqueue_local_sweep(plq_chain *c) {
     return PLQ_OK;
 }
 
+// Enqueue `sweeps` sweeps of a small chain, at most 4096 per launch, with
+// the records that fall inside them.
+int enqueue_small_sweeps(plq_chain *c, uint64_t sweeps) {
+    while (sweeps > 0) {
+        uint32_t n = sweeps > 4096 ? 4096u : (uint32_t)sweeps;
+        plq::SmallChainArgs a{};
+        a.c0 = c->col[0];
+        a.c1 = c->col[1];
+        a.D = c->D;
+        for (int mu = 0; mu < c->D; ++mu) a.lg[mu] = (uint32_t)ilog2(c->shape[mu]);
+        a.N = c->n_sites;
+        a.mode = c->kernel == PLQ_KERNEL_METROPOLIS ? 0 : 1;
+        a.seed = c->seed32;
+        a.sweep0 = (uint32_t)c->sweeps_done;
+        a.nsweeps = n;
+        a.table = c->d_table;
+        a.rec = c->d_rec;
+        a.rec_cap = c->rec_cap;
+        a.rec_first = c->rec_cap ? c->rec_count % c->rec_cap : 0;
+        a.every = c->rec_every;
+        a.phase = c->rec_every ? c->sweeps_done % c->rec_every : 0;
+        a.stream = c->stream;
+        if (c->rec_every)
+            for (uint64_t s = c->sweeps_done; s < c->sweeps_done + n; ++s)
+                if ((s + 1) % c->rec_every == 0) {
+                    c->rec_sweeps[c->rec_count % c->rec_cap] = s + 1;
+                    c->rec_count += 1;
+                }
+        plq::launch_small_chain(a);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail(PLQ_EDEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
+        c->sweeps_done += n;
+        sweeps -= n;
+    }
+    return PLQ_OK;
+}
+
 int check_flag(plq_chain *c) {
     if (c->kind != KIND_CLUSTER) return PLQ_OK;
     int flag = 0;
@@ -478,6 +516,13 @@ int plq_chain_create_local(const plq_local_desc *d, const uint8_t *start, size_t
         if (v == 1 || v == 4 || v == 16)
             while (c->vec > v) c->vec /= 4;
     }
+    // Small power-of-two Ising lattices run many sweeps per launch in one
+    // workgroup; PLQ_SMALL_CHAIN=0 keeps them on the per-pass kernels (tests
+    // use it to check that both paths produce identical bits).
+    c->small = c->kind == KIND_SITE && d->model == PLQ_MODEL_ISING && all_pow2 &&
+               ns <= plq::small_chain_max_sites(d->dim);
+    if (const char *sm = getenv("PLQ_SMALL_CHAIN"))
+        if (atoi(sm) == 0) c->small = false;
     // Grid cap of a colour pass (8..2048 blocks of 256 lanes); lanes walk
     // their block's span, so a smaller grid gives each lane several chunks
     // to pipeline.  PLQ_PASS_BLOCKS overrides the default.
@@ -594,6 +639,7 @@ int plq_chain_advance(plq_chain *c, uint64_t sweeps) {
     if (!c) return fail(PLQ_EINVAL, "chain is null");
     std::lock_guard<std::mutex> lk(c->mu);
     PLQ_HIP(hipSetDevice(c->device));
+    if (c->kind == KIND_SITE && c->small) return enqueue_small_sweeps(c, sweeps);
     for (uint64_t t = 0; t < sweeps; ++t) {
         int rc = c->kind == KIND_CLUSTER ? enqueue_cluster_sweep(c) : enqueue_local_sweep(c);
         if (rc != PLQ_OK) return rc;

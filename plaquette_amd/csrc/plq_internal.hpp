@@ -61,6 +61,29 @@ void launch_finalize_record(const long long *pa, int64_t na, const long long *pb
 void launch_finalize_columns(const long long *pa, int64_t na, const long long *pb, int64_t nb, int nk, long long *rec,
                              hipStream_t s);
 
+// One-workgroup chain of a small power-of-two Ising lattice (plq_small.hip):
+// nsweeps sweeps in one launch with the lattice held in LDS.
+constexpr int64_t SMALL_MAX_SITES = 8192;
+struct SmallChainArgs {
+    uint8_t *c0, *c1;     // colour arrays, read at the start and written back at the end
+    int D;
+    uint32_t lg[MAX_D];   // log2 of each extent; L0 >= 2
+    int64_t N;
+    int mode;             // 0 Metropolis, 1 heat bath
+    uint32_t seed;
+    uint32_t sweep0;      // chain sweep counter of the first sweep
+    uint32_t nsweeps;
+    const uint32_t *table;
+    long long *rec;       // record ring (REC_K per slot), rec_cap slots
+    uint64_t rec_cap;
+    uint64_t rec_first;   // ring position of this launch's first record
+    uint64_t every;       // record after sweep t iff (phase + t + 1) % every == 0; 0 = never
+    uint64_t phase;
+    hipStream_t stream;
+};
+int64_t small_chain_max_sites(int D);
+void launch_small_chain(const SmallChainArgs &a);
+
 // ------------------------------------------------------------------ links ---
 
 struct LinkPassArgs {

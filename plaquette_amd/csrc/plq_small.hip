@@ -1,0 +1,215 @@
+// plq_small.hip -- the whole chain of a small Ising lattice in one workgroup.
+//
+// For lattices of a few thousand sites (2D 64^2: 4,096) a colour pass is a
+// few microseconds of launch overhead around well under a microsecond of
+// work.  This kernel keeps both colour arrays in LDS and runs many sweeps of
+// the reference checkerboard (site_schedule.wgsl:15-31, updater.rs:316-324)
+// in one launch: colour 0, barrier, colour 1, barrier, per sweep.  The draws
+// are the same keyed draws (rng.wgsl:47-52) keyed by the global site index
+// and the chain's sweep counter, and the decisions the same integer
+// thresholds as the multi-launch kernels, so the chain is bit-identical to
+// them whatever path runs it.
+//
+// Each lane owns K sites of each colour for the whole launch and computes
+// their LDS offsets once: per site one word 2*offset + parity (which is also
+// the site's lattice index, since site = r*L0 + 2*i + parity and
+// offset = r*L0/2 + i) and, per axis, the two neighbour offsets packed in one
+// word.  A recorded sweep also sums the bond terms (colour-1 sites against
+// their updated colour-0 neighbours, so every bond once) and the down count,
+// reduced across the block into one record of the chain's ring
+// (ising/mod.rs:97-117, 177-186).
+#include <hip/hip_runtime.h>
+
+#include "plq_common.hpp"
+#include "plq_internal.hpp"
+
+namespace plq {
+
+namespace {
+
+constexpr int SMALL_THREADS = 1024;
+constexpr uint32_t NO_SITE = 0xffffffffu;
+
+struct SmallGeom {
+    uint32_t lg[MAX_D];     // log2 of each extent (0 for unused axes)
+    uint32_t rshift[MAX_D]; // bit position of axis mu >= 1 within the row index
+    uint32_t half;          // sites of one colour (N / 2)
+};
+
+template <int D, int K, int MODE>
+__global__ __launch_bounds__(SMALL_THREADS) void small_chain_kernel(uint8_t *__restrict__ c0g,
+                                                                    uint8_t *__restrict__ c1g, SmallGeom sg,
+                                                                    uint32_t seed, uint32_t sweep0, uint32_t nsweeps,
+                                                                    const uint32_t *__restrict__ table,
+                                                                    long long *__restrict__ rec, uint64_t rec_cap,
+                                                                    uint64_t rec_first, uint64_t every,
+                                                                    uint64_t phase) {
+    constexpr int NN = 2 * D;
+    __shared__ uint8_t lat[2][SMALL_MAX_SITES / 2];
+    __shared__ uint32_t tab[2 * (NN + 1)];
+    __shared__ int red[SMALL_THREADS / 64][2];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t half = sg.half;
+
+    for (uint32_t i = tid; i < half; i += SMALL_THREADS) {
+        lat[0][i] = c0g[i];
+        lat[1][i] = c1g[i];
+    }
+    if (tid < 2 * (NN + 1)) tab[tid] = table[tid];
+
+    // Per-lane site words: own[c][k] = 2*offset + parity (= lattice index),
+    // nb[c][k][mu] = (offset of the - neighbour << 16) | offset of the + one.
+    uint32_t own[2][K];
+    uint32_t nb[2][K][D];
+    const uint32_t lh = sg.lg[0] - 1;  // log2(L0 / 2)
+    const uint32_t L0m = (1u << sg.lg[0]) - 1u;
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            uint32_t w = tid + (uint32_t)k * SMALL_THREADS;
+            if (w >= half) {
+                own[c][k] = NO_SITE;
+#pragma unroll
+                for (int mu = 0; mu < D; ++mu) nb[c][k][mu] = 0;
+                continue;
+            }
+            uint32_t r = w >> lh;
+            uint32_t i = w & ((1u << lh) - 1u);
+            uint32_t prs = 0;
+#pragma unroll
+            for (int mu = 1; mu < D; ++mu) prs += (r >> sg.rshift[mu]) & ((1u << sg.lg[mu]) - 1u);
+            uint32_t p = (prs + (uint32_t)c) & 1u;
+            uint32_t x0 = 2 * i + p;
+            uint32_t rowbase = r << lh;
+            own[c][k] = 2 * w + p;
+            uint32_t xp = (x0 + 1) & L0m, xm = (x0 - 1) & L0m;
+            nb[c][k][0] = (rowbase + (xp >> 1)) | ((rowbase + (xm >> 1)) << 16);
+#pragma unroll
+            for (int mu = 1; mu < D; ++mu) {
+                uint32_t Lm = (1u << sg.lg[mu]) - 1u;
+                uint32_t x = (r >> sg.rshift[mu]) & Lm;
+                uint32_t rp = r + ((((x + 1) & Lm) - x) << sg.rshift[mu]);
+                uint32_t rm = r + ((((x - 1) & Lm) - x) << sg.rshift[mu]);
+                nb[c][k][mu] = ((rp << lh) + i) | (((rm << lh) + i) << 16);
+            }
+        }
+    __syncthreads();
+
+    uint64_t togo = every ? every - 1 - phase % every : 0;  // sweeps before the next record
+    uint64_t nrec = 0;
+    for (uint32_t t = 0; t < nsweeps; ++t) {
+        const uint32_t skey = (sweep0 + t) * GOLDEN;
+        const bool record = every != 0 && togo == 0;
+        int bond = 0, ndown = 0;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            uint8_t *mine = lat[c];
+            const uint8_t *oth = lat[1 - c];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const uint32_t op = own[c][k];
+                if (op == NO_SITE) continue;
+                int d = 0;
+#pragma unroll
+                for (int mu = 0; mu < D; ++mu) {
+                    uint32_t pr = nb[c][k][mu];
+                    d += oth[pr & 0xffffu] + oth[pr >> 16];
+                }
+                const uint32_t off = op >> 1;
+                const int cur = mine[off];
+                const uint32_t th = tab[cur * (NN + 1) + d];
+                bool hit;
+                if (th == 0u) hit = false;
+                else if (th >= DRAW_ONE) hit = true;
+                else hit = keyed_bits_k(seed, op, skey) < th;
+                const int nxt = MODE == 0 ? (hit ? 1 - cur : cur) : (hit ? 0 : 1);
+                mine[off] = (uint8_t)nxt;
+                if (record) {
+                    ndown += nxt;
+                    if (c == 1) bond += (1 - 2 * nxt) * (NN - 2 * d);
+                }
+            }
+            __syncthreads();
+        }
+        if (record) {
+            for (int o = 32; o > 0; o >>= 1) {
+                bond += __shfl_xor(bond, o, 64);
+                ndown += __shfl_xor(ndown, o, 64);
+            }
+            if ((tid & 63) == 0) {
+                red[tid >> 6][0] = bond;
+                red[tid >> 6][1] = ndown;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                long long b = 0, nd = 0;
+                for (int wv = 0; wv < SMALL_THREADS / 64; ++wv) {
+                    b += red[wv][0];
+                    nd += red[wv][1];
+                }
+                long long *out = rec + ((rec_first + nrec) % rec_cap) * REC_K;
+                out[0] = b;
+                out[1] = 0;
+                out[2] = 2 * (long long)half - nd;
+                out[3] = nd;
+                for (int q = 4; q < REC_K; ++q) out[q] = 0;
+            }
+            nrec += 1;
+            togo = every - 1;
+        } else if (every != 0) {
+            togo -= 1;
+        }
+    }
+
+    for (uint32_t i = tid; i < half; i += SMALL_THREADS) {
+        c0g[i] = lat[0][i];
+        c1g[i] = lat[1][i];
+    }
+}
+
+template <int D, int MODE>
+void launch_small_k(const SmallChainArgs &a, const SmallGeom &sg, int k) {
+    dim3 grid(1), block(SMALL_THREADS);
+#define PLQ_SMALL(KK)                                                                                           \
+    small_chain_kernel<D, KK, MODE><<<grid, block, 0, a.stream>>>(a.c0, a.c1, sg, a.seed, a.sweep0, a.nsweeps, \
+                                                                  a.table, a.rec, a.rec_cap, a.rec_first,      \
+                                                                  a.every, a.phase)
+    if (k <= 1) PLQ_SMALL(1);
+    else if (k <= 2) PLQ_SMALL(2);
+    else PLQ_SMALL(4);
+#undef PLQ_SMALL
+}
+
+template <int D>
+void launch_small_d(const SmallChainArgs &a, const SmallGeom &sg, int k) {
+    if (a.mode == 0) launch_small_k<D, 0>(a, sg, k);
+    else launch_small_k<D, 1>(a, sg, k);
+}
+
+}  // namespace
+
+// Up to four sites of each colour per lane (more spills registers in 4D).
+int64_t small_chain_max_sites(int D) { return D <= 3 ? SMALL_MAX_SITES : SMALL_MAX_SITES / 2; }
+
+void launch_small_chain(const SmallChainArgs &a) {
+    SmallGeom sg{};
+    uint32_t shift = 0;
+    for (int mu = 0; mu < MAX_D; ++mu) {
+        sg.lg[mu] = mu < a.D ? a.lg[mu] : 0;
+        if (mu >= 1) {
+            sg.rshift[mu] = shift;
+            shift += sg.lg[mu];
+        }
+    }
+    sg.half = (uint32_t)(a.N / 2);
+    int k = (int)((sg.half + SMALL_THREADS - 1) / SMALL_THREADS);
+    switch (a.D) {
+        case 1: launch_small_d<1>(a, sg, k); break;
+        case 2: launch_small_d<2>(a, sg, k); break;
+        case 3: launch_small_d<3>(a, sg, k); break;
+        default: launch_small_d<4>(a, sg, k); break;
+    }
+}
+
+}  // namespace plq

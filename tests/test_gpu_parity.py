@@ -154,6 +154,33 @@ def test_every_launch_shape_is_bit_exact_against_the_oracle(gpu, monkeypatch, ve
         assert np.array_equal(got, st)
 
 
+SMALL_CASES = [([64, 64], M), ([64, 64], HB), ([16], M), ([2, 4], HB), ([16, 16, 16], M), ([8, 8, 4, 4], HB),
+               ([128, 64], M), ([4, 2, 8], HB)]
+
+
+@pytest.mark.parametrize("shape,kernel", SMALL_CASES, ids=[f"{'x'.join(map(str, s))}-{k}" for s, k in SMALL_CASES])
+def test_small_chain_matches_per_pass_kernels(gpu, monkeypatch, shape, kernel):
+    # Small power-of-two lattices run whole sweeps in one workgroup; the
+    # per-pass kernels are an independent implementation.  Advances of 5,
+    # 4100 and 7 sweeps cross the 4096-sweep launch limit and shift the
+    # recording phase; the ring holds the last 40 of 1370 records.
+    n = int(np.prod(shape))
+    start = start_config(n, 2, True, 21)
+    outs = []
+    for small in ("1", "0"):
+        monkeypatch.setenv("PLQ_SMALL_CHAIN", small)
+        c = gpu.ising_gpu_chain(shape, 1.0, 0.1, 0.42, 11, start, kernel=kernel)
+        c.record(3, 40)
+        for k in (5, 4100, 7):
+            c.advance(k)
+        outs.append((c.read(), [(o.sweep, o.bond_sum, o.spin_sum) for o in c.series()], c.sweeps_done))
+        c.close()
+    assert outs[0][2] == outs[1][2] == 4112
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert outs[0][1] == outs[1][1]
+    assert [s for s, _, _ in outs[0][1]] == list(range(4113 - 3 * 40, 4113, 3))
+
+
 def test_recorded_series_matches_per_sweep_measurement(gpu):
     shape = [64, 64]
     n = 4096
