@@ -212,6 +212,56 @@ int plq_ensemble_stream(plq_ensemble *e, void **stream);
 int plq_ensemble_sync(plq_ensemble *e);
 int plq_ensemble_destroy(plq_ensemble *e);
 
+/* ---------------------------------------------------------------------------
+ * Learned block proposal (no reference counterpart: the reference's local
+ * updates change one variable at a time, updater.rs:291-347).  A 2D Ising
+ * chain whose extents are multiples of 16 is cut into 8x8 blocks coloured
+ * (bx + by) & 1.  A 3-layer CNN (3x3 kernels, 32 channels, tanh; BF16 weights
+ * and activations, FP32 accumulation) maps the 32 spins of a block's boundary
+ * ring to 64 logits, and the proposal draws the block's interior as 64
+ * independent Bernoulli variables, state 0 with probability sigmoid(logit).
+ * The move is accepted by Metropolis-Hastings with the true Ising action
+ * (ising/mod.rs:97-143): A = beta (J dBond + h dMag) + sum_p ([x_p = 0] -
+ * [x'_p = 0]) logit_p, accepted iff A >= 0 or m < exp(A) 2^24.  Draws are
+ * keyed like the chain's own (rng.wgsl:47-52): proposal bits by
+ * (seed ^ 0x3c6ef372, site, block sweep), acceptance by (seed ^ 0xa54ff53a,
+ * block index by * L0/8 + bx, block sweep); the acceptance arithmetic is IEEE
+ * single with a deterministic exp, reproducible bit for bit given the logits.
+ * Parameters (PLQ_BN_PARAMS floats): W1[9][32], b1[32], W2[9][32 out][32 in],
+ * b2[32], W3[9][32], b3, tap t = (dy + 1) * 3 + (dx + 1).
+ * Calls that change the network wait for all work on the device first.
+ * ------------------------------------------------------------------------- */
+#define PLQ_BN_PARAMS 9857
+typedef struct plq_blocknet plq_blocknet;
+
+/* Glorot-uniform weights drawn from a splitmix64 stream of `seed`, zero biases. */
+int plq_blocknet_create(uint64_t seed, plq_blocknet **out);
+int plq_blocknet_params(plq_blocknet *net, float *out, size_t n);
+/* Replace the parameters and reset the Adam state. */
+int plq_blocknet_set_params(plq_blocknet *net, const float *in, size_t n);
+/* Mean binary cross-entropy of every block's interior given its ring over the
+ * chain's current configuration, and its gradient. */
+int plq_blocknet_gradient(plq_blocknet *net, plq_chain *chain, float *grad, size_t n, double *loss);
+/* `steps` times: advance the chain `sweeps_between` sweeps, then one Adam step
+ * (0.9, 0.999, 1e-8) on the gradient over its blocks.  losses (nullable) gets
+ * each step's mean loss. */
+int plq_blocknet_train(plq_blocknet *net, plq_chain *chain, uint64_t steps, uint64_t sweeps_between, double lr,
+                       float *losses);
+/* Logits of every block of the chain's current configuration, indexed
+ * (by * L0/8 + bx) * 64 + py * 8 + px. */
+int plq_blocknet_logits(plq_blocknet *net, plq_chain *chain, float *out, size_t n);
+int plq_blocknet_destroy(plq_blocknet *net);
+
+/* Enqueue `sweeps` block sweeps (blocks of colour 0, then colour 1). */
+int plq_chain_block_sweeps(plq_chain *chain, plq_blocknet *net, uint64_t sweeps);
+/* Enqueue `steps` times: one block sweep, then `plain_sweeps` sweeps of the
+ * chain's own kernel (recording as plq_chain_record set it up). */
+int plq_chain_advance_with_blocks(plq_chain *chain, plq_blocknet *net, uint64_t steps, uint64_t plain_sweeps);
+/* Accepted and proposed block moves so far, and block sweeps done. */
+int plq_chain_block_stats(plq_chain *chain, uint64_t *accepted, uint64_t *proposed, uint64_t *block_sweeps);
+/* The logits each block used in its latest block pass (layout as above). */
+int plq_chain_block_logits(plq_chain *chain, float *out, size_t n);
+
 /* Host-side helpers shared with the device path, exposed for tests. */
 uint32_t plq_fold_seed(uint64_t seed);
 /* The integer acceptance table the local kernels compare the 24-bit draw

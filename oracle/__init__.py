@@ -128,6 +128,14 @@ def lib():
             "orc_ensemble_sweeps": (C.c_int, [P(EnsembleDesc), P(C.c_uint8), P(C.c_uint32), P(C.c_uint32),
                                               P(C.c_uint64), P(C.c_uint64), C.c_uint64, P(C.c_int64),
                                               P(C.c_uint32), C.c_uint64, P(C.c_uint64), P(C.c_uint64)]),
+            "orc_expf": (C.c_float, [C.c_float]),
+            "orc_bf16": (C.c_float, [C.c_float]),
+            "orc_blocknet_logits": (None, [P(C.c_float), C.c_int, C.c_int64, C.c_int64, P(C.c_uint8),
+                                           P(C.c_double)]),
+            "orc_blocknet_loss_grad": (C.c_double, [P(C.c_float), C.c_int, C.c_int64, C.c_int64, P(C.c_uint8),
+                                                    P(C.c_double)]),
+            "orc_block_mh_pass": (C.c_int64, [C.c_int64, C.c_int64, P(C.c_uint8), P(C.c_float), C.c_int,
+                                              C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double]),
             "orc_pmode_ising_sweeps": (C.c_double, [C.c_int32, P(C.c_int64), C.c_int32, C.c_double, C.c_double,
                                                     C.c_double, C.c_uint64, P(C.c_uint8), C.c_uint32]),
         }
@@ -412,3 +420,58 @@ class Ensemble:
 
     def betas_now(self):
         return self._betas[self.slot_of]
+
+
+# -- learned block proposal (plq_oracle_block.c) --------------------------------
+
+BN_PARAMS = 9857
+BN_CHANNELS = 32
+BLOCK_PROPOSAL_KEY = 0x3C6EF372
+BLOCK_ACCEPT_KEY = 0xA54FF53A
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def expf(x: float) -> float:
+    """orc_expf: the deterministic f32 exponential of the block update."""
+    return float(lib().orc_expf(float(x)))
+
+
+def bf16(x: float) -> float:
+    return float(lib().orc_bf16(float(x)))
+
+
+def blocknet_logits(params, shape, state, round_bf16=True) -> np.ndarray:
+    """orc_blocknet_logits: (n_blocks, 64) logits, FP64."""
+    L0, L1 = shape
+    p, pp = _f32(params)
+    assert p.size == BN_PARAMS
+    st = np.ascontiguousarray(state, dtype=np.uint8)
+    out = np.zeros((L0 // 8) * (L1 // 8) * 64, np.float64)
+    lib().orc_blocknet_logits(pp, int(round_bf16), L0, L1, _u8(st),
+                              out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out.reshape(-1, 64)
+
+
+def blocknet_loss_grad(params, shape, state, round_bf16=True):
+    """orc_blocknet_loss_grad: (mean BCE loss, gradient[BN_PARAMS])."""
+    L0, L1 = shape
+    p, pp = _f32(params)
+    st = np.ascontiguousarray(state, dtype=np.uint8)
+    g = np.zeros(BN_PARAMS, np.float64)
+    loss = lib().orc_blocknet_loss_grad(pp, int(round_bf16), L0, L1, _u8(st),
+                                        g.ctypes.data_as(C.POINTER(C.c_double)))
+    return loss, g
+
+
+def block_mh_pass(shape, state, logits, colour, seed32, bstep, beta, j, h):
+    """orc_block_mh_pass on a copy of state; returns (new state, accepted)."""
+    L0, L1 = shape
+    st = np.array(state, dtype=np.uint8, copy=True)
+    lg, lp = _f32(np.asarray(logits).reshape(-1))
+    acc = lib().orc_block_mh_pass(L0, L1, _u8(st), lp, int(colour),
+                                  int(seed32) & 0xFFFFFFFF, int(bstep) & 0xFFFFFFFF, float(beta), float(j), float(h))
+    return st, int(acc)

@@ -162,6 +162,28 @@ double orc_pmode_ising_sweeps(int32_t dim, const int64_t *shape, int32_t kernel,
                               double beta, double j, double h, uint64_t seed,
                               uint8_t *state, uint32_t n_sweeps);
 
+/* ---- learned block proposal (plq_oracle_block.c) ----------------------------
+ * 2D Ising, 8x8 blocks, block b = by * (L0/8) + bx, block colour (bx + by) & 1,
+ * block site p = py * 8 + px.  Network parameters (FP32, ORC_BN_PARAMS):
+ *   W1[9][32], b1[32], W2[9][32 out][32 in], b2[32], W3[9][32], b3
+ * tap t = (dy + 1) * 3 + (dx + 1); layer outputs tanh except the logits.
+ * The logit of a site is that of state 0 (spin +1).  `round` = 1 reproduces
+ * the device's BF16 rounding of weights and activations. */
+#define ORC_BN_BLOCK 8
+#define ORC_BN_CHANNELS 32
+#define ORC_BN_PARAMS 9857
+#define ORC_BLOCK_PROPOSAL_KEY 0x3c6ef372u
+#define ORC_BLOCK_ACCEPT_KEY 0xa54ff53au
+float orc_expf(float x);
+float orc_bf16(float f);
+void orc_blocknet_logits(const float *params, int round, int64_t L0, int64_t L1, const uint8_t *state,
+                         double *out);
+double orc_blocknet_loss_grad(const float *params, int round, int64_t L0, int64_t L1, const uint8_t *state,
+                              double *grad);
+/* One pass over the blocks of `colour`; returns the number accepted. */
+int64_t orc_block_mh_pass(int64_t L0, int64_t L1, uint8_t *state, const float *logits, int colour, uint32_t seed32,
+                          uint32_t bstep, double beta, double j, double h);
+
 #ifdef __cplusplus
 }
 #endif

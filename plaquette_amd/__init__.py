@@ -192,6 +192,17 @@ SIGNATURES = {
     "plq_ensemble_stream": (C.c_int, [_HANDLE, _P(C.c_void_p)]),
     "plq_ensemble_sync": (C.c_int, [_HANDLE]),
     "plq_ensemble_destroy": (C.c_int, [_HANDLE]),
+    "plq_blocknet_create": (C.c_int, [C.c_uint64, _P(C.c_void_p)]),
+    "plq_blocknet_params": (C.c_int, [C.c_void_p, _P(C.c_float), C.c_size_t]),
+    "plq_blocknet_set_params": (C.c_int, [C.c_void_p, _P(C.c_float), C.c_size_t]),
+    "plq_blocknet_gradient": (C.c_int, [C.c_void_p, C.c_void_p, _P(C.c_float), C.c_size_t, _P(C.c_double)]),
+    "plq_blocknet_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_double, _P(C.c_float)]),
+    "plq_blocknet_logits": (C.c_int, [C.c_void_p, C.c_void_p, _P(C.c_float), C.c_size_t]),
+    "plq_blocknet_destroy": (C.c_int, [C.c_void_p]),
+    "plq_chain_block_sweeps": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "plq_chain_advance_with_blocks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64]),
+    "plq_chain_block_stats": (C.c_int, [C.c_void_p, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64)]),
+    "plq_chain_block_logits": (C.c_int, [C.c_void_p, _P(C.c_float), C.c_size_t]),
     "plq_fold_seed": (C.c_uint32, [C.c_uint64]),
     "plq_threshold_table": (C.c_int, [_P(LocalDesc), _P(C.c_uint32), C.c_size_t]),
 }
@@ -351,6 +362,26 @@ class _Chain:
 class GpuChain(_Chain):
     """GpuChain (device.rs:590-701): a checkerboard chain on one device."""
 
+    # -- learned block proposal (2D Ising, extents multiples of 16) --
+
+    def block_sweeps(self, net: "BlockNet", sweeps: int) -> None:
+        _check(lib().plq_chain_block_sweeps(self._h, net._h, int(sweeps)))
+
+    def advance_with_blocks(self, net: "BlockNet", steps: int, plain_sweeps: int = 1) -> None:
+        _check(lib().plq_chain_advance_with_blocks(self._h, net._h, int(steps), int(plain_sweeps)))
+
+    def block_stats(self):
+        """(accepted, proposed, block sweeps done)."""
+        a, p, b = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+        _check(lib().plq_chain_block_stats(self._h, C.byref(a), C.byref(p), C.byref(b)))
+        return a.value, p.value, b.value
+
+    def block_logits(self) -> np.ndarray:
+        """Logits each block used in its latest pass, shape (n_blocks, 64)."""
+        out = np.zeros(self.n_sites, np.float32)
+        _check(lib().plq_chain_block_logits(self._h, out.ctypes.data_as(_P(C.c_float)), out.size))
+        return out.reshape(-1, 64)
+
     @classmethod
     def build(cls, desc: LocalDesc, start) -> "GpuChain":
         st = np.ascontiguousarray(start, dtype=np.uint8).ravel()
@@ -358,6 +389,59 @@ class GpuChain(_Chain):
         _check(lib().plq_chain_create_local(C.byref(desc), st.ctypes.data_as(_P(C.c_uint8)), st.size, C.byref(h)))
         n_sites = int(np.prod([desc.shape[i] for i in range(desc.dim)]))
         return cls(h, st.size, n_sites, desc.sweeps_between)
+
+
+class BlockNet:
+    """The learned block proposal's network (plq_blocknet_*): 32 boundary
+    spins of an 8x8 block -> 64 logits; PLQ_BN_PARAMS FP32 parameters."""
+
+    N_PARAMS = 9857
+
+    def __init__(self, seed: int = 0):
+        h = C.c_void_p(0)
+        _check(lib().plq_blocknet_create(int(seed) & 0xFFFFFFFFFFFFFFFF, C.byref(h)))
+        self._h = h
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.N_PARAMS, np.float32)
+        _check(lib().plq_blocknet_params(self._h, out.ctypes.data_as(_P(C.c_float)), out.size))
+        return out
+
+    @params.setter
+    def params(self, value) -> None:
+        v = np.ascontiguousarray(value, dtype=np.float32).ravel()
+        _check(lib().plq_blocknet_set_params(self._h, v.ctypes.data_as(_P(C.c_float)), v.size))
+
+    def gradient(self, chain: GpuChain):
+        """(mean loss, gradient) over every block of the chain's configuration."""
+        g = np.zeros(self.N_PARAMS, np.float32)
+        loss = C.c_double(0)
+        _check(lib().plq_blocknet_gradient(self._h, chain._h, g.ctypes.data_as(_P(C.c_float)), g.size,
+                                           C.byref(loss)))
+        return loss.value, g
+
+    def train(self, chain: GpuChain, steps: int, sweeps_between: int, lr: float) -> np.ndarray:
+        losses = np.zeros(max(int(steps), 1), np.float32)
+        _check(lib().plq_blocknet_train(self._h, chain._h, int(steps), int(sweeps_between), float(lr),
+                                        losses.ctypes.data_as(_P(C.c_float))))
+        return losses[: int(steps)]
+
+    def logits(self, chain: GpuChain) -> np.ndarray:
+        out = np.zeros(chain.n_sites, np.float32)
+        _check(lib().plq_blocknet_logits(self._h, chain._h, out.ctypes.data_as(_P(C.c_float)), out.size))
+        return out.reshape(-1, 64)
+
+    def close(self) -> None:
+        if self._h:
+            lib().plq_blocknet_destroy(self._h)
+            self._h = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class GpuClusterChain(_Chain):

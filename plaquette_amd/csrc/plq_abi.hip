@@ -17,6 +17,7 @@
 
 #include "../../include/plaquette_hip.h"
 #include "plq_common.hpp"
+#include "plq_chain.hpp"
 #include "plq_internal.hpp"
 #include "plq_tables.hpp"
 
@@ -44,8 +45,6 @@ int host_fail(int code, const std::string &msg) { return fail(code, msg); }
 
 namespace {
 
-enum Kind { KIND_SITE = 0, KIND_LINK = 1, KIND_CLUSTER = 2 };
-
 bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
 uint32_t ilog2(int64_t v) {
     uint32_t r = 0;
@@ -55,64 +54,6 @@ uint32_t ilog2(int64_t v) {
 
 }  // namespace
 
-struct plq_chain {
-    std::mutex mu;
-    Kind kind;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int D = 0;
-    int64_t shape[PLQ_MAX_DIM] = {0};
-    int64_t n_sites = 0, n_vars = 0;
-    int model = 0, q = 2, kernel = 0, profile = 0;
-    double beta = 0, j = 1, h = 0;
-    double offsets[PLQ_MAX_Q] = {0};
-    uint32_t seed32 = 0;
-    uint64_t sweeps_done = 0, sweeps_between = 1;
-    // local fields
-    plq::Geom g{};
-    plq::SwarGeom sg{};
-    int vec = 1;                           // sites per lane of the site kernel
-    bool small = false;                    // whole sweeps in one workgroup (plq_small.hip)
-    int64_t pass_blocks = plq::MAX_PASS_BLOCKS;  // grid cap of a colour pass
-    uint8_t *col[2] = {nullptr, nullptr};  // site colours
-    uint8_t *links[2 * plq::MAX_D] = {nullptr};
-    uint8_t **d_links = nullptr;           // device copy of the link array pointers
-    uint32_t *d_table = nullptr;
-    size_t table_len = 0;
-    // cluster
-    plq::CCGeom cg{};
-    uint8_t *labels = nullptr;
-    int32_t *parent = nullptr;
-    int32_t *roots = nullptr;
-    int32_t *sizes = nullptr;
-    unsigned long long *hist = nullptr;
-    int *d_flag = nullptr;
-    uint32_t bond_t = 0;
-    int seeded = 0, forced = 0;
-    uint64_t pass_cap = 0;
-    int64_t last_relabel_blocks = 0;
-    bool have_partition = false;
-    // shared scratch
-    uint8_t *d_lat = nullptr;              // lattice-order staging
-    long long *d_partials = nullptr;       // 2 * MAX_PASS_BLOCKS * REC_K
-    long long *d_meas = nullptr;           // REC_K (+ cluster stats)
-    long long *d_rec = nullptr;            // rec_cap * REC_K
-    uint64_t rec_every = 0, rec_cap = 0, rec_count = 0;
-    std::vector<uint64_t> rec_sweeps;
-    std::vector<hipEvent_t> pending;
-
-    ~plq_chain() {
-        (void)hipSetDevice(device);
-        if (stream) (void)hipStreamSynchronize(stream);
-        for (auto p : {(void *)col[0], (void *)col[1], (void *)d_links, (void *)d_table, (void *)labels,
-                       (void *)parent, (void *)roots, (void *)sizes, (void *)hist, (void *)d_flag, (void *)d_lat,
-                       (void *)d_partials, (void *)d_meas, (void *)d_rec})
-            if (p) (void)hipFree(p);
-        for (int k = 0; k < 2 * plq::MAX_D; ++k)
-            if (links[k]) (void)hipFree(links[k]);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
 
 namespace {
 

