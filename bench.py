@@ -307,11 +307,81 @@ def bench_ising2d_64(P, ev, steps, warmup):
     }
 
 
+BLOCK_FLOP = 1253376  # network FLOPs per 8x8 block: 64 x 32 x 9 x 2 (conv1) + 64 x 32 x 288 x 2 (conv2) + 64 x 288 x 2 (conv3)
+
+
+def bench_learned(P, ev):
+    """2D Ising 512^2 at beta_c: the learned block proposal against the plain
+    checkerboard.  A chain from a hot start runs 2,000 sweeps, then trains
+    the network 1,000 Adam steps, one per configuration taken every 5
+    sweeps (the plain sampler's configurations).  From the trained chain's
+    configuration, a plain chain (one checkerboard sweep per step) and a
+    mixed chain (one block sweep of both block colours, then one
+    checkerboard sweep, per step) each run the same number of steps with
+    |M| recorded every step; tau_int of |M| comes from Sokal's window
+    (statistics.rs:83-137) and samples/s = steps/s / (2 tau_int)."""
+    import numpy as np
+
+    from plaquette_amd import statistics as stats
+
+    L, beta, steps = 512, 0.4406868, 20000
+    n = L * L
+    start = np.random.default_rng(SEED).integers(0, 2, n).astype(np.uint8)
+    c = P.ising_gpu_chain([L, L], 1.0, 0.0, beta, SEED, start)
+    c.advance(2000)
+    net = P.BlockNet(SEED)
+    wall_train, ms_train = timed(P, ev, c.stream, lambda: net.train(c, 1000, 5, 3e-3))
+    losses = net.train(c, 1, 5, 3e-3)
+    st = c.read()
+    # the block sweep alone, by events on its stream
+    k = 50
+    c.block_sweeps(net, 5)
+    c.sync()
+    _, ms_blocks = timed(P, ev, c.stream, lambda: (c.block_sweeps(net, k), c.sync()))
+    block_us = ms_blocks * 1e3 / k
+    c.close()
+    runs = {}
+    for mode in ("plain", "mixed"):
+        ch = P.ising_gpu_chain([L, L], 1.0, 0.0, beta, SEED + 1, st)
+        ch.record(1, steps)
+        ch.sync()
+        if mode == "plain":
+            wall, _ = timed(P, ev, ch.stream, lambda: (ch.advance(steps), ch.sync()))
+        else:
+            wall, _ = timed(P, ev, ch.stream, lambda: (ch.advance_with_blocks(net, steps, 1), ch.sync()))
+        m = np.array([abs(o.spin_sum) / n for o in ch.series()])
+        est = stats.reduce(m)
+        acc, prop, _ = ch.block_stats()
+        runs[mode] = {"steps_per_s": steps / wall, "tau_int_abs_m": est.tau_int, "mean_abs_m": est.mean,
+                      "samples_per_s": steps / wall / (2.0 * est.tau_int),
+                      "window_converged": bool(steps >= 50 * est.tau_int),
+                      "block_acceptance": (acc / prop) if prop else None}
+        ch.close()
+    net.close()
+    ratio = runs["mixed"]["samples_per_s"] / runs["plain"]["samples_per_s"]
+    flop = (n // 64) * BLOCK_FLOP
+    tflops = flop / (block_us * 1e-6) / 1e12
+    return {
+        "metric": "tau_int-corrected samples/s of |M|, learned block proposal vs plain checkerboard",
+        "value": round(runs["mixed"]["samples_per_s"], 4), "unit": "samples/s", "n_gpus": 1, "steps": steps,
+        "warmup": 2000, "ms_per_step": round(1e3 / runs["mixed"]["steps_per_s"], 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16 network, fp32 accumulate; u8 states",
+        "data": "synthetic (chain configurations)",
+        "config": {"workload": "2D Ising 512^2 at beta_c = 0.4406868, 4,096 blocks of 8x8, 3-layer 3x3 CNN with "
+                               "32 channels, 1,000 training steps", "lattice": [L, L]},
+        "ratio_vs_plain": round(ratio, 4), "plain": runs["plain"], "mixed": runs["mixed"],
+        "training": {"wall_s": round(wall_train, 3), "final_loss": float(losses[-1])},
+        "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(tflops / 2500.0, 5), "traffic": None,
+                     "kernel": "block_pass_kernel (both block colours)", "avg_sweep_us": round(block_us, 2)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--profile", action="store_true",
                     help="run the fixed counter-collection workload instead of the benchmark")
-    ap.add_argument("--workload", default="ising3d", choices=["ising3d", "potts-sw", "z2-batch", "ising2d-64"],
+    ap.add_argument("--workload", default="ising3d", choices=["ising3d", "potts-sw", "z2-batch", "ising2d-64", "learned"],
                     help="ising3d is the headline line; the others time the remaining configurations")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
@@ -341,6 +411,9 @@ def main():
         return
     if args.workload == "z2-batch":
         print(json.dumps(bench_z2_batch(P, ev, 500, 20)))
+        return
+    if args.workload == "learned":
+        print(json.dumps(bench_learned(P, ev)))
         return
     if args.workload == "ising2d-64":
         print(json.dumps(bench_ising2d_64(P, ev, 10000, 1000)))

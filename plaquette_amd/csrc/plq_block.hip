@@ -57,6 +57,12 @@ __device__ __forceinline__ bf16x8 lds_frag(const uint16_t *p) {
 }
 __device__ __forceinline__ bf16x8 zero_frag() { return as_bf16x8(u32x4{0u, 0u, 0u, 0u}); }
 
+// tanh(x) = 1 - 2 / (1 + e^{2x}) with the hardware exponential and
+// reciprocal: absolute error ~1e-7, far below the BF16 step of an activation.
+__device__ __forceinline__ float tanh_fast(float x) {
+    return 1.0f - 2.0f * __frcp_rn(1.0f + __expf(2.0f * x));
+}
+
 __device__ __forceinline__ bool inside(int y, int x) { return (unsigned)y < (unsigned)B && (unsigned)x < (unsigned)B; }
 
 // State of lattice site (X, Y) in the colour-interleaved layout, periodic.
@@ -100,7 +106,7 @@ __device__ __forceinline__ void layer1(const BlockNetView &n, const float *patch
             a0 += n.w1[t * CH + c] * in[t];
             a1 += n.w1[t * CH + c + 1] * in[t];
         }
-        uint32_t pk = (uint32_t)bf16_bits(tanhf(a0)) | ((uint32_t)bf16_bits(tanhf(a1)) << 16);
+        uint32_t pk = (uint32_t)bf16_bits(tanh_fast(a0)) | ((uint32_t)bf16_bits(tanh_fast(a1)) << 16);
         *reinterpret_cast<uint32_t *>(&h1[lane * HS + c]) = pk;
     }
 }
@@ -138,8 +144,8 @@ __device__ __forceinline__ void layer2(const BlockNetView &n, const bf16x8 *wB, 
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int q = 16 * mt + 4 * (lane >> 4) + i;
-            h2[q * HS + col] = bf16_bits(tanhf(acc0[i] + n.b2[col]));
-            h2[q * HS + 16 + col] = bf16_bits(tanhf(acc1[i] + n.b2[16 + col]));
+            h2[q * HS + col] = bf16_bits(tanh_fast(acc0[i] + n.b2[col]));
+            h2[q * HS + 16 + col] = bf16_bits(tanh_fast(acc1[i] + n.b2[16 + col]));
         }
     }
 }
@@ -180,8 +186,9 @@ __global__ __launch_bounds__(256) void block_pass_kernel(uint8_t *__restrict__ c
                                                          int L0, int L1, int mode, int colour, uint32_t seed32,
                                                          uint32_t bstep, double beta, double jc, double hc,
                                                          BlockNetView net, float *__restrict__ logits_out,
-                                                         unsigned long long *__restrict__ counters) {
+                                                         unsigned long long *__restrict__ acc_slots) {
     __shared__ Wave waves[4];
+    __shared__ int wave_acc[4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     Wave &W = waves[w];
     const int nb0 = L0 / B, nb1 = L1 / B;
@@ -242,11 +249,16 @@ __global__ __launch_bounds__(256) void block_pass_kernel(uint8_t *__restrict__ c
                 int64_t idx = (int64_t)Y * (L0 >> 1) + (X >> 1);
                 (((X + Y) & 1) ? c1 : c0)[idx] = (uint8_t)prop;
             }
-            if (lane == 0) {
-                atomicAdd(&counters[0], (unsigned long long)(acc ? 1 : 0));
-                atomicAdd(&counters[1], 1ull);
-            }
+            if (lane == 0) wave_acc[w] = acc ? 1 : 0;
         }
+    }
+    // Accepted moves per workgroup, accumulated in the workgroup's own slot:
+    // passes run in stream order, so the read-modify-write needs no atomic.
+    if (mode == 0) {
+        if (!active && lane == 0) wave_acc[w] = 0;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            acc_slots[blockIdx.x] += (unsigned long long)(wave_acc[0] + wave_acc[1] + wave_acc[2] + wave_acc[3]);
     }
 }
 
@@ -314,6 +326,7 @@ __global__ __launch_bounds__(64) void block_train_kernel(const uint8_t *__restri
             if (t >= 9) break;
             const int dy = t / 3 - 1, dx = t % 3 - 1;
             float s = 0.f;
+#pragma unroll 8
             for (int p = 0; p < 64; ++p) {
                 const int yy = (p >> 3) + dy, xx = (p & 7) + dx;
                 if (inside(yy, xx)) s += S.g3[p] * bf16_float(S.u.a.h2[(yy * B + xx) * HS + ec]);
@@ -413,6 +426,7 @@ __global__ __launch_bounds__(64) void block_train_kernel(const uint8_t *__restri
             const int t = et0 + 2 * i;
             if (t >= 9) break;
             float s = 0.f;
+#pragma unroll 8
             for (int q = 0; q < 64; ++q)
                 s += S.u.g1[q * CH + ec] * S.patch[((q >> 3) + t / 3) * 10 + (q & 7) + t % 3];
             a_w1[i] += s;
@@ -457,13 +471,27 @@ __global__ __launch_bounds__(64) void block_train_kernel(const uint8_t *__restri
 }
 
 // grad[e] = (sum over workgroups of part[g][e]) / nsites, e <= BN_PARAMS (the
-// last entry is the mean loss).
-__global__ void block_reduce_kernel(const float *__restrict__ part, int nwg, int stride, float inv_sites,
-                                    float *__restrict__ grad) {
+// last entry is the mean loss), in two fixed-order stages: stage 1 sums runs
+// of REDUCE_RUN workgroups into part2[run][e], stage 2 sums the runs.
+constexpr int REDUCE_RUN = 16;
+
+__global__ void block_reduce1_kernel(const float *__restrict__ part, int nwg, int stride,
+                                     float *__restrict__ part2) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int run = blockIdx.y;
+    if (e > BN_PARAMS) return;
+    const int g0 = run * REDUCE_RUN, g1 = min(nwg, g0 + REDUCE_RUN);
+    float s = 0.f;
+    for (int g = g0; g < g1; ++g) s += part[(int64_t)g * stride + e];
+    part2[(int64_t)run * stride + e] = s;
+}
+
+__global__ void block_reduce2_kernel(const float *__restrict__ part2, int nruns, int stride, float inv_sites,
+                                     float *__restrict__ grad) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e > BN_PARAMS) return;
     float s = 0.f;
-    for (int g = 0; g < nwg; ++g) s += part[(int64_t)g * stride + e];
+    for (int r = 0; r < nruns; ++r) s += part2[(int64_t)r * stride + e];
     grad[e] = s * inv_sites;
 }
 
@@ -509,8 +537,10 @@ void launch_block_pass(const BlockPassArgs &a) {
     const int nb = (a.L0 / BN_BLOCK) * (a.L1 / BN_BLOCK);
     const int count = a.mode == 0 ? nb / 2 : nb;
     block_pass_kernel<<<dim3((count + 3) / 4), dim3(256), 0, a.stream>>>(
-        a.c0, a.c1, a.L0, a.L1, a.mode, a.colour, a.seed32, a.bstep, a.beta, a.j, a.h, a.net, a.logits, a.counters);
+        a.c0, a.c1, a.L0, a.L1, a.mode, a.colour, a.seed32, a.bstep, a.beta, a.j, a.h, a.net, a.logits, a.acc_slots);
 }
+
+int block_pass_groups(int L0, int L1) { return ((L0 / BN_BLOCK) * (L1 / BN_BLOCK) / 2 + 3) / 4; }
 
 int block_train_groups(int L0, int L1) {
     const int nb = (L0 / BN_BLOCK) * (L1 / BN_BLOCK);
@@ -523,8 +553,12 @@ void launch_block_grad(const BlockGradArgs &a) {
     const int per = (nb + nwg - 1) / nwg;
     block_train_kernel<<<dim3(nwg), dim3(64), 0, a.stream>>>(a.c0, a.c1, a.L0, a.L1, per, a.net, a.partials,
                                                              BLOCK_PART_STRIDE);
-    block_reduce_kernel<<<dim3((BN_PARAMS + 1 + 255) / 256), dim3(256), 0, a.stream>>>(
-        a.partials, nwg, BLOCK_PART_STRIDE, 1.0f / (float)((int64_t)a.L0 * a.L1), a.grad);
+    const int nruns = (nwg + REDUCE_RUN - 1) / REDUCE_RUN;
+    float *part2 = a.partials + (int64_t)BLOCK_TRAIN_GROUPS * BLOCK_PART_STRIDE;
+    block_reduce1_kernel<<<dim3((BN_PARAMS + 1 + 255) / 256, nruns), dim3(256), 0, a.stream>>>(
+        a.partials, nwg, BLOCK_PART_STRIDE, part2);
+    block_reduce2_kernel<<<dim3((BN_PARAMS + 1 + 255) / 256), dim3(256), 0, a.stream>>>(
+        part2, nruns, BLOCK_PART_STRIDE, 1.0f / (float)((int64_t)a.L0 * a.L1), a.grad);
 }
 
 void launch_block_adam(float *w, float *m, float *v, const float *grad, float lr, float beta1, float beta2,

@@ -23,7 +23,7 @@ struct plq_blocknet {
     float *w1r = nullptr, *w3r = nullptr;            // BF16-rounded W1, W3
     uint16_t *w2f = nullptr, *w2t = nullptr;         // W2 fragments (forward, input gradient)
     float *grad = nullptr;                           // BN_PARAMS + 1
-    float *partials = nullptr;                       // BLOCK_TRAIN_GROUPS * BLOCK_PART_STRIDE
+    float *partials = nullptr;                       // per-workgroup gradient sums, then per-run sums
     float *losses = nullptr;                         // per training step
     size_t losses_cap = 0;
     uint64_t adam_steps = 0;
@@ -105,8 +105,9 @@ int check_chain(const plq_chain *c) {
 
 int ensure_chain_buffers(plq_chain *c) {
     if (!c->d_block_counters) {
-        PLQ_TRY(alloc((void **)&c->d_block_counters, 2 * sizeof(unsigned long long)));
-        PLQ_HIP(hipMemsetAsync(c->d_block_counters, 0, 2 * sizeof(unsigned long long), c->stream));
+        const size_t slots = (size_t)plq::block_pass_groups((int)c->shape[0], (int)c->shape[1]);
+        PLQ_TRY(alloc((void **)&c->d_block_counters, slots * sizeof(unsigned long long)));
+        PLQ_HIP(hipMemsetAsync(c->d_block_counters, 0, slots * sizeof(unsigned long long), c->stream));
     }
     if (!c->d_block_logits) PLQ_TRY(alloc((void **)&c->d_block_logits, sizeof(float) * (size_t)c->n_sites));
     return PLQ_OK;
@@ -147,7 +148,7 @@ int enqueue_block_sweep(plq_chain *c, plq_blocknet *n) {
         a.h = c->h;
         a.net = n->view();
         a.logits = c->d_block_logits;
-        a.counters = c->d_block_counters;
+        a.acc_slots = c->d_block_counters;
         a.stream = c->stream;
         plq::launch_block_pass(a);
         PLQ_TRY(launch_status());
@@ -183,8 +184,8 @@ int plq_blocknet_create(uint64_t seed, plq_blocknet **out) {
     if ((rc = alloc((void **)&n->w2f, plq::BLOCK_W2_FRAG * sizeof(uint16_t))) != PLQ_OK) return bail(rc);
     if ((rc = alloc((void **)&n->w2t, plq::BLOCK_W2_FRAG * sizeof(uint16_t))) != PLQ_OK) return bail(rc);
     if ((rc = alloc((void **)&n->grad, (P + 1) * sizeof(float))) != PLQ_OK) return bail(rc);
-    if ((rc = alloc((void **)&n->partials,
-                    (size_t)plq::BLOCK_TRAIN_GROUPS * plq::BLOCK_PART_STRIDE * sizeof(float))) != PLQ_OK)
+    if ((rc = alloc((void **)&n->partials, (size_t)(plq::BLOCK_TRAIN_GROUPS + plq::BLOCK_TRAIN_GROUPS / 16) *
+                                               plq::BLOCK_PART_STRIDE * sizeof(float))) != PLQ_OK)
         return bail(rc);
     // Glorot-uniform weights (fan_in + fan_out of the 3x3 kernels) from a
     // splitmix64 stream of the seed; zero biases.
@@ -306,7 +307,7 @@ int plq_blocknet_logits(plq_blocknet *n, plq_chain *c, float *out, size_t cnt) {
     a.mode = 1;
     a.net = n->view();
     a.logits = tmp;
-    a.counters = c->d_block_counters;
+    a.acc_slots = nullptr;
     a.stream = c->stream;
     plq::launch_block_pass(a);
     int rc = launch_status();
@@ -351,13 +352,16 @@ int plq_chain_block_stats(plq_chain *c, uint64_t *accepted, uint64_t *proposed, 
     if (!c) return fail(PLQ_EINVAL, "chain is null");
     std::lock_guard<std::mutex> lc(c->mu);
     PLQ_HIP(hipSetDevice(c->device));
-    unsigned long long h[2] = {0, 0};
+    unsigned long long acc = 0;
     if (c->d_block_counters) {
-        PLQ_HIP(hipMemcpyAsync(h, c->d_block_counters, sizeof h, hipMemcpyDeviceToHost, c->stream));
+        std::vector<unsigned long long> slots((size_t)plq::block_pass_groups((int)c->shape[0], (int)c->shape[1]));
+        PLQ_HIP(hipMemcpyAsync(slots.data(), c->d_block_counters, slots.size() * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, c->stream));
         PLQ_HIP(hipStreamSynchronize(c->stream));
+        for (unsigned long long v : slots) acc += v;
     }
-    if (accepted) *accepted = h[0];
-    if (proposed) *proposed = h[1];
+    if (accepted) *accepted = acc;
+    if (proposed) *proposed = c->block_sweeps * (uint64_t)(c->n_sites / 64);
     if (block_sweeps) *block_sweeps = c->block_sweeps;
     return PLQ_OK;
 }

@@ -60,7 +60,7 @@ struct plq_chain {
     std::vector<hipEvent_t> pending;
     // learned block proposal (plq_block_abi.hip)
     uint64_t block_sweeps = 0;                       // key of the block passes' draws
-    unsigned long long *d_block_counters = nullptr;  // [accepted, proposed]
+    unsigned long long *d_block_counters = nullptr;  // accepted moves, one slot per pass workgroup
     float *d_block_logits = nullptr;                 // logits of each block's last pass
 
     ~plq_chain() {

@@ -118,19 +118,20 @@ struct BlockPassArgs {
     double beta, j, h;
     BlockNetView net;
     float *logits;        // nullable: n_blocks * 64 logits of the blocks evaluated
-    unsigned long long *counters;  // [accepted, proposed]
+    unsigned long long *acc_slots;  // mode 0: accepted moves, one slot per workgroup (block_pass_groups)
     hipStream_t stream;
 };
 struct BlockGradArgs {
     const uint8_t *c0, *c1;
     int L0, L1;
     BlockNetView net;
-    float *partials;      // block_train_groups * BLOCK_PART_STRIDE
+    float *partials;      // (BLOCK_TRAIN_GROUPS + BLOCK_TRAIN_GROUPS / 16) * BLOCK_PART_STRIDE
     float *grad;          // BN_PARAMS + 1: mean gradient, then the mean loss
     hipStream_t stream;
 };
 void launch_block_pack(const float *master, float *w1r, float *w3r, uint16_t *w2f, uint16_t *w2t, hipStream_t s);
 void launch_block_pass(const BlockPassArgs &a);
+int block_pass_groups(int L0, int L1);  // workgroups of a mode-0 pass
 int block_train_groups(int L0, int L1);
 void launch_block_grad(const BlockGradArgs &a);
 void launch_block_adam(float *w, float *m, float *v, const float *grad, float lr, float beta1, float beta2,
