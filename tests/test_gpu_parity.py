@@ -308,3 +308,27 @@ def test_cluster_budget_exhaustion_is_reported(gpu):
         c.advance(1)
     assert err.value.code == gpu.PLQ_ENOCONV
     c.close()
+
+
+# -- physics: 2D Ising against Kaufman's exact finite-torus solution -------------
+
+@pytest.mark.parametrize("L,beta,kernel", [(64, 0.4406868, M), (64, 0.4406868, HB), (32, 0.3, M), (32, 0.6, HB)])
+def test_ising2d_energy_and_specific_heat_match_kaufman(gpu, L, beta, kernel):
+    import exact_ising2d as X
+    from plaquette_amd import statistics as S
+
+    n = L * L
+    start = start_config(n, 2, True, 77)
+    c = gpu.ising_gpu_chain([L, L], 1.0, 0.0, beta, 1234, start, kernel=kernel)
+    c.advance(20000)
+    steps = 300000
+    c.record(1, steps)
+    c.advance(steps)
+    e = np.array([o.energy for o in c.series()])
+    c.close()
+    est = S.reduce(e / n)
+    want = X.energy_per_site(L, L, beta)
+    assert abs(est.mean - want) < 4.5 * est.stderr, (est.mean, want, est.stderr)
+    cv = S.specific_heat(e, beta, n)
+    want_c = X.specific_heat(L, L, beta)
+    assert abs(cv.value - want_c) < 4.5 * cv.stderr, (cv.value, want_c, cv.stderr)
