@@ -16,9 +16,12 @@
 //   2. cc_boundary one lane per box-face site: draw the bonds that leave a
 //                 box and union their ends in the global forest with
 //                 compare-and-swap.
-//   3. cc_relabel one lane per site: find the root, draw the new label keyed
-//                 on it (Redraw or ForcedChange, optionally only the seed
-//                 site's cluster for Wolff), write the label.
+//   3. cc_flatten one lane per box-local root (cc_local lists them): find
+//                 the global root, draw the cluster's new label keyed on it
+//                 (Redraw or ForcedChange; for Wolff only the seed site's
+//                 cluster) and store it at the local root.
+//   4. cc_relabel one lane per site: the site's parent is a local root, whose
+//                 stored label is the site's new label.
 //
 // Every union links the larger root under the smaller one, so parent[x] <= x
 // always holds and the root of a tree is the minimum site index in it: the
@@ -79,10 +82,14 @@ __device__ __forceinline__ void lds_union(int32_t *P, int32_t a, int32_t b) {
 // M = ceil(2^40 / d) is exact while n * d < 2^40.
 __device__ __forceinline__ uint32_t cdiv(uint32_t n, uint64_t M) { return (uint32_t)(((uint64_t)n * M) >> 40); }
 
-template <int D>
+// POW2: every box has the same power-of-two extents, so a local index splits
+// into box coordinates by shifts and masks instead of reciprocal divisions.
+template <int D, bool POW2>
 __global__ __launch_bounds__(1024) void cc_local(const uint8_t *__restrict__ labels, int32_t *__restrict__ parent,
-                                                 CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t) {
+                                                 CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t,
+                                                 int32_t *__restrict__ rootlist, int *__restrict__ nroots) {
     extern __shared__ int32_t P[];
+    __shared__ uint32_t box_roots, box_base, box_next;
     uint32_t o[MAX_D], e[MAX_D], ls[MAX_D];
     uint64_t me[MAX_D];
     uint32_t rest = blockIdx.x;
@@ -101,34 +108,75 @@ __global__ __launch_bounds__(1024) void cc_local(const uint8_t *__restrict__ lab
         nloc *= e[mu];
     }
     // local index -> (box coordinates, global site)
+    uint32_t lsh[MAX_D];
+    if (POW2) {
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) lsh[mu] = (uint32_t)(31 - __clz((int)ls[mu]));
+    }
     auto locate = [&](uint32_t l, uint32_t *y) -> uint32_t {
         uint32_t r = l, g = 0;
 #pragma unroll
         for (int mu = 0; mu < D; ++mu) {
-            uint32_t q = mu + 1 < D ? cdiv(r, me[mu]) : 0u;
-            y[mu] = mu + 1 < D ? r - q * e[mu] : r;
-            r = q;
+            if (POW2) {
+                y[mu] = (l >> lsh[mu]) & (e[mu] - 1u);
+            } else {
+                uint32_t q = mu + 1 < D ? cdiv(r, me[mu]) : 0u;
+                y[mu] = mu + 1 < D ? r - q * e[mu] : r;
+                r = q;
+            }
             g += (o[mu] + y[mu]) * (uint32_t)cg.stride[mu];
         }
         return g;
     };
-    for (uint32_t l = threadIdx.x; l < nloc; l += blockDim.x) P[l] = (int32_t)l;
+    if (threadIdx.x == 0) {
+        box_roots = 0;
+        box_next = 0;
+    }
+    // Axis 0 by a wave scan: lanes hold consecutive local indices, so a run of
+    // open axis-0 bonds is a run of lanes; every site points at the first site
+    // of its run (parent <= child, as every union keeps), with no atomics.
+    // Bonds across a row end (x = e0 - 1) are cut here; the wrap bond of a box
+    // that spans axis 0 and the link between two waves of one run are joined
+    // below with the other axes' bonds.
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);
+    for (uint32_t base = threadIdx.x - lane; base < nloc; base += blockDim.x) {
+        const uint32_t l = base + lane;
+        bool open = false;
+        if (l < nloc) {
+            uint32_t y[MAX_D];
+            const uint32_t gsite = locate(l, y);
+            if (y[0] + 1u < e[0] && l + 1u < nloc) {
+                const uint8_t lab = labels[gsite];
+                open = labels[gsite + 1u] == lab &&
+                       keyed_bits_k(bond_seed, gsite * (uint32_t)D, skey) < bond_t;
+            }
+        }
+        const uint64_t m = __ballot(open);
+        const uint64_t starts = (~m << 1) | 1ull;  // bit j: lane j begins a run
+        const uint32_t first = 63u - (uint32_t)__clzll(starts & below);
+        if (l < nloc) P[l] = (int32_t)(base + first);
+    }
     __syncthreads();
     for (uint32_t l = threadIdx.x; l < nloc; l += blockDim.x) {
         uint32_t y[MAX_D];
         const uint32_t gsite = locate(l, y);
         const uint8_t lab = labels[gsite];
+        // the axis-0 link from the last lane of a wave to the next wave
+        if ((l & 63u) == 63u && y[0] + 1u < e[0] && l + 1u < nloc && labels[gsite + 1u] == lab &&
+            keyed_bits_k(bond_seed, gsite * (uint32_t)D, skey) < bond_t)
+            lds_union(P, (int32_t)l, (int32_t)(l + 1u));
 #pragma unroll
         for (int mu = 0; mu < D; ++mu) {
             uint32_t lp, gp;
-            if (y[mu] + 1u < e[mu]) {
+            if (mu > 0 && y[mu] + 1u < e[mu]) {
                 lp = l + ls[mu];
                 gp = gsite + (uint32_t)cg.stride[mu];
-            } else if (e[mu] == (uint32_t)cg.L[mu]) {  // the box spans the axis: the wrap bond is internal
+            } else if (y[mu] + 1u == e[mu] && e[mu] == (uint32_t)cg.L[mu]) {  // the box spans the axis: the wrap bond is internal
                 lp = l - (e[mu] - 1u) * ls[mu];
                 gp = gsite - (uint32_t)(cg.L[mu] - 1) * (uint32_t)cg.stride[mu];
             } else {
-                continue;  // leaves the box: cc_boundary draws it
+                continue;  // axis 0 inside the box (the scan), or a bond leaving the box (cc_boundary)
             }
             if (labels[gp] != lab) continue;
             const uint32_t bond = gsite * (uint32_t)D + (uint32_t)mu;
@@ -142,6 +190,25 @@ __global__ __launch_bounds__(1024) void cc_local(const uint8_t *__restrict__ lab
         const uint32_t gsite = locate(l, y);
         const uint32_t groot = root == l ? gsite : locate(root, yr);
         parent[gsite] = (int32_t)groot;
+        const uint64_t rm = __ballot(root == l);  // one LDS atomic per wave
+        if (lane == (uint32_t)(__ffsll((unsigned long long)__ballot(1)) - 1)) atomicAdd(&box_roots, (uint32_t)__popcll(rm));
+    }
+    __syncthreads();
+    // Append the box's local roots to the sweep's root list.
+    if (threadIdx.x == 0) box_base = (uint32_t)atomicAdd(nroots, (int)box_roots);
+    __syncthreads();
+    const uint64_t lower = (1ull << lane) - 1ull;
+    for (uint32_t l = threadIdx.x; l < nloc; l += blockDim.x) {
+        const bool is_root = lds_load(&P[l]) == (int32_t)l;  // roots keep P[r] = r; path halving moved only non-roots
+        const uint64_t rm = __ballot(is_root);
+        const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)__ballot(1)) - 1);
+        uint32_t wbase = 0;
+        if (lane == leader) wbase = atomicAdd(&box_next, (uint32_t)__popcll(rm));
+        wbase = __shfl(wbase, (int)leader, 64);
+        if (is_root) {
+            uint32_t y[MAX_D];
+            rootlist[box_base + wbase + (uint32_t)__popcll(rm & lower)] = (int32_t)locate(l, y);
+        }
     }
 }
 
@@ -245,12 +312,19 @@ __device__ __forceinline__ long long blk_sum(long long v, long long *lds) {
     return t;
 }
 
-__global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, const int32_t *__restrict__ parent,
-                                                  int64_t N, uint32_t root_seed, uint32_t seed_seed, uint32_t skey,
-                                                  uint32_t sweep, int q, int seeded, int forced, uint32_t cap,
-                                                  int *__restrict__ flag, long long *__restrict__ partials,
-                                                  int64_t span) {
-    __shared__ long long lds[16];
+// One lane per local root of this sweep (rootlist[0 .. *nroots)): the new
+// label of its cluster, keyed on the global root (cluster_relabel.wgsl:39-90),
+// stored at the local root; NO_LABEL where the cluster keeps its label
+// (Wolff: every cluster but the seed site's).  The hop budget counts the hop
+// from a member site to its local root as well.
+constexpr uint8_t NO_LABEL = 0xff;
+
+__global__ __launch_bounds__(256) void cc_flatten(const uint8_t *__restrict__ labels,
+                                                  const int32_t *__restrict__ parent,
+                                                  const int32_t *__restrict__ rootlist, const int *__restrict__ nroots,
+                                                  uint8_t *__restrict__ newlab, int64_t N, uint32_t root_seed,
+                                                  uint32_t seed_seed, uint32_t skey, uint32_t sweep, int q,
+                                                  int seeded, int forced, uint32_t cap, int *__restrict__ flag) {
     __shared__ int32_t seed_root;
     if (seeded) {
         if (threadIdx.x == 0) {
@@ -266,42 +340,80 @@ __global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, 
         }
         __syncthreads();
     }
-    long long moved = 0;
-    int counts[MAX_Q_DEV];
-    for (int a = 0; a < MAX_Q_DEV; ++a) counts[a] = 0;
-    int64_t begin = (int64_t)blockIdx.x * span;
-    int64_t end = begin + span;
-    if (end > N) end = N;
-    for (int64_t s = begin + threadIdx.x; s < end; s += blockDim.x) {
-        uint32_t hops = 0;
-        int32_t root = find_ro(parent, (int32_t)s, cap, hops);
+    const int n = *nroots;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int32_t r = rootlist[i];
+        uint32_t hops = 1;
+        const int32_t root = find_ro(parent, r, cap, hops);
         if (hops > cap) atomicOr(flag, 1);
-        uint32_t lab = labels[s];
+        uint8_t nl = NO_LABEL;
         if (!seeded || root == seed_root) {
             float u = keyed_uniform_f32(keyed_bits_k(root_seed, (uint32_t)root, skey));
             if (forced) {
                 uint32_t step = (uint32_t)(u * (float)(uint32_t)(q - 1));
                 if (step >= (uint32_t)(q - 1)) step = (uint32_t)(q - 2);
-                lab = (lab + 1u + step) % (uint32_t)q;
+                nl = (uint8_t)((labels[r] + 1u + step) % (uint32_t)q);
             } else {
                 uint32_t draw = (uint32_t)(u * (float)(uint32_t)q);
                 if (draw >= (uint32_t)q) draw = (uint32_t)(q - 1);
-                lab = draw;
+                nl = (uint8_t)draw;
             }
-            labels[s] = (uint8_t)lab;
-            moved += 1;
         }
-        counts[lab] += 1;
+        newlab[r] = nl;
     }
-    long long *dst = partials + (int64_t)blockIdx.x * REC_K;
+}
+
+// Number of zero bytes of x (exact: no borrow between bytes).
+__device__ __forceinline__ int zero_bytes(uint32_t x) {
+    uint32_t t = (x & 0x7f7f7f7fu) + 0x7f7f7f7fu;
+    return __popc(~(t | x | 0x7f7f7f7fu));
+}
+
+// The site's parent is a local root of this sweep (cc_local stores local
+// roots, cc_boundary links local roots and halves paths through them), whose
+// stored label is the site's new label.  Four sites per lane (one dword of
+// labels, one int4 of parents); the last block also takes the N mod 4 tail
+// and resets the root count for the next sweep.  Partials per block:
+// [0, moved, 0 ...].
+__global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, const int32_t *__restrict__ parent,
+                                                  const uint8_t *__restrict__ newlab, int64_t N,
+                                                  long long *__restrict__ partials, int64_t span,
+                                                  int *__restrict__ nroots) {
+    __shared__ long long lds[16];
+    long long moved = 0;
+    const int64_t nd = N >> 2;
+    int64_t begin = (int64_t)blockIdx.x * span;
+    int64_t end = begin + span;
+    if (end > nd) end = nd;
+    uint32_t *w32 = reinterpret_cast<uint32_t *>(labels);
+    const int4 *p4 = reinterpret_cast<const int4 *>(parent);
+    for (int64_t v = begin + threadIdx.x; v < end; v += blockDim.x) {
+        const int4 p = p4[v];
+        const uint32_t n0 = newlab[p.x], n1 = newlab[p.y], n2 = newlab[p.z], n3 = newlab[p.w];
+        const uint32_t nw = n0 | (n1 << 8) | (n2 << 16) | (n3 << 24);
+        const uint32_t keep = (n0 == NO_LABEL ? 0xffu : 0u) | (n1 == NO_LABEL ? 0xff00u : 0u) |
+                              (n2 == NO_LABEL ? 0xff0000u : 0u) | (n3 == NO_LABEL ? 0xff000000u : 0u);
+        if (keep == 0xffffffffu) continue;
+        const uint32_t old = keep ? w32[v] : 0u;
+        w32[v] = (old & keep) | (nw & ~keep);
+        moved += 4 - (__popc(keep) >> 3);
+    }
+    if (blockIdx.x == gridDim.x - 1) {
+        for (int64_t s = (nd << 2) + threadIdx.x; s < N; s += blockDim.x) {
+            const uint8_t nl = newlab[parent[s]];
+            if (nl != NO_LABEL) {
+                labels[s] = nl;
+                moved += 1;
+            }
+        }
+    }
     long long t = blk_sum(moved, lds);
     if (threadIdx.x == 0) {
+        long long *dst = partials + (int64_t)blockIdx.x * REC_K;
         dst[0] = 0;
         dst[1] = t;
-    }
-    for (int a = 0; a < MAX_Q_DEV; ++a) {
-        t = blk_sum((long long)(a < q ? counts[a] : 0), lds);
-        if (threadIdx.x == 0) dst[2 + a] = t;
+        for (int k = 2; k < REC_K; ++k) dst[k] = 0;
+        if (blockIdx.x == gridDim.x - 1) *nroots = 0;
     }
 }
 
@@ -346,6 +458,61 @@ __global__ __launch_bounds__(256) void potts_measure_kernel(const uint8_t *__res
     }
 }
 
+// The same sums four sites per lane, for L0 a multiple of 4: a dword of
+// labels never straddles a row; agreement along axis 0 compares each byte
+// with the next (the row's first byte after its last), along the other axes
+// with the dword one stride on.  Column-major partials: column k of block b
+// at part[k * gridDim.x + b], columns [agree, 0, counts[0..q)].
+template <int D>
+__global__ __launch_bounds__(256) void potts_measure_vec(const uint8_t *__restrict__ labels, CCGeom cg, int q,
+                                                         long long *__restrict__ part, int64_t span) {
+    __shared__ long long lds[16];
+    const uint32_t *w32 = reinterpret_cast<const uint32_t *>(labels);
+    const int64_t nd = cg.N >> 2;
+    long long agree = 0;
+    int counts[MAX_Q_DEV];
+#pragma unroll
+    for (int a = 0; a < MAX_Q_DEV; ++a) counts[a] = 0;
+    int64_t begin = (int64_t)blockIdx.x * span;
+    int64_t end = begin + span;
+    if (end > nd) end = nd;
+    const uint32_t L0 = (uint32_t)cg.L[0];
+    for (int64_t v = begin + threadIdx.x; v < end; v += blockDim.x) {
+        const uint32_t s0 = (uint32_t)v << 2;
+        const uint32_t w = w32[v];
+        uint32_t rest = s0;
+        uint32_t x[MAX_D];
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) {
+            const uint32_t qt = mu + 1 < D ? (cg.mul_div ? cdiv(rest, cg.mL[mu]) : rest / (uint32_t)cg.L[mu]) : 0u;
+            x[mu] = mu + 1 < D ? rest - qt * (uint32_t)cg.L[mu] : rest;
+            rest = qt;
+        }
+        const uint32_t nb = x[0] + 4u == L0 ? labels[s0 - x[0]] : labels[s0 + 4u];
+        int a0 = zero_bytes(w ^ ((w >> 8) | (nb << 24)));
+#pragma unroll
+        for (int mu = 1; mu < D; ++mu) {
+            const uint32_t st = (uint32_t)cg.stride[mu];
+            const uint32_t gp = (x[mu] + 1u == (uint32_t)cg.L[mu]) ? s0 - x[mu] * st : s0 + st;
+            a0 += zero_bytes(w ^ w32[gp >> 2]);
+        }
+        agree += a0;
+#pragma unroll
+        for (int a = 0; a < MAX_Q_DEV; ++a)
+            if (a < q) counts[a] += zero_bytes(w ^ (0x01010101u * (uint32_t)a));
+    }
+    const int64_t G = gridDim.x;
+    long long t = blk_sum(agree, lds);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = t;
+        part[G + blockIdx.x] = 0;
+    }
+    for (int a = 0; a < q; ++a) {
+        t = blk_sum((long long)counts[a], lds);
+        if (threadIdx.x == 0) part[(2 + a) * G + blockIdx.x] = t;
+    }
+}
+
 // ---------------------------------------------------- roots and sizes ---
 
 __global__ void cc_resolve(int32_t *__restrict__ parent, int32_t *__restrict__ roots, int64_t N, uint32_t cap,
@@ -358,10 +525,28 @@ __global__ void cc_resolve(int32_t *__restrict__ parent, int32_t *__restrict__ r
     roots[s] = r;
 }
 
+// Cluster sizes: each lane walks CC_RUN consecutive sites and adds one count
+// per run of equal roots, so a large cluster takes one atomic per run rather
+// than one per site.
+constexpr int CC_RUN = 64;
+
 __global__ void cc_count(const int32_t *__restrict__ roots, int32_t *__restrict__ sizes, int64_t N) {
-    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= N) return;
-    atomicAdd(&sizes[roots[s]], 1);
+    const int64_t s0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * CC_RUN;
+    if (s0 >= N) return;
+    const int64_t s1 = s0 + CC_RUN < N ? s0 + CC_RUN : N;
+    int32_t cur = roots[s0];
+    int run = 1;
+    for (int64_t s = s0 + 1; s < s1; ++s) {
+        const int32_t r = roots[s];
+        if (r == cur) {
+            ++run;
+        } else {
+            atomicAdd(&sizes[cur], run);
+            cur = r;
+            run = 1;
+        }
+    }
+    atomicAdd(&sizes[cur], run);
 }
 
 // hist[0..39] clusters by floor(log2 size); hist[40] clusters; hist[41] largest
@@ -443,9 +628,17 @@ int64_t launch_cluster_sweep(const ClusterSweepArgs &a) {
     size_t lds = (size_t)cg.box_sites * sizeof(int32_t);
     dim3 gl((uint32_t)cg.nboxes), bl(1024);
     int64_t faces = face_sites(cg);
+    bool pow2 = true;
+    for (int mu = 0; mu < cg.D; ++mu)
+        pow2 = pow2 && (cg.L[mu] & (cg.L[mu] - 1)) == 0 && (cg.B[mu] & (cg.B[mu] - 1)) == 0;
     dim3 gb((uint32_t)((faces + 255) / 256)), bb(256);
 #define PLQ_CC(DD)                                                                                          \
-    cc_local<DD><<<gl, bl, lds, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t);            \
+    if (pow2)                                                                                               \
+        cc_local<DD, true><<<gl, bl, lds, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t,    \
+                                                      a.rootlist, a.nroots);                                \
+    else                                                                                                    \
+        cc_local<DD, false><<<gl, bl, lds, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t,   \
+                                                       a.rootlist, a.nroots);                               \
     if (faces > 0)                                                                                          \
         cc_boundary<DD><<<gb, bb, 0, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t, a.cap, \
                                                  a.flag)
@@ -459,14 +652,33 @@ int64_t launch_cluster_sweep(const ClusterSweepArgs &a) {
     int64_t need = (cg.N + 255) / 256;
     int64_t G = need < a.max_blocks ? need : a.max_blocks;
     int64_t span = (cg.N + G - 1) / G;
-    cc_relabel<<<dim3((uint32_t)G), dim3(256), 0, a.stream>>>(
-        a.labels, a.parent, cg.N, a.seed32 ^ ROOT_SALT, a.seed32 ^ SEED_SALT, skey, a.sweep, a.q, a.seeded,
-        a.forced, a.cap, a.flag, a.partials, span);
+    cc_flatten<<<dim3((uint32_t)G), dim3(256), 0, a.stream>>>(
+        a.labels, a.parent, a.rootlist, a.nroots, a.newlab, cg.N, a.seed32 ^ ROOT_SALT, a.seed32 ^ SEED_SALT, skey,
+        a.sweep, a.q, a.seeded, a.forced, a.cap, a.flag);
+    const int64_t nd = cg.N >> 2;
+    const int64_t span4 = nd > 0 ? (nd + G - 1) / G : 1;
+    cc_relabel<<<dim3((uint32_t)G), dim3(256), 0, a.stream>>>(a.labels, a.parent, a.newlab, cg.N, a.partials, span4,
+                                                              a.nroots);
     return G;
 }
 
 void launch_potts_measure(const uint8_t *labels, const CCGeom &cg, int q, long long *partials, int64_t max_blocks,
                           long long *rec, hipStream_t s) {
+    if (cg.L[0] % 4 == 0) {
+        const int64_t nd = cg.N >> 2;
+        int64_t need = (nd + 255) / 256;
+        int64_t G = need < max_blocks ? need : max_blocks;
+        int64_t span = (nd + G - 1) / G;
+        dim3 g((uint32_t)G), b(256);
+        switch (cg.D) {
+            case 1: potts_measure_vec<1><<<g, b, 0, s>>>(labels, cg, q, partials, span); break;
+            case 2: potts_measure_vec<2><<<g, b, 0, s>>>(labels, cg, q, partials, span); break;
+            case 3: potts_measure_vec<3><<<g, b, 0, s>>>(labels, cg, q, partials, span); break;
+            default: potts_measure_vec<4><<<g, b, 0, s>>>(labels, cg, q, partials, span); break;
+        }
+        launch_finalize_columns(partials, G, nullptr, 0, 2 + q, rec, s);
+        return;
+    }
     int64_t need = (cg.N + 255) / 256;
     int64_t G = need < max_blocks ? need : max_blocks;
     int64_t span = (cg.N + G - 1) / G;
@@ -485,7 +697,7 @@ void launch_cluster_resolve(int32_t *parent, int32_t *roots, const CCGeom &cg, u
     dim3 g((uint32_t)((cg.N + 255) / 256)), b(256);
     cc_resolve<<<g, b, 0, s>>>(parent, roots, cg.N, cap, flag);
     if (sizes && hist) {
-        cc_count<<<g, b, 0, s>>>(roots, sizes, cg.N);
+        cc_count<<<dim3((uint32_t)((cg.N + 256LL * CC_RUN - 1) / (256LL * CC_RUN))), b, 0, s>>>(roots, sizes, cg.N);
         cc_hist<<<g, b, 0, s>>>(roots, sizes, cg.N, hist);
     }
 }

@@ -103,20 +103,22 @@ __device__ __forceinline__ void emit_partials(const MeasArgs &m, long long bond,
 }
 
 // Fold the column-major partials of the two colour passes (na and nb blocks)
-// into a record: nk columns, each a contiguous run, read coalesced.
-__global__ __launch_bounds__(1024) void finalize_columns_kernel(const long long *__restrict__ pa, int64_t na,
-                                                                const long long *__restrict__ pb, int64_t nb, int nk,
-                                                                long long *__restrict__ rec) {
+// into a record: one workgroup per record column, each column a contiguous
+// run read coalesced; columns >= nk are cleared.
+__global__ __launch_bounds__(256) void finalize_columns_kernel(const long long *__restrict__ pa, int64_t na,
+                                                               const long long *__restrict__ pb, int64_t nb, int nk,
+                                                               long long *__restrict__ rec) {
     __shared__ long long lds[16];
-    if (threadIdx.x == 0)
-        for (int k = nk; k < REC_K; ++k) rec[k] = 0;
-    for (int k = 0; k < nk; ++k) {
-        long long v = 0;
-        for (int64_t b = threadIdx.x; b < na; b += blockDim.x) v += pa[k * na + b];
-        for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) v += pb[k * nb + b];
-        long long t = block_sum(v, lds);
-        if (threadIdx.x == 0) rec[k] = t;
+    const int k = blockIdx.x;
+    if (k >= nk) {
+        if (threadIdx.x == 0) rec[k] = 0;
+        return;
     }
+    long long v = 0;
+    for (int64_t b = threadIdx.x; b < na; b += blockDim.x) v += pa[k * na + b];
+    for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) v += pb[k * nb + b];
+    long long t = block_sum(v, lds);
+    if (threadIdx.x == 0) rec[k] = t;
 }
 
 // Sum two runs of block partials (one per colour pass) into one record: each
@@ -588,7 +590,7 @@ void launch_finalize_record(const long long *pa, int64_t na, const long long *pb
 
 void launch_finalize_columns(const long long *pa, int64_t na, const long long *pb, int64_t nb, int nk, long long *rec,
                              hipStream_t s) {
-    finalize_columns_kernel<<<dim3(1), dim3(256), 0, s>>>(pa, na, pb, nb, nk, rec);
+    finalize_columns_kernel<<<dim3(REC_K), dim3(256), 0, s>>>(pa, na, pb, nb, nk, rec);
 }
 
 static int64_t pass_work(const SitePassArgs &a) { return (int64_t)a.g.R * a.g.H / a.vec; }

@@ -266,6 +266,8 @@ CLUSTER_CASES = [
     ([32, 32], 3, 1.0, 1.2, True, True, 0, 30),
     ([32, 32], 2, 2.0, 0.4406868, True, True, 1, 30),
     ([1000], 3, 1.0, 2.0, False, False, 0, 10),
+    ([256, 256], 3, 1.0, 1.0050525, False, False, 0, 4),   # power-of-two boxes, 4 of 128x128
+    ([32, 32, 32], 2, 2.0, 0.2216546, True, True, 0, 6),   # power-of-two boxes, 2 of 32x32x16
 ]
 
 
