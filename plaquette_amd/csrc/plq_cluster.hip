@@ -54,13 +54,16 @@ __device__ __forceinline__ void lds_store(int32_t *p, int32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Root of x with path splitting: every node on the walk is re-pointed at its
+// grandparent (an ancestor, so parent <= child still holds under concurrent
+// unions); the grandparent just read is the next node's parent.
 __device__ __forceinline__ int32_t lds_find(int32_t *P, int32_t x) {
     int32_t p = lds_load(&P[x]);
     while (p != x) {
-        int32_t gp = lds_load(&P[p]);
-        if (gp != p) lds_store(&P[x], gp);  // path halving; gp is an ancestor of x
+        const int32_t gp = lds_load(&P[p]);
+        if (gp != p) lds_store(&P[x], gp);
         x = p;
-        p = lds_load(&P[x]);
+        p = gp;
     }
     return x;
 }

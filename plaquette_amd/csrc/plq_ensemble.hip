@@ -98,6 +98,17 @@ __device__ __forceinline__ int link_negs(const uint32_t *__restrict__ words, con
     return neg;
 }
 
+// Logical block of this workgroup.  Workgroups are dispatched round-robin
+// over the 8 XCDs; with the grid a multiple of 8 (padded blocks carry
+// NO_REP), XCD x runs logical blocks [x G/8, (x+1) G/8), i.e. a contiguous
+// run of replicas, so each XCD's L2 holds only its replicas' links.
+constexpr uint16_t NO_REP = 0xffff;
+__device__ __forceinline__ uint32_t ens_logical_block() {
+    const uint32_t b = blockIdx.x, G = gridDim.x;
+    if ((G & 7u) != 0) return b;
+    return (b & 7u) * (G >> 3) + (b >> 3);
+}
+
 // One (MU, c) pass of every replica.  sweep = *sweep_base + sweep_off.
 template <int D, int MODE, int MU>
 __global__ __launch_bounds__(256) void ens_link_pass(uint32_t *__restrict__ words, const EnsReplica *__restrict__ reps,
@@ -108,13 +119,15 @@ __global__ __launch_bounds__(256) void ens_link_pass(uint32_t *__restrict__ word
                                                      int c, int per_lane) {
     constexpr int K = 2 * (D - 1);
     __shared__ uint32_t T[2 * (K + 1)];
-    const int ri = block_rep[blockIdx.x];
+    const uint32_t lb = ens_logical_block();
+    const int ri = block_rep[lb];
+    if (ri == NO_REP) return;
     const EnsReplica R = reps[ri];
     if (threadIdx.x < 2 * (K + 1)) T[threadIdx.x] = tables[beta_slot[ri] * (2 * (K + 1)) + threadIdx.x];
     __syncthreads();
     const uint32_t skey = (*sweep_base + sweep_off) * GOLDEN;
     const uint32_t half = R.nsites >> 1;
-    const uint32_t base = (blockIdx.x - R.block0) * blockDim.x * (uint32_t)per_lane + threadIdx.x;
+    const uint32_t base = (lb - R.block0) * blockDim.x * (uint32_t)per_lane + threadIdx.x;
     for (int k = 0; k < per_lane; ++k) {
         const uint32_t item = base + (uint32_t)k * blockDim.x;
         if (item >= half) break;
@@ -175,10 +188,12 @@ __global__ __launch_bounds__(256) void ens_plaquette(const uint32_t *__restrict_
                                                      const uint16_t *__restrict__ block_rep,
                                                      unsigned long long *__restrict__ sums, int per_lane) {
     __shared__ long long lds[8];
-    const int ri = block_rep[blockIdx.x];
+    const uint32_t lb = ens_logical_block();
+    const int ri = block_rep[lb];
+    if (ri == NO_REP) return;
     const EnsReplica R = reps[ri];
     long long acc = 0;
-    const uint32_t base = (blockIdx.x - R.block0) * blockDim.x * (uint32_t)per_lane + threadIdx.x;
+    const uint32_t base = (lb - R.block0) * blockDim.x * (uint32_t)per_lane + threadIdx.x;
     for (int k = 0; k < 2 * per_lane; ++k) {
         // every site once: items [0, nsites) in pairs of half-lattices
         const uint32_t item = base + (uint32_t)(k >> 1) * blockDim.x;

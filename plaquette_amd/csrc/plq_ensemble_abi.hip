@@ -111,10 +111,14 @@ int layout_blocks(plq_ensemble *e) {
         uint32_t half = R.nsites / 2;
         b += (half + per_block - 1) / per_block;
     }
+    // Pad the grid to a multiple of 8 so the kernels can give each XCD a
+    // contiguous run of replicas; padded blocks map to no replica (0xffff).
+    const uint32_t used = b;
+    b = (b + 7u) & ~7u;
     e->total_blocks = b;
-    std::vector<uint16_t> block_rep(b);
+    std::vector<uint16_t> block_rep(b, (uint16_t)0xffff);
     for (int r = 0; r < e->nrep; ++r) {
-        uint32_t end = r + 1 < e->nrep ? e->reps[r + 1].block0 : b;
+        uint32_t end = r + 1 < e->nrep ? e->reps[r + 1].block0 : used;
         for (uint32_t k = e->reps[r].block0; k < end; ++k) block_rep[k] = (uint16_t)r;
     }
     if (e->block_rep_cap < b) {
@@ -259,6 +263,7 @@ int plq_ensemble_create(const plq_ensemble_desc *d, const uint8_t *start, size_t
     if (d->dim < 2 || d->dim > 4) return plq::host_fail(PLQ_EINVAL, "ensembles support dimensions 2..4");
     if (d->n_replicas < 1 || !d->shapes || !d->betas || !d->seeds)
         return plq::host_fail(PLQ_EINVAL, "an ensemble needs at least one replica with shape, beta and seed");
+    if (d->n_replicas > 65535) return plq::host_fail(PLQ_EINVAL, "an ensemble holds at most 65535 replicas");
     if (d->kernel < 0 || d->kernel > 1 || d->profile < 0 || d->profile > 1)
         return plq::host_fail(PLQ_EINVAL, "unknown kernel or threshold profile");
     if (!std::isfinite(d->j)) return plq::host_fail(PLQ_EINVAL, "j must be finite");
