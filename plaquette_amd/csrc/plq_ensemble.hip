@@ -7,17 +7,21 @@
 // replica of an ensemble at once; a replica-exchange step (no reference
 // counterpart) swaps inverse temperatures between replicas of equal shape.
 //
-// Layout: the D links of a site are one word (byte mu = U_mu(x), D <= 4),
-// and sites are colour-interleaved as in plq_common.hpp (Geom): colour c of
-// replica r is the word array words + rep.off[c], row-major with H = L0 / 2
-// words per row.  A pass (mu, c) rewrites byte mu of the colour-c words only;
-// every other byte it reads (the other directions, and direction mu of the
-// other colour) is not written during the pass, which is the link-colouring
-// argument of link_schedule.wgsl.  The new link is written as a single byte,
-// so no lane ever rewrites a byte another lane reads.
+// Layout: replicas of one shape are packed four to a word (EnsQuad): byte j
+// of a word is replica rep[j]'s link, and a quad has one word array per
+// (direction mu, colour c), each colour-interleaved as in plq_common.hpp
+// (Geom: site (x0, row r) of colour c at r * H + x0 / 2, H = L0 / 2).  A lane
+// updates the link (x, mu) of all four replicas at once: the coordinates,
+// the 18 neighbour words and the staple products (XORs of 0/1 bytes, summed
+// per byte) are shared; only the draw, keyed by each replica's own seed, and
+// the threshold of each replica's current beta are per byte.  A pass (mu, c)
+// rewrites words of array (mu, c) only and reads arrays it does not write,
+// which is the link-colouring argument of link_schedule.wgsl; each word is
+// owned by one lane.  Every neighbour's colour is fixed by its distance: one
+// step away is the other colour, two steps the same (extents are even).
 //
-// Each block belongs to one replica (blocks never straddle replicas), so the
-// replica's geometry and its threshold table are uniform per block.
+// Each block belongs to one quad (blocks never straddle quads), so the
+// geometry and the four threshold tables are uniform per block.
 #include <hip/hip_runtime.h>
 
 #include "plq_common.hpp"
@@ -39,14 +43,14 @@ struct LinkSite {
 // axis is a template parameter so every coordinate lives in a register (a
 // runtime index into the coordinate array would put it in scratch memory).
 template <int A, int SIGN>
-__device__ __forceinline__ LinkSite lstep(const EnsReplica &R, const LinkSite &s) {
+__device__ __forceinline__ LinkSite lstep(const EnsQuad &Q, const LinkSite &s) {
     LinkSite t = s;
-    const uint32_t l = (uint32_t)R.L[A];
+    const uint32_t l = (uint32_t)Q.L[A];
     if (A == 0) {
         t.x0 = SIGN > 0 ? (s.x0 + 1u == l ? 0u : s.x0 + 1u) : (s.x0 == 0u ? l - 1u : s.x0 - 1u);
     } else {
         const uint32_t xa = s.x[A];
-        const uint32_t st = (uint32_t)R.rstride[A];
+        const uint32_t st = (uint32_t)Q.rstride[A];
         if (SIGN > 0) {
             t.x[A] = xa + 1u == l ? 0u : xa + 1u;
             t.r = xa + 1u == l ? s.r - (l - 1u) * st : s.r + st;
@@ -58,51 +62,62 @@ __device__ __forceinline__ LinkSite lstep(const EnsReplica &R, const LinkSite &s
     return t;
 }
 
-// The word of a site: colour (x0 + row parity) & 1, index r * H + x0 / 2.
-__device__ __forceinline__ uint32_t lword(const uint32_t *__restrict__ words, const EnsReplica &R, const LinkSite &s,
-                                          uint32_t rowpar) {
-    const uint32_t off = ((s.x0 + rowpar) & 1u) ? R.off[1] : R.off[0];
-    return ld32(words, off + s.r * (uint32_t)R.H + (s.x0 >> 1));
+// Index of a site within its colour array.
+__device__ __forceinline__ uint32_t cidx(const EnsQuad &Q, const LinkSite &s) { return s.r * (uint32_t)Q.H + (s.x0 >> 1); }
+
+// Word array of (direction mu, colour c).
+__device__ __forceinline__ uint32_t arr(const EnsQuad &Q, uint32_t half, int mu, uint32_t c) {
+    return Q.woff + ((uint32_t)mu * 2u + c) * half;
 }
 
-__device__ __forceinline__ uint32_t row_parity(const LinkSite &s) { return (s.x[1] + s.x[2] + s.x[3]) & 1u; }
+// Site of colour c at index item of its colour array.
+template <int D>
+__device__ __forceinline__ LinkSite site_of(const EnsQuad &Q, uint32_t item, uint32_t c) {
+    LinkSite s;
+    s.r = fdiv(item, Q.mH);
+    const uint32_t i = item - s.r * (uint32_t)Q.H;
+    uint32_t rest = s.r;
+    s.x[0] = s.x[1] = s.x[2] = s.x[3] = 0;
+    if (D >= 2) {
+        uint32_t q = fdiv(rest, Q.mL[1]);
+        s.x[1] = rest - q * (uint32_t)Q.L[1];
+        rest = q;
+    }
+    if (D >= 3) {
+        uint32_t q = fdiv(rest, Q.mL[2]);
+        s.x[2] = rest - q * (uint32_t)Q.L[2];
+        rest = q;
+    }
+    if (D >= 4) s.x[3] = rest;
+    s.x0 = 2u * i + ((s.x[1] + s.x[2] + s.x[3] + c) & 1u);
+    return s;
+}
 
-__device__ __forceinline__ uint32_t byte_of(uint32_t w, int d) { return (w >> (8 * d)) & 1u; }
-
-// The two staples of link (x, MU) in the (MU, NU) plane, as the number of
-// negative products among them:
+// Negative staple products of link (x, MU) in the (MU, NU) plane, one count
+// per byte (replica):
 //   forward  U_NU(x+MU) U_MU(x+NU) U_NU(x)
 //   backward U_MU(x-NU) U_NU(x-NU+MU) U_NU(x-NU)
+// x has colour c; x+MU, x+NU, x-NU have colour 1-c; x-NU+MU has colour c.
 template <int MU, int NU>
-__device__ __forceinline__ int staple_negs(const uint32_t *__restrict__ words, const EnsReplica &R, const LinkSite &s,
-                                           uint32_t wx, uint32_t wxm) {
-    const LinkSite xpn = lstep<NU, +1>(R, s);
-    const LinkSite xmn = lstep<NU, -1>(R, s);
-    const LinkSite xmnp = lstep<MU, +1>(R, xmn);
-    const uint32_t wpn = lword(words, R, xpn, row_parity(xpn));
-    const uint32_t wmn = lword(words, R, xmn, row_parity(xmn));
-    const uint32_t wmnp = lword(words, R, xmnp, row_parity(xmnp));
-    int neg = (int)(byte_of(wxm, NU) ^ byte_of(wpn, MU) ^ byte_of(wx, NU));
-    neg += (int)(byte_of(wmn, MU) ^ byte_of(wmnp, NU) ^ byte_of(wmn, NU));
-    return neg;
-}
-
-template <int D, int MU>
-__device__ __forceinline__ int link_negs(const uint32_t *__restrict__ words, const EnsReplica &R, const LinkSite &s,
-                                         uint32_t wx, uint32_t wxm) {
-    int neg = 0;
-    if (MU != 0) neg += staple_negs<MU, 0>(words, R, s, wx, wxm);
-    if (D > 1 && MU != 1) neg += staple_negs<MU, 1>(words, R, s, wx, wxm);
-    if (D > 2 && MU != 2) neg += staple_negs<MU, (D > 2 ? 2 : 0)>(words, R, s, wx, wxm);
-    if (D > 3 && MU != 3) neg += staple_negs<MU, (D > 3 ? 3 : 0)>(words, R, s, wx, wxm);
-    return neg;
+__device__ __forceinline__ uint32_t staple_negs4(const uint32_t *__restrict__ words, const EnsQuad &Q, uint32_t half,
+                                                 const LinkSite &s, uint32_t c, uint32_t item, uint32_t ixp) {
+    const uint32_t o = 1u - c;
+    const LinkSite xpn = lstep<NU, +1>(Q, s);
+    const LinkSite xmn = lstep<NU, -1>(Q, s);
+    const LinkSite xmnp = lstep<MU, +1>(Q, xmn);
+    const uint32_t imn = cidx(Q, xmn);
+    const uint32_t fw = ld32(words, arr(Q, half, NU, o) + ixp) ^ ld32(words, arr(Q, half, MU, o) + cidx(Q, xpn)) ^
+                        ld32(words, arr(Q, half, NU, c) + item);
+    const uint32_t bw = ld32(words, arr(Q, half, MU, o) + imn) ^ ld32(words, arr(Q, half, NU, c) + cidx(Q, xmnp)) ^
+                        ld32(words, arr(Q, half, NU, o) + imn);
+    return fw + bw;
 }
 
 // Logical block of this workgroup.  Workgroups are dispatched round-robin
 // over the 8 XCDs; with the grid a multiple of 8 (padded blocks carry
-// NO_REP), XCD x runs logical blocks [x G/8, (x+1) G/8), i.e. a contiguous
-// run of replicas, so each XCD's L2 holds only its replicas' links.
-constexpr uint16_t NO_REP = 0xffff;
+// NO_QUAD), XCD x runs logical blocks [x G/8, (x+1) G/8), i.e. a contiguous
+// run of quads, so each XCD's L2 holds only its replicas' links.
+constexpr uint16_t NO_QUAD = 0xffff;
 __device__ __forceinline__ uint32_t ens_logical_block() {
     const uint32_t b = blockIdx.x, G = gridDim.x;
     if ((G & 7u) != 0) return b;
@@ -111,127 +126,112 @@ __device__ __forceinline__ uint32_t ens_logical_block() {
 
 // One (MU, c) pass of every replica.  sweep = *sweep_base + sweep_off.
 template <int D, int MODE, int MU>
-__global__ __launch_bounds__(256) void ens_link_pass(uint32_t *__restrict__ words, const EnsReplica *__restrict__ reps,
-                                                     const uint16_t *__restrict__ block_rep,
+__global__ __launch_bounds__(256) void ens_link_pass(uint32_t *__restrict__ words, const EnsQuad *__restrict__ quads,
+                                                     const uint16_t *__restrict__ block_quad,
                                                      const uint32_t *__restrict__ tables,
                                                      const uint32_t *__restrict__ beta_slot,
                                                      const uint32_t *__restrict__ sweep_base, uint32_t sweep_off,
                                                      int c, int per_lane) {
     constexpr int K = 2 * (D - 1);
-    __shared__ uint32_t T[2 * (K + 1)];
+    constexpr int TL = 2 * (K + 1);
+    __shared__ uint32_t T[4 * TL];
     const uint32_t lb = ens_logical_block();
-    const int ri = block_rep[lb];
-    if (ri == NO_REP) return;
-    const EnsReplica R = reps[ri];
-    if (threadIdx.x < 2 * (K + 1)) T[threadIdx.x] = tables[beta_slot[ri] * (2 * (K + 1)) + threadIdx.x];
+    const int qi = block_quad[lb];
+    if (qi == NO_QUAD) return;
+    const EnsQuad Q = quads[qi];
+    if (threadIdx.x < 4 * TL) {
+        const int j = threadIdx.x / TL, e = threadIdx.x - j * TL;
+        const int rj = quads[qi].rep[j];  // runtime index: read from memory, not the register copy
+        T[threadIdx.x] = rj >= 0 ? tables[beta_slot[rj] * TL + e] : 0u;
+    }
     __syncthreads();
+    const uint32_t uc = (uint32_t)c;
     const uint32_t skey = (*sweep_base + sweep_off) * GOLDEN;
-    const uint32_t half = R.nsites >> 1;
-    const uint32_t base = (lb - R.block0) * blockDim.x * (uint32_t)per_lane + threadIdx.x;
+    const uint32_t half = Q.nsites >> 1;
+    const uint32_t base = (lb - Q.block0) * blockDim.x * (uint32_t)per_lane + threadIdx.x;
     for (int k = 0; k < per_lane; ++k) {
         const uint32_t item = base + (uint32_t)k * blockDim.x;
         if (item >= half) break;
-        LinkSite s;
-        s.r = fdiv(item, R.mH);
-        const uint32_t i = item - s.r * (uint32_t)R.H;
-        uint32_t rest = s.r;
-        s.x[0] = 0;
-        s.x[1] = 0;
-        s.x[2] = 0;
-        s.x[3] = 0;
-        if (D >= 2) {
-            uint32_t q = fdiv(rest, R.mL[1]);
-            s.x[1] = rest - q * (uint32_t)R.L[1];
-            rest = q;
+        const LinkSite s = site_of<D>(Q, item, uc);
+        const uint32_t ixp = cidx(Q, lstep<MU, +1>(Q, s));
+        uint32_t neg = 0;
+        if (MU != 0) neg += staple_negs4<MU, 0>(words, Q, half, s, uc, item, ixp);
+        if (D > 1 && MU != 1) neg += staple_negs4<MU, 1>(words, Q, half, s, uc, item, ixp);
+        if (D > 2 && MU != 2) neg += staple_negs4<MU, (D > 2 ? 2 : 0)>(words, Q, half, s, uc, item, ixp);
+        if (D > 3 && MU != 3) neg += staple_negs4<MU, (D > 3 ? 3 : 0)>(words, Q, half, s, uc, item, ixp);
+        const uint32_t widx = arr(Q, half, MU, uc) + item;
+        const uint32_t wcur = ld32(words, widx);
+        const uint32_t link = (s.r * (uint32_t)Q.L[0] + s.x0) * (uint32_t)D + (uint32_t)MU;
+        uint32_t wnew = wcur;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (Q.rep[j] < 0) continue;
+            const uint32_t cur = (wcur >> (8 * j)) & 1u;
+            const uint32_t n = (neg >> (8 * j)) & 0xffu;
+            const uint32_t t = T[j * TL + cur * (K + 1) + n];
+            const bool hit = keyed_bits_k(Q.seed32[j], link, skey) < t;
+            const uint32_t nxt = MODE == 0 ? (cur ^ (hit ? 1u : 0u)) : (hit ? 0u : 1u);
+            wnew = (wnew & ~(0xffu << (8 * j))) | (nxt << (8 * j));
         }
-        if (D >= 3) {
-            uint32_t q = fdiv(rest, R.mL[2]);
-            s.x[2] = rest - q * (uint32_t)R.L[2];
-            rest = q;
-        }
-        if (D >= 4) s.x[3] = rest;
-        const uint32_t rp = row_parity(s);
-        s.x0 = 2u * i + ((rp + (uint32_t)c) & 1u);
-        const uint32_t wx = lword(words, R, s, rp);
-        const LinkSite xm = lstep<MU, +1>(R, s);
-        const uint32_t wxm = lword(words, R, xm, row_parity(xm));
-        const int neg = link_negs<D, MU>(words, R, s, wx, wxm);
-        const uint32_t cur = byte_of(wx, MU);
-        const uint32_t t = T[cur * (K + 1) + (uint32_t)neg];
-        const uint32_t site = s.r * (uint32_t)R.L[0] + s.x0;
-        const uint32_t mm = keyed_bits_k(R.seed32, site * (uint32_t)D + (uint32_t)MU, skey);
-        const bool hit = mm < t;
-        const uint32_t nxt = MODE == 0 ? (cur ^ (hit ? 1u : 0u)) : (hit ? 0u : 1u);
-        if (nxt != cur) {
-            uint8_t *bytes = reinterpret_cast<uint8_t *>(words);
-            const uint32_t widx = (c ? R.off[1] : R.off[0]) + s.r * (uint32_t)R.H + i;
-            bytes[(size_t)widx * 4u + (uint32_t)MU] = (uint8_t)nxt;
-        }
+        if (wnew != wcur) words[widx] = wnew;
     }
 }
 
-// U_A(x) U_B(x+A) U_A(x+B) U_B(x) (lattice.rs:420-430) as 0 (+1) / 1 (-1)
+// Signed plaquette sum of every replica into sums[r] (zeroed by the caller):
+// per site and plane (A < B), U_A(x) U_B(x+A) U_A(x+B) U_B(x)
+// (lattice.rs:420-430), negative when the XOR of the four bits is 1.
 template <int A, int B>
-__device__ __forceinline__ uint32_t plaq_bit(const uint32_t *__restrict__ words, const EnsReplica &R,
-                                             const LinkSite &s, uint32_t wx) {
-    const LinkSite sa = lstep<A, +1>(R, s);
-    const LinkSite sb = lstep<B, +1>(R, s);
-    const uint32_t wa = lword(words, R, sa, row_parity(sa));
-    const uint32_t wb = lword(words, R, sb, row_parity(sb));
-    return byte_of(wx, A) ^ byte_of(wa, B) ^ byte_of(wb, A) ^ byte_of(wx, B);
+__device__ __forceinline__ uint32_t plaq_bits4(const uint32_t *__restrict__ words, const EnsQuad &Q, uint32_t half,
+                                               const LinkSite &s, uint32_t c, uint32_t item) {
+    const uint32_t o = 1u - c;
+    return ld32(words, arr(Q, half, A, c) + item) ^ ld32(words, arr(Q, half, B, o) + cidx(Q, lstep<A, +1>(Q, s))) ^
+           ld32(words, arr(Q, half, A, o) + cidx(Q, lstep<B, +1>(Q, s))) ^ ld32(words, arr(Q, half, B, c) + item);
 }
 
-// Signed plaquette sum of every replica into sums[r] (zeroed by the caller).
 template <int D>
 __global__ __launch_bounds__(256) void ens_plaquette(const uint32_t *__restrict__ words,
-                                                     const EnsReplica *__restrict__ reps,
-                                                     const uint16_t *__restrict__ block_rep,
+                                                     const EnsQuad *__restrict__ quads,
+                                                     const uint16_t *__restrict__ block_quad,
                                                      unsigned long long *__restrict__ sums, int per_lane) {
-    __shared__ long long lds[8];
+    constexpr int PLANES = D * (D - 1) / 2;
+    __shared__ long long lds[4][8];
     const uint32_t lb = ens_logical_block();
-    const int ri = block_rep[lb];
-    if (ri == NO_REP) return;
-    const EnsReplica R = reps[ri];
-    long long acc = 0;
-    const uint32_t base = (lb - R.block0) * blockDim.x * (uint32_t)per_lane + threadIdx.x;
+    const int qi = block_quad[lb];
+    if (qi == NO_QUAD) return;
+    const EnsQuad Q = quads[qi];
+    const uint32_t half = Q.nsites >> 1;
+    uint32_t negs = 0;   // per byte: negative plaquettes, at most 6 per site and 8 sites
+    uint32_t sites = 0;
+    const uint32_t base = (lb - Q.block0) * blockDim.x * (uint32_t)per_lane + threadIdx.x;
     for (int k = 0; k < 2 * per_lane; ++k) {
-        // every site once: items [0, nsites) in pairs of half-lattices
+        // every site once: both colours of each of the lane's items
         const uint32_t item = base + (uint32_t)(k >> 1) * blockDim.x;
-        const uint32_t half = R.nsites >> 1;
         if (item >= half) break;
-        LinkSite s;
-        s.r = fdiv(item, R.mH);
-        const uint32_t i = item - s.r * (uint32_t)R.H;
-        uint32_t rest = s.r;
-        s.x[0] = s.x[1] = s.x[2] = s.x[3] = 0;
-        if (D >= 2) {
-            uint32_t q = fdiv(rest, R.mL[1]);
-            s.x[1] = rest - q * (uint32_t)R.L[1];
-            rest = q;
-        }
-        if (D >= 3) {
-            uint32_t q = fdiv(rest, R.mL[2]);
-            s.x[2] = rest - q * (uint32_t)R.L[2];
-            rest = q;
-        }
-        if (D >= 4) s.x[3] = rest;
-        const uint32_t rp = row_parity(s);
-        s.x0 = 2u * i + ((rp + (uint32_t)(k & 1)) & 1u);
-        const uint32_t wx = lword(words, R, s, rp);
-        uint32_t neg = plaq_bit<0, 1>(words, R, s, wx);
-        if (D >= 3) neg += plaq_bit<0, (D >= 3 ? 2 : 1)>(words, R, s, wx) + plaq_bit<1, (D >= 3 ? 2 : 1)>(words, R, s, wx);
-        if (D >= 4)
-            neg += plaq_bit<0, (D >= 4 ? 3 : 1)>(words, R, s, wx) + plaq_bit<1, (D >= 4 ? 3 : 1)>(words, R, s, wx) +
-                   plaq_bit<2, (D >= 4 ? 3 : 1)>(words, R, s, wx);
-        acc += (long long)(D * (D - 1) / 2) - 2 * (long long)neg;
+        const uint32_t c = (uint32_t)(k & 1);
+        const LinkSite s = site_of<D>(Q, item, c);
+        uint32_t n = plaq_bits4<0, 1>(words, Q, half, s, c, item);
+        if (D >= 3) n += plaq_bits4<0, (D >= 3 ? 2 : 1)>(words, Q, half, s, c, item) +
+                         plaq_bits4<1, (D >= 3 ? 2 : 1)>(words, Q, half, s, c, item);
+        if (D >= 4) n += plaq_bits4<0, (D >= 4 ? 3 : 1)>(words, Q, half, s, c, item) +
+                         plaq_bits4<1, (D >= 4 ? 3 : 1)>(words, Q, half, s, c, item) +
+                         plaq_bits4<2, (D >= 4 ? 3 : 1)>(words, Q, half, s, c, item);
+        negs += n;
+        sites += 1;
     }
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = acc;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        long long acc = (long long)sites * PLANES - 2 * (long long)((negs >> (8 * j)) & 0xffu);
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if ((threadIdx.x & 63) == 0) lds[j][threadIdx.x >> 6] = acc;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        long long t = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += lds[w];
-        atomicAdd(&sums[ri], (unsigned long long)t);
+    if (threadIdx.x < 4) {
+        const int rj = quads[qi].rep[threadIdx.x];  // runtime index: read from memory
+        if (rj >= 0) {
+            long long t = 0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += lds[threadIdx.x][w];
+            atomicAdd(&sums[rj], (unsigned long long)t);
+        }
     }
 }
 
@@ -298,51 +298,52 @@ __global__ void ens_record(const unsigned long long *__restrict__ sums, const ui
     if (threadIdx.x == 0) counters[2] += 1;
 }
 
-// lattice order (link = site * D + dir) <-> words
-__global__ void ens_pack(const uint8_t *__restrict__ lat, uint32_t *__restrict__ words, EnsReplica R, int D) {
+// lattice order (link = site * D + dir) <-> byte j of the quad's words
+__global__ void ens_pack(const uint8_t *__restrict__ lat, uint32_t *__restrict__ words, EnsQuad Q, int j, int D) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= R.nsites) return;
-    uint32_t r = s / (uint32_t)R.L[0];
-    uint32_t x0 = s - r * (uint32_t)R.L[0];
+    if (s >= Q.nsites) return;
+    uint32_t r = s / (uint32_t)Q.L[0];
+    uint32_t x0 = s - r * (uint32_t)Q.L[0];
     uint32_t rest = r, par = x0;
     for (int a = 1; a < D; ++a) {
-        par += rest % (uint32_t)R.L[a];
-        rest /= (uint32_t)R.L[a];
+        par += rest % (uint32_t)Q.L[a];
+        rest /= (uint32_t)Q.L[a];
     }
-    uint32_t w = 0;
-    for (int d = 0; d < D; ++d) w |= (uint32_t)lat[(size_t)s * D + d] << (8 * d);
-    words[R.off[par & 1u] + r * (uint32_t)R.H + (x0 >> 1)] = w;
+    const uint32_t half = Q.nsites >> 1, c = par & 1u, idx = r * (uint32_t)Q.H + (x0 >> 1);
+    uint8_t *bytes = reinterpret_cast<uint8_t *>(words);
+    for (int d = 0; d < D; ++d)
+        bytes[((size_t)Q.woff + ((uint32_t)d * 2u + c) * half + idx) * 4u + (uint32_t)j] = lat[(size_t)s * D + d];
 }
 
-__global__ void ens_unpack(uint8_t *__restrict__ lat, const uint32_t *__restrict__ words, EnsReplica R, int D) {
+__global__ void ens_unpack(uint8_t *__restrict__ lat, const uint32_t *__restrict__ words, EnsQuad Q, int j, int D) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= R.nsites) return;
-    uint32_t r = s / (uint32_t)R.L[0];
-    uint32_t x0 = s - r * (uint32_t)R.L[0];
+    if (s >= Q.nsites) return;
+    uint32_t r = s / (uint32_t)Q.L[0];
+    uint32_t x0 = s - r * (uint32_t)Q.L[0];
     uint32_t rest = r, par = x0;
     for (int a = 1; a < D; ++a) {
-        par += rest % (uint32_t)R.L[a];
-        rest /= (uint32_t)R.L[a];
+        par += rest % (uint32_t)Q.L[a];
+        rest /= (uint32_t)Q.L[a];
     }
-    uint32_t w = words[R.off[par & 1u] + r * (uint32_t)R.H + (x0 >> 1)];
-    for (int d = 0; d < D; ++d) lat[(size_t)s * D + d] = (uint8_t)((w >> (8 * d)) & 0xffu);
+    const uint32_t half = Q.nsites >> 1, c = par & 1u, idx = r * (uint32_t)Q.H + (x0 >> 1);
+    const uint8_t *bytes = reinterpret_cast<const uint8_t *>(words);
+    for (int d = 0; d < D; ++d)
+        lat[(size_t)s * D + d] = bytes[((size_t)Q.woff + ((uint32_t)d * 2u + c) * half + idx) * 4u + (uint32_t)j];
 }
 
-// ------------------------------------------------------------- launchers ---
-
-void launch_ens_pack(const uint8_t *lat, uint32_t *words, const EnsReplica &R, int D, hipStream_t s) {
-    ens_pack<<<dim3((R.nsites + 255) / 256), dim3(256), 0, s>>>(lat, words, R, D);
+void launch_ens_pack(const uint8_t *lat, uint32_t *words, const EnsQuad &Q, int j, int D, hipStream_t s) {
+    ens_pack<<<dim3((Q.nsites + 255) / 256), dim3(256), 0, s>>>(lat, words, Q, j, D);
 }
 
-void launch_ens_unpack(uint8_t *lat, const uint32_t *words, const EnsReplica &R, int D, hipStream_t s) {
-    ens_unpack<<<dim3((R.nsites + 255) / 256), dim3(256), 0, s>>>(lat, words, R, D);
+void launch_ens_unpack(uint8_t *lat, const uint32_t *words, const EnsQuad &Q, int j, int D, hipStream_t s) {
+    ens_unpack<<<dim3((Q.nsites + 255) / 256), dim3(256), 0, s>>>(lat, words, Q, j, D);
 }
 
 template <int D, int MODE>
 static void ens_pass_mu(const EnsPassArgs &a) {
     dim3 grid(a.blocks), block(a.block_size);
-#define PLQ_ENS_MU(M)                                                                                              \
-    ens_link_pass<D, MODE, M><<<grid, block, 0, a.stream>>>(a.words, a.reps, a.block_rep, a.tables, a.beta_slot, \
+#define PLQ_ENS_MU(M)                                                                                                \
+    ens_link_pass<D, MODE, M><<<grid, block, 0, a.stream>>>(a.words, a.quads, a.block_quad, a.tables, a.beta_slot, \
                                                             a.sweep_base, a.sweep_off, a.c, a.per_lane)
     switch (a.mu) {
         case 0: PLQ_ENS_MU(0); break;
@@ -364,14 +365,14 @@ void launch_ens_pass(const EnsPassArgs &a) {
     }
 }
 
-void launch_ens_plaquette(const uint32_t *words, const EnsReplica *reps, const uint16_t *block_rep, int nrep, int D,
+void launch_ens_plaquette(const uint32_t *words, const EnsQuad *quads, const uint16_t *block_quad, int nrep, int D,
                           uint32_t blocks, int block_size, int per_lane, unsigned long long *sums, hipStream_t s) {
     ens_zero<<<dim3((nrep + 255) / 256), dim3(256), 0, s>>>(sums, nrep);
     dim3 grid(blocks), block(block_size);
     switch (D) {
-        case 2: ens_plaquette<2><<<grid, block, 0, s>>>(words, reps, block_rep, sums, per_lane); break;
-        case 3: ens_plaquette<3><<<grid, block, 0, s>>>(words, reps, block_rep, sums, per_lane); break;
-        default: ens_plaquette<4><<<grid, block, 0, s>>>(words, reps, block_rep, sums, per_lane); break;
+        case 2: ens_plaquette<2><<<grid, block, 0, s>>>(words, quads, block_quad, sums, per_lane); break;
+        case 3: ens_plaquette<3><<<grid, block, 0, s>>>(words, quads, block_quad, sums, per_lane); break;
+        default: ens_plaquette<4><<<grid, block, 0, s>>>(words, quads, block_quad, sums, per_lane); break;
     }
 }
 

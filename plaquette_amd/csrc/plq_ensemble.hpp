@@ -18,6 +18,23 @@ struct EnsReplica {
     uint32_t seed32;      // fold_seed of the replica's seed
 };
 
+// Four replicas of one shape share their link words: byte j of every word
+// is replica rep[j]'s link (rep[j] < 0: an unused byte).  Word arrays per
+// (direction mu, colour c): woff + (mu * 2 + c) * nsites / 2, each in the
+// colour-interleaved site order r * H + x0 / 2.
+struct EnsQuad {
+    int32_t L[4];         // extents (unused axes 1)
+    int32_t H;            // L0 / 2 sites per colour row
+    int32_t rstride[4];   // row stride of axis a >= 1
+    uint32_t nsites;
+    uint64_t mH;          // ceil(2^40 / H)
+    uint64_t mL[4];       // ceil(2^40 / L[a])
+    uint32_t woff;        // first word of the quad
+    uint32_t block0;      // first block of this quad in a pass launch
+    int32_t rep[4];       // replica in byte j, or -1
+    uint32_t seed32[4];   // fold_seed of each replica's seed
+};
+
 struct EnsGroupPair {
     uint32_t k;           // the pair (k, k+1) of its group's sorted betas
     uint32_t rep_at0;     // offset of the group's rep_at list
@@ -28,8 +45,8 @@ struct EnsGroupPair {
 
 struct EnsPassArgs {
     uint32_t *words;
-    const EnsReplica *reps;
-    const uint16_t *block_rep;   // replica of each block
+    const EnsQuad *quads;
+    const uint16_t *block_quad;  // quad of each (logical) block
     int nrep;
     int D;
     const uint32_t *tables;      // per beta slot: 2 * (2D - 1) thresholds
@@ -44,10 +61,11 @@ struct EnsPassArgs {
     hipStream_t stream;
 };
 
-void launch_ens_pack(const uint8_t *lat, uint32_t *words, const EnsReplica &R, int D, hipStream_t s);
-void launch_ens_unpack(uint8_t *lat, const uint32_t *words, const EnsReplica &R, int D, hipStream_t s);
+// Replica in byte j of quad Q: lattice order (link = site * D + dir) <-> words.
+void launch_ens_pack(const uint8_t *lat, uint32_t *words, const EnsQuad &Q, int j, int D, hipStream_t s);
+void launch_ens_unpack(uint8_t *lat, const uint32_t *words, const EnsQuad &Q, int j, int D, hipStream_t s);
 void launch_ens_pass(const EnsPassArgs &a);
-void launch_ens_plaquette(const uint32_t *words, const EnsReplica *reps, const uint16_t *block_rep, int nrep, int D,
+void launch_ens_plaquette(const uint32_t *words, const EnsQuad *quads, const uint16_t *block_quad, int nrep, int D,
                           uint32_t blocks, int block_size, int per_lane, unsigned long long *sums, hipStream_t s);
 void launch_ens_exchange(const EnsGroupPair *pairs, int npairs, uint32_t *slot_of, uint32_t *rep_at,
                          const unsigned long long *sums, const uint32_t *ex_tables, const uint32_t *round_base,

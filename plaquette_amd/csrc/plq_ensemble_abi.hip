@@ -56,12 +56,14 @@ struct plq_ensemble {
     hipStream_t stream = nullptr;
     int D = 0, nrep = 0, kernel = 0, profile = 0;
     double j = 1.0;
-    std::vector<plq::EnsReplica> reps;
+    std::vector<plq::EnsReplica> reps;  // per replica geometry and seed (host side)
     std::vector<double> betas;
     std::vector<size_t> lat_off;  // per replica offset in the lattice-order concatenation
     size_t lat_bytes = 0;
-    plq::EnsReplica *d_reps = nullptr;
-    uint16_t *d_block_rep = nullptr;  // replica of each pass block
+    std::vector<plq::EnsQuad> quads;  // replicas of one shape, four to a word
+    std::vector<int> rep_quad, rep_byte;
+    plq::EnsQuad *d_quads = nullptr;
+    uint16_t *d_block_rep = nullptr;  // quad of each pass block
     size_t block_rep_cap = 0;
     uint32_t *d_words = nullptr;
     size_t nwords = 0;
@@ -92,7 +94,7 @@ struct plq_ensemble {
         (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        for (void *p : {(void *)d_reps, (void *)d_block_rep, (void *)d_words, (void *)d_lat, (void *)d_tables, (void *)d_slot_of,
+        for (void *p : {(void *)d_quads, (void *)d_block_rep, (void *)d_words, (void *)d_lat, (void *)d_tables, (void *)d_slot_of,
                         (void *)d_rep_at, (void *)d_pairs, (void *)d_ex_tables, (void *)d_sums, (void *)d_accepted,
                         (void *)d_counters, (void *)d_ring_sums, (void *)d_ring_slots})
             if (p) (void)hipFree(p);
@@ -106,20 +108,21 @@ namespace {
 int layout_blocks(plq_ensemble *e) {
     uint32_t b = 0;
     uint32_t per_block = (uint32_t)e->block_size * (uint32_t)e->per_lane;
-    for (auto &R : e->reps) {
-        R.block0 = b;
-        uint32_t half = R.nsites / 2;
+    for (auto &Q : e->quads) {
+        Q.block0 = b;
+        uint32_t half = Q.nsites / 2;
         b += (half + per_block - 1) / per_block;
     }
     // Pad the grid to a multiple of 8 so the kernels can give each XCD a
-    // contiguous run of replicas; padded blocks map to no replica (0xffff).
+    // contiguous run of quads; padded blocks map to no quad (0xffff).
     const uint32_t used = b;
     b = (b + 7u) & ~7u;
     e->total_blocks = b;
+    const int nq = (int)e->quads.size();
     std::vector<uint16_t> block_rep(b, (uint16_t)0xffff);
-    for (int r = 0; r < e->nrep; ++r) {
-        uint32_t end = r + 1 < e->nrep ? e->reps[r + 1].block0 : used;
-        for (uint32_t k = e->reps[r].block0; k < end; ++k) block_rep[k] = (uint16_t)r;
+    for (int q = 0; q < nq; ++q) {
+        uint32_t end = q + 1 < nq ? e->quads[q + 1].block0 : used;
+        for (uint32_t k = e->quads[q].block0; k < end; ++k) block_rep[k] = (uint16_t)q;
     }
     if (e->block_rep_cap < b) {
         if (e->d_block_rep) (void)hipFree(e->d_block_rep);
@@ -129,7 +132,7 @@ int layout_blocks(plq_ensemble *e) {
         e->block_rep_cap = b;
     }
     ENS_HIP(hipMemcpy(e->d_block_rep, block_rep.data(), sizeof(uint16_t) * b, hipMemcpyHostToDevice));
-    ENS_HIP(hipMemcpy(e->d_reps, e->reps.data(), sizeof(plq::EnsReplica) * e->reps.size(), hipMemcpyHostToDevice));
+    ENS_HIP(hipMemcpy(e->d_quads, e->quads.data(), sizeof(plq::EnsQuad) * e->quads.size(), hipMemcpyHostToDevice));
     if (e->graph_exec) {
         (void)hipGraphExecDestroy(e->graph_exec);
         e->graph_exec = nullptr;
@@ -140,8 +143,8 @@ int layout_blocks(plq_ensemble *e) {
 void enqueue_block(plq_ensemble *e, uint32_t nsweeps, bool measure) {
     plq::EnsPassArgs a{};
     a.words = e->d_words;
-    a.reps = e->d_reps;
-    a.block_rep = e->d_block_rep;
+    a.quads = e->d_quads;
+    a.block_quad = e->d_block_rep;
     a.nrep = e->nrep;
     a.D = e->D;
     a.tables = e->d_tables;
@@ -163,7 +166,7 @@ void enqueue_block(plq_ensemble *e, uint32_t nsweeps, bool measure) {
     }
     uint32_t rounds = 0;
     if (measure) {
-        plq::launch_ens_plaquette(e->d_words, e->d_reps, e->d_block_rep, e->nrep, e->D, e->total_blocks, e->block_size,
+        plq::launch_ens_plaquette(e->d_words, e->d_quads, e->d_block_rep, e->nrep, e->D, e->total_blocks, e->block_size,
                                   e->per_lane, e->d_sums, e->stream);
         plq::launch_ens_record(e->d_sums, e->d_slot_of, e->nrep, e->d_ring_sums, e->d_ring_slots,
                                (uint32_t)e->rec_cap, e->d_counters, e->stream);
@@ -371,12 +374,44 @@ int plq_ensemble_create(const plq_ensemble_desc *d, const uint8_t *start, size_t
             e->pairs.push_back(P);
         }
     }
+    // quads: the replicas of each shape in descriptor order, four to a word
+    e->rep_quad.assign(e->nrep, -1);
+    e->rep_byte.assign(e->nrep, -1);
+    uint32_t qwoff = 0;
+    for (auto &key : order) {
+        const std::vector<int> &members = groups[key];
+        for (size_t m0 = 0; m0 < members.size(); m0 += 4) {
+            const plq::EnsReplica &R = e->reps[members[m0]];
+            plq::EnsQuad Q{};
+            for (int a = 0; a < 4; ++a) {
+                Q.L[a] = R.L[a];
+                Q.rstride[a] = R.rstride[a];
+                Q.mL[a] = R.mL[a];
+                Q.rep[a] = -1;
+            }
+            Q.H = R.H;
+            Q.nsites = R.nsites;
+            Q.mH = R.mH;
+            Q.woff = qwoff;
+            qwoff += (uint32_t)e->D * R.nsites;
+            for (size_t jj = 0; jj < 4 && m0 + jj < members.size(); ++jj) {
+                const int r = members[m0 + jj];
+                Q.rep[jj] = r;
+                Q.seed32[jj] = e->reps[r].seed32;
+                e->rep_quad[r] = (int)e->quads.size();
+                e->rep_byte[r] = (int)jj;
+            }
+            e->quads.push_back(Q);
+        }
+    }
+    e->nwords = qwoff;
+    if (e->quads.size() > 65535) return bail(plq::host_fail(PLQ_EINVAL, "too many replica quads"));
     std::vector<uint32_t> slot_of(e->nrep);
     for (int r = 0; r < e->nrep; ++r) slot_of[r] = (uint32_t)r;
     int rc;
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(plq::host_fail(PLQ_EDEVICE, "hipStreamCreate failed"));
-    if ((rc = dalloc((void **)&e->d_reps, sizeof(plq::EnsReplica) * e->nrep)) ||
+    if ((rc = dalloc((void **)&e->d_quads, sizeof(plq::EnsQuad) * e->quads.size())) ||
         (rc = dalloc((void **)&e->d_words, sizeof(uint32_t) * e->nwords)) ||
         (rc = dalloc((void **)&e->d_lat, e->lat_bytes)) ||
         (rc = dalloc((void **)&e->d_tables, sizeof(uint32_t) * tables.size())) ||
@@ -399,10 +434,12 @@ int plq_ensemble_create(const plq_ensemble_desc *d, const uint8_t *start, size_t
          hipMemcpy(e->d_ex_tables, ex_tables.data(), sizeof(uint32_t) * ex_tables.size(), hipMemcpyHostToDevice)) ||
         hipMemset(e->d_accepted, 0, sizeof(unsigned long long) * std::max<size_t>(1, e->pairs.size())) ||
         hipMemset(e->d_counters, 0, sizeof(uint32_t) * 4) ||
+        hipMemset(e->d_words, 0, sizeof(uint32_t) * e->nwords) ||  // unused quad bytes stay 0
         hipMemcpy(e->d_lat, start, n, hipMemcpyHostToDevice))
         return bail(plq::host_fail(PLQ_EDEVICE, "uploading the ensemble failed"));
     for (int r = 0; r < e->nrep; ++r)
-        plq::launch_ens_pack(e->d_lat + e->lat_off[r], e->d_words, e->reps[r], e->D, e->stream);
+        plq::launch_ens_pack(e->d_lat + e->lat_off[r], e->d_words, e->quads[e->rep_quad[r]], e->rep_byte[r], e->D,
+                             e->stream);
     if ((rc = layout_blocks(e)) != PLQ_OK) return bail(rc);
     if (hipStreamSynchronize(e->stream) != hipSuccess) return bail(plq::host_fail(PLQ_EDEVICE, "pack failed"));
     *out = e;
@@ -428,7 +465,8 @@ int plq_ensemble_read(plq_ensemble *e, uint8_t *out, size_t n) {
     std::lock_guard<std::mutex> lk(e->mu);
     ENS_HIP(hipSetDevice(e->device));
     for (int r = 0; r < e->nrep; ++r)
-        plq::launch_ens_unpack(e->d_lat + e->lat_off[r], e->d_words, e->reps[r], e->D, e->stream);
+        plq::launch_ens_unpack(e->d_lat + e->lat_off[r], e->d_words, e->quads[e->rep_quad[r]], e->rep_byte[r], e->D,
+                               e->stream);
     ENS_HIP(hipMemcpyAsync(out, e->d_lat, n, hipMemcpyDeviceToHost, e->stream));
     ENS_HIP(hipStreamSynchronize(e->stream));
     return PLQ_OK;
@@ -439,7 +477,7 @@ int plq_ensemble_plaquette_sums(plq_ensemble *e, int64_t *sums, size_t n) {
     if (n != (size_t)e->nrep) return plq::host_fail(PLQ_EINVAL, "one sum per replica");
     std::lock_guard<std::mutex> lk(e->mu);
     ENS_HIP(hipSetDevice(e->device));
-    plq::launch_ens_plaquette(e->d_words, e->d_reps, e->d_block_rep, e->nrep, e->D, e->total_blocks, e->block_size, e->per_lane,
+    plq::launch_ens_plaquette(e->d_words, e->d_quads, e->d_block_rep, e->nrep, e->D, e->total_blocks, e->block_size, e->per_lane,
                               e->d_sums, e->stream);
     std::vector<unsigned long long> h(e->nrep);
     ENS_HIP(hipMemcpyAsync(h.data(), e->d_sums, sizeof(unsigned long long) * e->nrep, hipMemcpyDeviceToHost,
