@@ -310,6 +310,44 @@ def bench_ising2d_64(P, ev, steps, warmup):
 BLOCK_FLOP = 1253376  # network FLOPs per 8x8 block: 64 x 32 x 9 x 2 (conv1) + 64 x 32 x 288 x 2 (conv2) + 64 x 288 x 2 (conv3)
 
 
+def learned_small(P, ev, stats):
+    """The same comparison on 64^2 at beta_c, where tau_int of |M| (about 150
+    sweeps) converges within 100,000 steps: train 300 steps on the chain's
+    configurations, then time plain and mixed chains of 100,000 steps each."""
+    import numpy as np
+
+    L, beta, steps = 64, 0.4406868, 100000
+    n = L * L
+    start = np.random.default_rng(SEED + 2).integers(0, 2, n).astype(np.uint8)
+    c = P.ising_gpu_chain([L, L], 1.0, 0.0, beta, SEED + 2, start)
+    c.advance(5000)
+    net = P.BlockNet(SEED + 2)
+    net.train(c, 300, 2, 3e-3)
+    st = c.read()
+    c.close()
+    out = {}
+    for mode in ("plain", "mixed"):
+        ch = P.ising_gpu_chain([L, L], 1.0, 0.0, beta, SEED + 3, st)
+        ch.record(1, steps)
+        ch.sync()
+        if mode == "plain":
+            wall, _ = timed(P, ev, ch.stream, lambda: (ch.advance(steps), ch.sync()))
+        else:
+            wall, _ = timed(P, ev, ch.stream, lambda: (ch.advance_with_blocks(net, steps, 1), ch.sync()))
+        m = np.array([abs(o.spin_sum) / n for o in ch.series()])
+        est = stats.reduce(m)
+        acc, prop, _ = ch.block_stats()
+        out[mode] = {"steps_per_s": round(steps / wall, 1), "tau_int_abs_m": round(est.tau_int, 2),
+                     "samples_per_s": round(steps / wall / (2.0 * est.tau_int), 2),
+                     "window_converged": bool(steps >= 50 * est.tau_int),
+                     "block_acceptance": (acc / prop) if prop else None}
+        ch.close()
+    net.close()
+    out["ratio_vs_plain"] = round(out["mixed"]["samples_per_s"] / out["plain"]["samples_per_s"], 4)
+    out["tau_ratio_plain_over_mixed"] = round(out["plain"]["tau_int_abs_m"] / out["mixed"]["tau_int_abs_m"], 3)
+    return out
+
+
 def bench_learned(P, ev):
     """2D Ising 512^2 at beta_c: the learned block proposal against the plain
     checkerboard.  A chain from a hot start runs 2,000 sweeps, then trains
@@ -359,6 +397,7 @@ def bench_learned(P, ev):
         ch.close()
     net.close()
     ratio = runs["mixed"]["samples_per_s"] / runs["plain"]["samples_per_s"]
+    small = learned_small(P, ev, stats)
     flop = (n // 64) * BLOCK_FLOP
     tflops = flop / (block_us * 1e-6) / 1e12
     return {
@@ -370,6 +409,7 @@ def bench_learned(P, ev):
         "config": {"workload": "2D Ising 512^2 at beta_c = 0.4406868, 4,096 blocks of 8x8, 3-layer 3x3 CNN with "
                                "32 channels, 1,000 training steps", "lattice": [L, L]},
         "ratio_vs_plain": round(ratio, 4), "plain": runs["plain"], "mixed": runs["mixed"],
+        "converged_64": small,
         "training": {"wall_s": round(wall_train, 3), "final_loss": float(losses[-1])},
         "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(tflops / 2500.0, 5), "traffic": None,

@@ -85,10 +85,11 @@ __global__ void unpack_links_kernel(uint8_t *__restrict__ lat, uint8_t *const *_
 }
 
 // One (dir, colour) pass: a lane per base site of that colour.
-template <int D, int MODE>
-__global__ __launch_bounds__(256) void link_pass_kernel(uint8_t *const *__restrict__ arr, Geom g, int mu, int c,
+template <int D, int MODE, int MU>
+__global__ __launch_bounds__(256) void link_pass_kernel(uint8_t *const *__restrict__ arr, Geom g, int c,
                                                         uint32_t seed, uint32_t skey,
                                                         const uint32_t *__restrict__ table) {
+    constexpr int mu = MU;  // a template parameter, so every coordinate index is static (no scratch)
     constexpr int K = 2 * (D - 1);
     __shared__ uint32_t T[2 * (K + 1)];
     if (threadIdx.x < 2 * (K + 1)) T[threadIdx.x] = table[threadIdx.x];
@@ -176,17 +177,25 @@ void launch_unpack_links(uint8_t *lat, uint8_t *const *arr, const Geom &g, hipSt
 void launch_link_pass(const LinkPassArgs &a) {
     int64_t work = (int64_t)a.g.R * a.g.H;
     dim3 grid((uint32_t)((work + 255) / 256)), block(256);
-#define PLQ_LINK(DD)                                                                                               \
-    if (a.mode == 0)                                                                                               \
-        link_pass_kernel<DD, 0><<<grid, block, 0, a.stream>>>(a.arr, a.g, a.dir, a.color, a.seed, a.skey, a.table); \
-    else                                                                                                           \
-        link_pass_kernel<DD, 1><<<grid, block, 0, a.stream>>>(a.arr, a.g, a.dir, a.color, a.seed, a.skey, a.table)
-    switch (a.g.D) {
-        case 2: PLQ_LINK(2); break;
-        case 3: PLQ_LINK(3); break;
-        default: PLQ_LINK(4); break;
+#define PLQ_LINK_MU(DD, MD, M) \
+    link_pass_kernel<DD, MD, M><<<grid, block, 0, a.stream>>>(a.arr, a.g, a.color, a.seed, a.skey, a.table)
+#define PLQ_LINK(DD, MD)                                 \
+    switch (a.dir) {                                     \
+        case 0: PLQ_LINK_MU(DD, MD, 0); break;           \
+        case 1: PLQ_LINK_MU(DD, MD, 1); break;           \
+        case 2: PLQ_LINK_MU(DD, MD, (2 < DD ? 2 : 0)); break; \
+        default: PLQ_LINK_MU(DD, MD, (3 < DD ? 3 : 0)); break; \
+    }
+    switch (a.g.D * 2 + a.mode) {
+        case 4: PLQ_LINK(2, 0); break;
+        case 5: PLQ_LINK(2, 1); break;
+        case 6: PLQ_LINK(3, 0); break;
+        case 7: PLQ_LINK(3, 1); break;
+        case 8: PLQ_LINK(4, 0); break;
+        default: PLQ_LINK(4, 1); break;
     }
 #undef PLQ_LINK
+#undef PLQ_LINK_MU
 }
 
 void launch_plaquette_sum(uint8_t *const *arr, const Geom &g, long long *partials, int64_t max_blocks,
