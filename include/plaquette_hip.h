@@ -50,7 +50,14 @@ extern "C" {
 
 enum { PLQ_MODEL_ISING = 0, PLQ_MODEL_POTTS = 1, PLQ_MODEL_Z2 = 2 };
 enum { PLQ_KERNEL_METROPOLIS = 0, PLQ_KERNEL_HEAT_BATH = 1 };
-enum { PLQ_PROFILE_F64 = 0, PLQ_PROFILE_F32 = 1 };
+/* PLQ_PROFILE_CPU_STREAM (Ising site chains): the CPU path itself -- its f64
+ * arithmetic and its own sequential stream, rand_pcg Lcg128Xsl64 seeded by
+ * rand_core seed_from_u64(seed) (rng.rs:30-50), consumed in the CPU's site
+ * order (colour 0's sites in lattice order, then colour 1's) with one draw
+ * per uphill Metropolis site or per heat-bath site (updater.rs:77-167,
+ * 316-324).  The draws are not keyed by the sweep, so a chain continues from
+ * its stream state (plq_chain_pcg_state / plq_chain_set_pcg_state). */
+enum { PLQ_PROFILE_F64 = 0, PLQ_PROFILE_F32 = 1, PLQ_PROFILE_CPU_STREAM = 2 };
 enum { PLQ_EXTENT_ALL = 0, PLQ_EXTENT_SEEDED = 1 };
 enum { PLQ_RELABEL_REDRAW = 0, PLQ_RELABEL_FORCED_CHANGE = 1 };
 
@@ -152,6 +159,11 @@ int plq_chain_cluster_stats(plq_chain *chain, plq_cluster_stats *out);
  * used to force PLQ_ENOCONV from the host. */
 int plq_chain_set_pass_cap(plq_chain *chain, uint64_t cap);
 int plq_chain_destroy(plq_chain *chain);
+/* PLQ_PROFILE_CPU_STREAM chains only: the stream state as four u64
+ * (state lo, state hi, increment lo, increment hi) -- the two u128 fields of
+ * rand_pcg's Lcg128Xsl64 -- read, or replaced to continue a CPU run's stream. */
+int plq_chain_pcg_state(plq_chain *chain, uint64_t *out4);
+int plq_chain_set_pcg_state(plq_chain *chain, const uint64_t *in4);
 
 /* ---------------------------------------------------------------------------
  * Batched Z2 gauge ensembles (no reference counterpart: the reference builds

@@ -60,6 +60,7 @@ class Profile(enum.IntEnum):
     """Acceptance arithmetic the integer thresholds reproduce."""
     F64 = 0   # the reference CPU path (updater.rs:77-167)
     F32 = 1   # the reference WGSL kernels (metropolis.wgsl, heat_bath.wgsl)
+    CPU_STREAM = 2  # the CPU path's arithmetic and its own PCG64 stream (rng.rs:30-50), Ising only
 
 
 class Extent(enum.IntEnum):
@@ -178,6 +179,8 @@ SIGNATURES = {
     "plq_chain_cluster_stats": (C.c_int, [_HANDLE, _P(ClusterStats)]),
     "plq_chain_set_pass_cap": (C.c_int, [_HANDLE, C.c_uint64]),
     "plq_chain_destroy": (C.c_int, [_HANDLE]),
+    "plq_chain_pcg_state": (C.c_int, [_HANDLE, _P(C.c_uint64)]),
+    "plq_chain_set_pcg_state": (C.c_int, [_HANDLE, _P(C.c_uint64)]),
     "plq_ensemble_create": (C.c_int, [_P(EnsembleDesc), _P(C.c_uint8), C.c_size_t, _P(_HANDLE)]),
     "plq_ensemble_advance": (C.c_int, [_HANDLE, C.c_uint64]),
     "plq_ensemble_read": (C.c_int, [_HANDLE, _P(C.c_uint8), C.c_size_t]),
@@ -361,6 +364,19 @@ class _Chain:
 
 class GpuChain(_Chain):
     """GpuChain (device.rs:590-701): a checkerboard chain on one device."""
+
+    # -- the CPU stream (Profile.CPU_STREAM) --
+
+    def pcg_state(self):
+        """(state, increment) of the chain's Lcg128Xsl64 stream as two ints."""
+        v = (C.c_uint64 * 4)()
+        _check(lib().plq_chain_pcg_state(self._h, v))
+        return v[0] | (v[1] << 64), v[2] | (v[3] << 64)
+
+    def set_pcg_state(self, state: int, increment: int) -> None:
+        m = (1 << 64) - 1
+        v = (C.c_uint64 * 4)(state & m, (state >> 64) & m, increment & m, (increment >> 64) & m)
+        _check(lib().plq_chain_set_pcg_state(self._h, v))
 
     # -- learned block proposal (2D Ising, extents multiples of 16) --
 

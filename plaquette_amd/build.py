@@ -27,6 +27,7 @@ SOURCES = [
     "plq_ensemble.hip",
     "plq_ensemble_abi.hip",
     "plq_small.hip",
+    "plq_pmode.hip",
     "plq_block.hip",
     "plq_block_abi.hip",
     "plq_thresholds.cpp",

@@ -116,7 +116,7 @@ int upload_table(plq_chain *c) {
     tp.model = c->model;
     tp.q = c->q;
     tp.kernel = c->kernel;
-    tp.profile = c->profile;
+    tp.profile = c->profile == PLQ_PROFILE_CPU_STREAM ? PLQ_PROFILE_F64 : c->profile;
     tp.beta = c->beta;
     tp.j = c->j;
     tp.h = c->h;
@@ -137,6 +137,32 @@ int alloc_scratch(plq_chain *c) {
     PLQ_TRY(alloc((void **)&c->d_partials, sizeof(long long) * 2 * plq::MAX_PASS_BLOCKS * plq::REC_K));
     PLQ_TRY(alloc((void **)&c->d_meas, sizeof(long long) * plq::REC_K));
     return PLQ_OK;
+}
+
+// The CPU stream: 53-bit class thresholds, the seeded stream state and its
+// jump table.
+int pmode_upload_state(plq_chain *c, const uint64_t st[4]) {
+    std::vector<uint64_t> jt;
+    plq::pm_jump_table(st, jt);
+    PLQ_HIP(hipMemcpy(c->d_pm_state, st, 4 * sizeof(uint64_t), hipMemcpyHostToDevice));
+    PLQ_HIP(hipMemcpy(c->d_pm_jt, jt.data(), jt.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    return PLQ_OK;
+}
+
+int pmode_setup(plq_chain *c, uint64_t seed) {
+    std::vector<uint64_t> cls;
+    std::string err;
+    if (!plq::pm_class_table(c->D, c->kernel, c->beta, c->j, c->h, cls, err)) return fail(PLQ_EINVAL, err);
+    const int64_t nb = plq::pm_blocks(c->g);
+    PLQ_TRY(alloc((void **)&c->d_pm_cls, cls.size() * sizeof(uint64_t)));
+    PLQ_TRY(alloc((void **)&c->d_pm_state, 4 * sizeof(uint64_t)));
+    PLQ_TRY(alloc((void **)&c->d_pm_jt, 4 * 64 * sizeof(uint64_t)));
+    PLQ_TRY(alloc((void **)&c->d_pm_blk, (size_t)nb * sizeof(uint32_t)));
+    PLQ_TRY(alloc((void **)&c->d_pm_off, (size_t)(nb + 1) * sizeof(uint32_t)));
+    PLQ_HIP(hipMemcpy(c->d_pm_cls, cls.data(), cls.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    uint64_t st[4];
+    plq::pm_seed(seed, st);
+    return pmode_upload_state(c, st);
 }
 
 // ---------------------------------------------------------- local sweeps ---
@@ -200,7 +226,25 @@ int enqueue_local_sweep(plq_chain *c) {
     uint32_t sweep = (uint32_t)c->sweeps_done;
     bool record = c->rec_every && ((c->sweeps_done + 1) % c->rec_every == 0);
     int mode = c->kernel == PLQ_KERNEL_METROPOLIS ? 0 : 1;
-    if (c->kind == KIND_SITE) {
+    if (c->kind == KIND_SITE && c->pmode) {
+        for (int color = 0; color < 2; ++color) {
+            plq::PmodeArgs a{};
+            a.own = c->col[color];
+            a.oth = c->col[1 - color];
+            a.g = c->g;
+            a.color = color;
+            a.mode = mode;
+            a.cls = c->d_pm_cls;
+            a.blk = c->d_pm_blk;
+            a.off = c->d_pm_off;
+            a.state = c->d_pm_state;
+            a.jt = c->d_pm_jt;
+            a.stream = c->stream;
+            plq::launch_pm_pass(a);
+        }
+        c->sweeps_done += 1;
+        if (record) enqueue_site_measure(c, rec_slot(c));
+    } else if (c->kind == KIND_SITE) {
         c->sweeps_done += 1;
         enqueue_site_passes(c, sweep, mode, record ? rec_slot(c) : nullptr);
     } else {
@@ -381,6 +425,8 @@ int plq_device_count(int32_t *count) {
 
 int plq_threshold_table(const plq_local_desc *d, uint32_t *out, size_t cap) {
     if (!d) return fail(PLQ_EINVAL, "descriptor is null");
+    if (d->profile == PLQ_PROFILE_CPU_STREAM)
+        return fail(PLQ_EINVAL, "the CPU-stream profile compares 53-bit draws, not against a 24-bit table");
     plq::TableParams tp{};
     tp.D = d->dim;
     tp.model = d->model;
@@ -407,7 +453,9 @@ int plq_chain_create_local(const plq_local_desc *d, const uint8_t *start, size_t
     bool link = d->model == PLQ_MODEL_Z2;
     if (d->model < 0 || d->model > 2) return fail(PLQ_EINVAL, "unknown model");
     if (d->kernel < 0 || d->kernel > 1) return fail(PLQ_EINVAL, "unknown kernel");
-    if (d->profile < 0 || d->profile > 1) return fail(PLQ_EINVAL, "unknown threshold profile");
+    if (d->profile < 0 || d->profile > 2) return fail(PLQ_EINVAL, "unknown threshold profile");
+    if (d->profile == PLQ_PROFILE_CPU_STREAM && d->model != PLQ_MODEL_ISING)
+        return fail(PLQ_EINVAL, "the CPU-stream profile is implemented for Ising site chains");
     if (d->dim < (link ? 2 : 1) || d->dim > plq::MAX_D)
         return fail(PLQ_EINVAL, link ? "the Z2 gauge device path supports dimensions 2..4"
                                      : "the site device path supports dimensions 1..4");
@@ -463,7 +511,8 @@ int plq_chain_create_local(const plq_local_desc *d, const uint8_t *start, size_t
     // Small power-of-two Ising lattices run many sweeps per launch in one
     // workgroup; PLQ_SMALL_CHAIN=0 keeps them on the per-pass kernels (tests
     // use it to check that both paths produce identical bits).
-    c->small = c->kind == KIND_SITE && d->model == PLQ_MODEL_ISING && all_pow2 &&
+    c->pmode = d->profile == PLQ_PROFILE_CPU_STREAM;
+    c->small = c->kind == KIND_SITE && d->model == PLQ_MODEL_ISING && all_pow2 && !c->pmode &&
                ns <= plq::small_chain_max_sites(d->dim);
     if (const char *sm = getenv("PLQ_SMALL_CHAIN"))
         if (atoi(sm) == 0) c->small = false;
@@ -491,6 +540,7 @@ int plq_chain_create_local(const plq_local_desc *d, const uint8_t *start, size_t
         return bail(fail(PLQ_EDEVICE, "hipStreamCreate failed"));
     if ((rc = upload_table(c)) != PLQ_OK) return bail(rc);
     if ((rc = alloc_scratch(c)) != PLQ_OK) return bail(rc);
+    if (c->pmode && (rc = pmode_setup(c, d->seed)) != PLQ_OK) return bail(rc);
     size_t half = (size_t)(ns / 2) + 16;
     if (c->kind == KIND_SITE) {
         for (int k = 0; k < 2; ++k)
@@ -575,6 +625,26 @@ int plq_chain_create_cluster(const plq_cluster_desc *d, const uint8_t *start, si
     if ((rc = write_lattice(c, start, n)) != PLQ_OK) return bail(rc);
     *out = c;
     return PLQ_OK;
+}
+
+int plq_chain_pcg_state(plq_chain *c, uint64_t *out4) {
+    if (!c || !out4) return fail(PLQ_EINVAL, "chain or buffer is null");
+    if (!c->pmode) return fail(PLQ_EINVAL, "only PLQ_PROFILE_CPU_STREAM chains have a stream state");
+    std::lock_guard<std::mutex> lk(c->mu);
+    PLQ_HIP(hipSetDevice(c->device));
+    PLQ_HIP(hipMemcpyAsync(out4, c->d_pm_state, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    PLQ_HIP(hipStreamSynchronize(c->stream));
+    return PLQ_OK;
+}
+
+int plq_chain_set_pcg_state(plq_chain *c, const uint64_t *in4) {
+    if (!c || !in4) return fail(PLQ_EINVAL, "chain or buffer is null");
+    if (!c->pmode) return fail(PLQ_EINVAL, "only PLQ_PROFILE_CPU_STREAM chains have a stream state");
+    if ((in4[2] & 1u) == 0) return fail(PLQ_EINVAL, "the increment of an Lcg128Xsl64 stream is odd");
+    std::lock_guard<std::mutex> lk(c->mu);
+    PLQ_HIP(hipSetDevice(c->device));
+    PLQ_HIP(hipStreamSynchronize(c->stream));
+    return pmode_upload_state(c, in4);
 }
 
 int plq_chain_destroy(plq_chain *c) {

@@ -61,6 +61,12 @@ struct plq_chain {
     uint64_t rec_every = 0, rec_cap = 0, rec_count = 0;
     std::vector<uint64_t> rec_sweeps;
     std::vector<hipEvent_t> pending;
+    // the CPU stream (PLQ_PROFILE_CPU_STREAM, plq_pmode.hip)
+    bool pmode = false;
+    uint64_t *d_pm_cls = nullptr;    // per class threshold
+    uint64_t *d_pm_state = nullptr;  // state lo, hi, increment lo, hi
+    uint64_t *d_pm_jt = nullptr;     // jump table
+    uint32_t *d_pm_blk = nullptr, *d_pm_off = nullptr;
     // learned block proposal (plq_block_abi.hip)
     uint64_t block_sweeps = 0;                       // key of the block passes' draws
     unsigned long long *d_block_counters = nullptr;  // accepted moves, one slot per pass workgroup
@@ -73,7 +79,8 @@ struct plq_chain {
                        (void *)parent, (void *)roots, (void *)sizes, (void *)hist, (void *)d_flag, (void *)d_lat,
                        (void *)d_partials, (void *)d_meas, (void *)d_rec, (void *)d_block_counters, (void *)rootlist,
                        (void *)d_nroots, (void *)newlab,
-                       (void *)d_block_logits})
+                       (void *)d_block_logits, (void *)d_pm_cls, (void *)d_pm_state, (void *)d_pm_jt, (void *)d_pm_blk,
+                       (void *)d_pm_off})
             if (p) (void)hipFree(p);
         for (int k = 0; k < 2 * plq::MAX_D; ++k)
             if (links[k]) (void)hipFree(links[k]);

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "plq_common.hpp"
 
@@ -136,6 +137,28 @@ int block_train_groups(int L0, int L1);
 void launch_block_grad(const BlockGradArgs &a);
 void launch_block_adam(float *w, float *m, float *v, const float *grad, float lr, float beta1, float beta2,
                        float c1, float c2, float eps, hipStream_t s);
+
+// ------------------------------------------- the CPU stream (plq_pmode.hip) ---
+
+struct PmodeArgs {
+    uint8_t *own;            // the colour being updated
+    const uint8_t *oth;
+    Geom g;
+    int color;
+    int mode;                // 0 Metropolis, 1 heat bath
+    const uint64_t *cls;     // per class (state, down neighbours): 53-bit threshold, or ~0 (no draw)
+    uint32_t *blk, *off;     // pm_blocks entries, pm_blocks + 1 entries
+    uint64_t *state;         // [state lo, state hi, increment lo, increment hi]
+    const uint64_t *jt;      // 64 x (A lo, A hi, C lo, C hi): k = 2^b steps
+    hipStream_t stream;
+};
+// rand_core seed_from_u64 + rand_pcg Lcg128Xsl64 (state lo, hi, increment lo, hi).
+void pm_seed(uint64_t seed, uint64_t out[4]);
+void pm_jump_table(const uint64_t state[4], std::vector<uint64_t> &jt);
+bool pm_class_table(int D, int kernel, double beta, double j, double h, std::vector<uint64_t> &cls,
+                    std::string &err);
+int64_t pm_blocks(const Geom &g);
+void launch_pm_pass(const PmodeArgs &a);
 
 // ------------------------------------------------------------------ links ---
 

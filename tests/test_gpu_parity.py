@@ -334,3 +334,57 @@ def test_ising2d_energy_and_specific_heat_match_kaufman(gpu, L, beta, kernel):
     cv = S.specific_heat(e, beta, n)
     want_c = X.specific_heat(L, L, beta)
     assert abs(cv.value - want_c) < 4.5 * cv.stderr, (cv.value, want_c, cv.stderr)
+
+
+# -- the CPU stream: the reference CPU path's own PCG64 draws ---------------------
+
+PMODE_CASES = [
+    # shape, kernel, beta, j, h, sweeps (in advances)
+    ([16, 16], M, 0.4406868, 1.0, 0.0, (1, 4, 15)),
+    ([16, 16], HB, 0.4406868, 1.0, 0.1, (1, 4, 15)),
+    ([6, 10], M, 0.3, 1.0, -0.2, (2, 8)),
+    ([4, 6, 8], HB, 0.25, 0.8, 0.05, (3, 7)),
+    ([8, 4, 4, 2], M, 0.15, 1.0, 0.0, (5, 5)),
+    ([16], M, 0.8, 1.0, 0.1, (10, 10)),
+    ([64, 64, 32], M, 0.2216546, 1.0, 0.0, (1, 2)),   # 131,072 sites: several blocks per pass
+    ([128, 128], HB, 0.4406868, 1.0, 0.0, (2, 2)),
+]
+
+
+@pytest.mark.parametrize("case", PMODE_CASES, ids=[f"{'x'.join(map(str, c[0]))}-{c[1]}" for c in PMODE_CASES])
+def test_cpu_stream_profile_reproduces_the_cpu_path(gpu, case):
+    # The CPU path as written (updater.rs:77-167, 316-324; RandRng = PCG64
+    # seed_from_u64, rng.rs:30-50), restated sequentially by the oracle, and
+    # the device's scan + jump-ahead reproduction of the same stream.
+    shape, kernel, beta, j, h, advances = case
+    n = int(np.prod(shape))
+    start = start_config(n, 2, True, 61)
+    seed = 20260728
+    c = gpu.ising_gpu_chain(shape, j, h, beta, seed, start, kernel=kernel, profile=gpu.Profile.CPU_STREAM)
+    g = O.pcg64(seed)
+    assert c.pcg_state() == (g.state_lo | (g.state_hi << 64), g.inc_lo | (g.inc_hi << 64))
+    done = 0
+    for k in advances:
+        c.advance(k)
+        done += k
+        st, _ = O.pmode_ising_sweeps(shape, kernel, beta, j, h, seed, start, done)
+        got = c.read()
+        assert np.array_equal(got, st), f"{(got != st).sum()} sites differ"
+    c.close()
+
+
+def test_cpu_stream_state_continues_in_a_new_chain(gpu):
+    shape, beta = [32, 32], 0.4406868
+    n = 1024
+    start = start_config(n, 2, True, 62)
+    a = gpu.ising_gpu_chain(shape, 1.0, 0.0, beta, 7, start, profile=gpu.Profile.CPU_STREAM)
+    a.advance(5)
+    mid, state = a.read(), a.pcg_state()
+    a.advance(6)
+    b = gpu.ising_gpu_chain(shape, 1.0, 0.0, beta, 999, mid, profile=gpu.Profile.CPU_STREAM)
+    b.set_pcg_state(*state)
+    b.advance(6)
+    assert np.array_equal(a.read(), b.read())
+    assert a.pcg_state() == b.pcg_state()
+    a.close()
+    b.close()
