@@ -310,6 +310,29 @@ def bench_ising2d_64(P, ev, steps, warmup):
 BLOCK_FLOP = 1253376  # network FLOPs per 8x8 block: 64 x 32 x 9 x 2 (conv1) + 64 x 32 x 288 x 2 (conv2) + 64 x 288 x 2 (conv3)
 
 
+def bench_cpu_stream(P, ev, steps, warmup):
+    """3D Ising 256^3 at beta = 0.2216546, checkerboard Metropolis with the
+    CPU path's own PCG64 stream (Profile.CPU_STREAM): the same configurations
+    as the reference CPU run with the same seed, on the device."""
+    import numpy as np
+
+    n = int(np.prod(SHAPE))
+    c = P.ising_gpu_chain(SHAPE, 1.0, 0.0, BETA, SEED, np.zeros(n, np.uint8), profile=P.Profile.CPU_STREAM)
+    c.advance(warmup)
+    c.sync()
+    wall, ms = timed(P, ev, c.stream, lambda: (c.advance(steps), c.sync()))
+    c.close()
+    rate = steps * n / (wall * 1e9)
+    return {
+        "metric": "site-updates/ns, reference CPU stream reproduced", "value": round(rate, 3),
+        "unit": "site-updates/ns", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": round(wall * 1e3 / steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": "3D Ising 256^3, checkerboard Metropolis, beta=0.2216546, cold start, PCG64 "
+                               "seed_from_u64 stream in CPU site order", "lattice": SHAPE},
+    }
+
+
 def learned_small(P, ev, stats):
     """The same comparison on 64^2 at beta_c, where tau_int of |M| (about 150
     sweeps) converges within 100,000 steps: train 300 steps on the chain's
@@ -421,7 +444,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--profile", action="store_true",
                     help="run the fixed counter-collection workload instead of the benchmark")
-    ap.add_argument("--workload", default="ising3d", choices=["ising3d", "potts-sw", "z2-batch", "ising2d-64", "learned"],
+    ap.add_argument("--workload", default="ising3d", choices=["ising3d", "potts-sw", "z2-batch", "ising2d-64", "learned", "cpu-stream"],
                     help="ising3d is the headline line; the others time the remaining configurations")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
@@ -451,6 +474,9 @@ def main():
         return
     if args.workload == "z2-batch":
         print(json.dumps(bench_z2_batch(P, ev, 500, 20)))
+        return
+    if args.workload == "cpu-stream":
+        print(json.dumps(bench_cpu_stream(P, ev, 50, 5)))
         return
     if args.workload == "learned":
         print(json.dumps(bench_learned(P, ev)))
