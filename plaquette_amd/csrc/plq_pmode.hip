@@ -49,29 +49,52 @@ __device__ __forceinline__ uint64_t xsl_rr(u128 s) {
     return (x >> rot) | (x << ((64u - rot) & 63u));
 }
 
-// The class of site w of colour c: (current state, down neighbours).
+// A lane's walk over consecutive entries of a colour array: row r, position
+// i in the row and the row's coordinates, advanced without divisions.
+struct PmCursor {
+    int32_t r, i;
+    int32_t x[MAX_D];
+};
+
+__device__ __forceinline__ PmCursor pm_cursor(const Geom &g, int64_t w) {
+    PmCursor k;
+    k.r = (int32_t)(w / g.H);
+    k.i = (int32_t)(w - (int64_t)k.r * g.H);
+    row_coords(g, k.r, k.x);
+    return k;
+}
+
+template <int D>
+__device__ __forceinline__ void pm_next(const Geom &g, PmCursor &k) {
+    if (++k.i < g.H) return;
+    k.i = 0;
+    ++k.r;
+#pragma unroll
+    for (int mu = 1; mu < D; ++mu) {
+        if (++k.x[mu] < g.L[mu]) return;
+        k.x[mu] = 0;
+    }
+}
+
+// The class of the cursor's site of colour c: (current state, down neighbours).
 template <int D>
 __device__ __forceinline__ uint32_t site_class(const uint8_t *__restrict__ own, const uint8_t *__restrict__ oth,
-                                               const Geom &g, int c, int64_t w, uint32_t *site_out) {
-    const int32_t r = (int32_t)(w / g.H);
-    const int32_t i = (int32_t)(w - (int64_t)r * g.H);
-    int32_t x[MAX_D];
-    row_coords(g, r, x);
+                                               const Geom &g, int c, const PmCursor &k) {
     int32_t prs = 0;
-    for (int mu = 1; mu < D; ++mu) prs += x[mu];
-    const int32_t x0 = 2 * i + ((prs + c) & 1);
+#pragma unroll
+    for (int mu = 1; mu < D; ++mu) prs += k.x[mu];
+    const int32_t x0 = 2 * k.i + ((prs + c) & 1);
     const int32_t L0 = g.L[0];
-    const int64_t rowbase = (int64_t)r * g.H;
+    const int64_t rowbase = (int64_t)k.r * g.H;
     const int32_t xp = (x0 + 1 == L0) ? 0 : x0 + 1;
     const int32_t xm = (x0 == 0) ? L0 - 1 : x0 - 1;
     uint32_t d = oth[rowbase + (xp >> 1)] + oth[rowbase + (xm >> 1)];
 #pragma unroll
     for (int mu = 1; mu < D; ++mu) {
-        d += oth[(int64_t)row_step(g, r, x, mu, +1) * g.H + i];
-        d += oth[(int64_t)row_step(g, r, x, mu, -1) * g.H + i];
+        d += oth[(int64_t)row_step(g, k.r, k.x, mu, +1) * g.H + k.i];
+        d += oth[(int64_t)row_step(g, k.r, k.x, mu, -1) * g.H + k.i];
     }
-    *site_out = (uint32_t)((int64_t)r * L0 + x0);
-    return (uint32_t)own[w] * (2 * D + 1) + d;
+    return (uint32_t)own[rowbase + k.i] * (2 * D + 1) + d;
 }
 
 template <int D>
@@ -81,18 +104,20 @@ __global__ __launch_bounds__(PM_BLOCK) void pm_count(const uint8_t *__restrict__
     __shared__ uint32_t wsum[PM_BLOCK / 64];
     const int64_t w0 = ((int64_t)blockIdx.x * PM_BLOCK + threadIdx.x) * PM_RUN;
     uint32_t n = 0;
-    for (int k = 0; k < PM_RUN; ++k) {
-        const int64_t w = w0 + k;
-        if (w >= work) break;
-        uint32_t site;
-        n += cls[site_class<D>(own, oth, g, c, w, &site)] != NO_DRAW ? 1u : 0u;
+    if (w0 < work) {
+        PmCursor k = pm_cursor(g, w0);
+        for (int q = 0; q < PM_RUN; ++q) {
+            if (w0 + q >= work) break;
+            n += cls[site_class<D>(own, oth, g, c, k)] != NO_DRAW ? 1u : 0u;
+            pm_next<D>(g, k);
+        }
     }
     for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = n;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t t = 0;
-        for (int k = 0; k < PM_BLOCK / 64; ++k) t += wsum[k];
+        for (int q = 0; q < PM_BLOCK / 64; ++q) t += wsum[q];
         blk[blockIdx.x] = t;
     }
 }
@@ -134,31 +159,39 @@ __global__ __launch_bounds__(PM_BLOCK) void pm_update(uint8_t *__restrict__ own,
                                                      const uint32_t *__restrict__ off,
                                                      const uint64_t *__restrict__ state,
                                                      const uint64_t *__restrict__ jt, int64_t work) {
-    __shared__ uint32_t lane_n[PM_BLOCK];
+    __shared__ uint32_t wave_n[PM_BLOCK / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t w0 = ((int64_t)blockIdx.x * PM_BLOCK + threadIdx.x) * PM_RUN;
     uint32_t cl[PM_RUN];
     uint32_t n = 0;
+    PmCursor k{};
+    if (w0 < work) k = pm_cursor(g, w0);
 #pragma unroll
-    for (int k = 0; k < PM_RUN; ++k) {
-        const int64_t w = w0 + k;
-        cl[k] = 0xffffffffu;
-        if (w < work) {
-            uint32_t site;
-            cl[k] = site_class<D>(own, oth, g, c, w, &site);
-            n += cls[cl[k]] != NO_DRAW ? 1u : 0u;
+    for (int q = 0; q < PM_RUN; ++q) {
+        cl[q] = 0xffffffffu;
+        if (w0 + q < work) {
+            cl[q] = site_class<D>(own, oth, g, c, k);
+            n += cls[cl[q]] != NO_DRAW ? 1u : 0u;
+            pm_next<D>(g, k);
         }
     }
-    lane_n[threadIdx.x] = n;
+    // draws before this lane within the block: a wave scan, then the waves before
+    uint32_t incl = n;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += t;
+    }
+    if (lane == 63) wave_n[wv] = incl;
     __syncthreads();
-    uint64_t before = off[blockIdx.x];
-    for (int l = 0; l < (int)threadIdx.x; ++l) before += lane_n[l];
+    uint64_t before = off[blockIdx.x] + (incl - n);
+    for (int q = 0; q < wv; ++q) before += wave_n[q];
     const u128 mul = make128(jt[0], jt[1]), inc = make128(jt[2], jt[3]);
     u128 s = jump(make128(state[0], state[1]), before, jt);
 #pragma unroll
-    for (int k = 0; k < PM_RUN; ++k) {
-        if (cl[k] == 0xffffffffu) continue;
-        const uint64_t t = cls[cl[k]];
-        const int64_t w = w0 + k;
+    for (int q = 0; q < PM_RUN; ++q) {
+        if (cl[q] == 0xffffffffu) continue;
+        const uint64_t t = cls[cl[q]];
+        const int64_t w = w0 + q;
         const uint32_t cur = own[w];
         uint32_t nxt;
         if (t == NO_DRAW) {
