@@ -166,6 +166,29 @@ int plq_chain_pcg_state(plq_chain *chain, uint64_t *out4);
 int plq_chain_set_pcg_state(plq_chain *chain, const uint64_t *in4);
 
 /* ---------------------------------------------------------------------------
+ * Per-configuration measurements, computed where the configuration lives (the
+ * reference reads the configuration back and walks it on the CPU).  The sums
+ * are exact integers and the f64 results use the reference's own final
+ * division, so every value equals the reference's bit for bit.
+ * ------------------------------------------------------------------------- */
+/* Two-point correlator of a site chain along every axis: Ising::correlator
+ * (ising/mod.rs:83-94) for model PLQ_MODEL_ISING (q = 2), Potts::correlator
+ * (potts/mod.rs:218-233) for PLQ_MODEL_POTTS, both over axis_pair_sums
+ * (models/mod.rs:44-60).  Entries run axis after axis, r = 0..shape[mu]/2,
+ * so len = sum over mu of (shape[mu]/2 + 1).  `sums` (may be NULL) receives
+ * the raw pair sums (Ising sign products, Potts agreement counts), `values`
+ * (may be NULL) sum / N (Ising) or sum / N - 1/q (Potts).  A local site chain
+ * takes its own model; a cluster chain either (Ising needs q = 2). */
+int plq_chain_correlator(plq_chain *chain, int32_t model, int64_t *sums, double *values, size_t len);
+/* Z2 gauge chains: Z2Gauge::polyakov_loop (gauge/mod.rs:178-201) along dir. */
+int plq_chain_polyakov_loop(plq_chain *chain, int32_t dir, double *value);
+/* Z2 gauge chains: Z2Gauge::wilson_rectangles (gauge/mod.rs:224-268).
+ * side = min(max_side, shortest extent / 2) goes to *side; `table` (NULL to
+ * ask for the side only) receives the (side + 1)^2 entries row-major,
+ * table[r * (side + 1) + t], with len >= (side + 1)^2. */
+int plq_chain_wilson_rectangles(plq_chain *chain, int32_t max_side, double *table, size_t len, int32_t *side);
+
+/* ---------------------------------------------------------------------------
  * Batched Z2 gauge ensembles (no reference counterpart: the reference builds
  * one GpuChain per lattice, gauge/gpu.rs:54-110).  Many replicas of uneven
  * shapes advance together, one launch per colour pass for all of them; every

@@ -138,6 +138,10 @@ def lib():
                                               C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double]),
             "orc_pmode_ising_sweeps": (C.c_double, [C.c_int32, P(C.c_int64), C.c_int32, C.c_double, C.c_double,
                                                     C.c_double, C.c_uint64, P(C.c_uint8), C.c_uint32]),
+            "orc_axis_pair_sums": (None, [C.c_int32, P(C.c_int64), P(C.c_uint8), C.c_int32, P(C.c_int64)]),
+            "orc_polyakov_sum": (C.c_int64, [C.c_int32, P(C.c_int64), P(C.c_uint8), C.c_int32, P(C.c_int64)]),
+            "orc_wilson_sums": (C.c_int32, [C.c_int32, P(C.c_int64), P(C.c_uint8), C.c_int32, P(C.c_int64),
+                                            P(C.c_int64)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -311,6 +315,57 @@ def potts_sums(shape, q, state):
 def plaquette_sum(shape, links) -> int:
     st = np.ascontiguousarray(links, dtype=np.uint8)
     return lib().orc_plaquette_sum(len(shape), _shape(shape), _u8(st))
+
+
+# ------------------------------------------------------- measurements ---
+
+def axis_pair_sums(shape, state, pair_kind: int) -> list:
+    """Raw pair sums per axis (models/mod.rs:44-60); pair_kind 0 Ising, 1 Potts."""
+    st = np.ascontiguousarray(state, dtype=np.uint8)
+    lens = [int(l) // 2 + 1 for l in shape]
+    out = np.zeros(sum(lens), dtype=np.int64)
+    lib().orc_axis_pair_sums(len(shape), _shape(shape), _u8(st), pair_kind, _i64(out))
+    return list(np.split(out, np.cumsum(lens)[:-1]))
+
+
+def ising_correlator(shape, state) -> list:
+    """Ising::correlator (ising/mod.rs:83-94): sum / N per axis and displacement."""
+    n = float(np.prod(shape))
+    return [s.astype(np.float64) / n for s in axis_pair_sums(shape, state, 0)]
+
+
+def potts_correlator(shape, q, state) -> list:
+    """Potts::correlator (potts/mod.rs:218-233): agreements / N - 1/q."""
+    n = float(np.prod(shape))
+    return [s.astype(np.float64) / n - 1.0 / q for s in axis_pair_sums(shape, state, 1)]
+
+
+def polyakov_loop(shape, links, direction: int) -> float:
+    """Z2Gauge::polyakov_loop (gauge/mod.rs:178-201)."""
+    st = np.ascontiguousarray(links, dtype=np.uint8)
+    lines = C.c_int64()
+    total = lib().orc_polyakov_sum(len(shape), _shape(shape), _u8(st), direction, C.byref(lines))
+    return float(total) / float(lines.value)
+
+
+def wilson_sums(shape, links, max_side: int):
+    """(totals, placements) of Z2Gauge::wilson_rectangles (gauge/mod.rs:224-268)."""
+    st = np.ascontiguousarray(links, dtype=np.uint8)
+    shortest = min(int(l) for l in shape)
+    side = min(max_side, shortest // 2)
+    totals = np.zeros((side + 1) * (side + 1), dtype=np.int64)
+    placements = C.c_int64()
+    got = lib().orc_wilson_sums(len(shape), _shape(shape), _u8(st), max_side, _i64(totals), C.byref(placements))
+    assert got == side
+    return totals.reshape(side + 1, side + 1), placements.value
+
+
+def wilson_rectangles(shape, links, max_side: int) -> np.ndarray:
+    """The table itself: 1.0 on row and column 0, totals / placements elsewhere."""
+    totals, placements = wilson_sums(shape, links, max_side)
+    table = np.ones(totals.shape, dtype=np.float64)
+    table[1:, 1:] = totals[1:, 1:].astype(np.float64) / float(placements)
+    return table
 
 
 # ----------------------------------------------------------- statistics ---

@@ -24,6 +24,8 @@
  *                          src/wgsl/cluster_relabel.wgsl:39-90,
  *                          src/updater.rs:447-613, src/cluster.rs:44-227
  *   - observables          src/models/ising/mod.rs:97-186,
+ *                          src/models/mod.rs:44-60 (correlator pair sums),
+ *                          src/models/gauge/mod.rs:178-268 (Polyakov, Wilson),
  *                          src/models/potts/mod.rs:142-181, 237-365,
  *                          src/models/gauge/mod.rs:99-122
  *   - statistics (FP64)    src/statistics.rs:83-317
@@ -183,6 +185,18 @@ double orc_blocknet_loss_grad(const float *params, int round, int64_t L0, int64_
 /* One pass over the blocks of `colour`; returns the number accepted. */
 int64_t orc_block_mh_pass(int64_t L0, int64_t L1, uint8_t *state, const float *logits, int colour, uint32_t seed32,
                           uint32_t bstep, double beta, double j, double h);
+
+/* ---- per-configuration measurements (plq_oracle_measure.c) -----------------
+ * Correlator pair sums (models/mod.rs:44-60): out holds, axis after axis,
+ * shape[mu] / 2 + 1 sums each; pair_kind 0 = Ising sign product, 1 = Potts
+ * agreement.  Polyakov loop (gauge/mod.rs:178-201): returns the sum of the line
+ * products, *lines the line count.  Wilson rectangles (gauge/mod.rs:224-268):
+ * returns max = min(max_side, shortest / 2); totals is (max + 1)^2, row r,
+ * column t, the sum over orientations and sites for r, t >= 1. */
+void orc_axis_pair_sums(int32_t dim, const int64_t *shape, const uint8_t *state, int32_t pair_kind, int64_t *out);
+int64_t orc_polyakov_sum(int32_t dim, const int64_t *shape, const uint8_t *links, int32_t dir, int64_t *lines);
+int32_t orc_wilson_sums(int32_t dim, const int64_t *shape, const uint8_t *links, int32_t max_side, int64_t *totals,
+                        int64_t *placements);
 
 #ifdef __cplusplus
 }

@@ -181,6 +181,9 @@ SIGNATURES = {
     "plq_chain_destroy": (C.c_int, [_HANDLE]),
     "plq_chain_pcg_state": (C.c_int, [_HANDLE, _P(C.c_uint64)]),
     "plq_chain_set_pcg_state": (C.c_int, [_HANDLE, _P(C.c_uint64)]),
+    "plq_chain_correlator": (C.c_int, [_HANDLE, C.c_int32, _P(C.c_int64), _P(C.c_double), C.c_size_t]),
+    "plq_chain_polyakov_loop": (C.c_int, [_HANDLE, C.c_int32, _P(C.c_double)]),
+    "plq_chain_wilson_rectangles": (C.c_int, [_HANDLE, C.c_int32, _P(C.c_double), C.c_size_t, _P(C.c_int32)]),
     "plq_ensemble_create": (C.c_int, [_P(EnsembleDesc), _P(C.c_uint8), C.c_size_t, _P(_HANDLE)]),
     "plq_ensemble_advance": (C.c_int, [_HANDLE, C.c_uint64]),
     "plq_ensemble_read": (C.c_int, [_HANDLE, _P(C.c_uint8), C.c_size_t]),
@@ -288,6 +291,8 @@ class _Chain:
         self.n_vars = n_vars
         self.n_sites = n_sites
         self.sweeps_between = sweeps_between
+        self.shape = None   # extents, set by the constructors
+        self.model = None   # Model of a local chain; None for a cluster chain
 
     # -- lifecycle ----------------------------------------------------------
     def close(self):
@@ -361,6 +366,45 @@ class _Chain:
         _check(lib().plq_chain_stream(self._h, C.byref(p)))
         return p.value or 0
 
+    # -- per-configuration measurements on the device --------------------------
+    def _correlator(self, model):
+        if model is None:
+            model = self.model
+        if model is None:
+            raise ValueError("a cluster chain needs the correlator convention: Model.ISING or Model.POTTS")
+        lens = [int(l) // 2 + 1 for l in self.shape]
+        sums = np.zeros(sum(lens), np.int64)
+        vals = np.zeros(sum(lens), np.float64)
+        _check(lib().plq_chain_correlator(self._h, int(model), sums.ctypes.data_as(_P(C.c_int64)),
+                                          vals.ctypes.data_as(_P(C.c_double)), sums.size))
+        cut = np.cumsum(lens)[:-1]
+        return list(np.split(sums, cut)), list(np.split(vals, cut))
+
+    def correlator(self, model=None) -> list:
+        """Ising::correlator / Potts::correlator (ising/mod.rs:83-94,
+        potts/mod.rs:218-233): one f64 array per axis, r = 0..L_mu/2."""
+        return self._correlator(model)[1]
+
+    def correlator_sums(self, model=None) -> list:
+        """The raw pair sums behind the correlator (models/mod.rs:44-60)."""
+        return self._correlator(model)[0]
+
+    def polyakov_loop(self, direction: int) -> float:
+        """Z2Gauge::polyakov_loop (gauge/mod.rs:178-201)."""
+        v = C.c_double(0.0)
+        _check(lib().plq_chain_polyakov_loop(self._h, int(direction), C.byref(v)))
+        return v.value
+
+    def wilson_rectangles(self, max_side: int) -> np.ndarray:
+        """Z2Gauge::wilson_rectangles (gauge/mod.rs:224-268) as a (side+1) x (side+1) array."""
+        side = C.c_int32(0)
+        _check(lib().plq_chain_wilson_rectangles(self._h, int(max_side), None, 0, C.byref(side)))
+        n = side.value + 1
+        table = np.zeros(n * n, np.float64)
+        _check(lib().plq_chain_wilson_rectangles(self._h, int(max_side), table.ctypes.data_as(_P(C.c_double)),
+                                                 table.size, C.byref(side)))
+        return table.reshape(n, n)
+
 
 class GpuChain(_Chain):
     """GpuChain (device.rs:590-701): a checkerboard chain on one device."""
@@ -403,8 +447,11 @@ class GpuChain(_Chain):
         st = np.ascontiguousarray(start, dtype=np.uint8).ravel()
         h = C.c_void_p(0)
         _check(lib().plq_chain_create_local(C.byref(desc), st.ctypes.data_as(_P(C.c_uint8)), st.size, C.byref(h)))
-        n_sites = int(np.prod([desc.shape[i] for i in range(desc.dim)]))
-        return cls(h, st.size, n_sites, desc.sweeps_between)
+        shape = [int(desc.shape[i]) for i in range(desc.dim)]
+        chain = cls(h, st.size, int(np.prod(shape)), desc.sweeps_between)
+        chain.shape = shape
+        chain.model = Model(desc.model)
+        return chain
 
 
 class BlockNet:
@@ -480,7 +527,9 @@ class GpuClusterChain(_Chain):
         st = np.ascontiguousarray(start, dtype=np.uint8).ravel()
         h = C.c_void_p(0)
         _check(lib().plq_chain_create_cluster(C.byref(d), st.ctypes.data_as(_P(C.c_uint8)), st.size, C.byref(h)))
-        return cls(h, st.size, st.size, int(sweeps_between))
+        chain = cls(h, st.size, st.size, int(sweeps_between))
+        chain.shape = [int(l) for l in shape]
+        return chain
 
     @classmethod
     def swendsen_wang(cls, shape, q, gap, beta, seed, start, sweeps_between=1, profile=Profile.F64):

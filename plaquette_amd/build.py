@@ -30,6 +30,8 @@ SOURCES = [
     "plq_pmode.hip",
     "plq_block.hip",
     "plq_block_abi.hip",
+    "plq_measure.hip",
+    "plq_measure_abi.hip",
     "plq_thresholds.cpp",
 ]
 

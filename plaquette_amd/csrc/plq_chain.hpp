@@ -71,6 +71,11 @@ struct plq_chain {
     uint64_t block_sweeps = 0;                       // key of the block passes' draws
     unsigned long long *d_block_counters = nullptr;  // accepted moves, one slot per pass workgroup
     float *d_block_logits = nullptr;                 // logits of each block's last pass
+    // per-configuration measurements (plq_measure_abi.hip)
+    uint32_t *d_meas_bits = nullptr;           // bit planes (and Wilson line planes)
+    size_t meas_bits_cap = 0;                  // words
+    unsigned long long *d_meas_counts = nullptr;
+    size_t meas_counts_cap = 0;
 
     ~plq_chain() {
         (void)hipSetDevice(device);
@@ -80,7 +85,7 @@ struct plq_chain {
                        (void *)d_partials, (void *)d_meas, (void *)d_rec, (void *)d_block_counters, (void *)rootlist,
                        (void *)d_nroots, (void *)newlab,
                        (void *)d_block_logits, (void *)d_pm_cls, (void *)d_pm_state, (void *)d_pm_jt, (void *)d_pm_blk,
-                       (void *)d_pm_off})
+                       (void *)d_pm_off, (void *)d_meas_bits, (void *)d_meas_counts})
             if (p) (void)hipFree(p);
         for (int k = 0; k < 2 * plq::MAX_D; ++k)
             if (links[k]) (void)hipFree(links[k]);

@@ -654,7 +654,6 @@ int64_t launch_cluster_sweep(const ClusterSweepArgs &a) {
 #undef PLQ_CC
     int64_t need = (cg.N + 255) / 256;
     int64_t G = need < a.max_blocks ? need : a.max_blocks;
-    int64_t span = (cg.N + G - 1) / G;
     cc_flatten<<<dim3((uint32_t)G), dim3(256), 0, a.stream>>>(
         a.labels, a.parent, a.rootlist, a.nroots, a.newlab, cg.N, a.seed32 ^ ROOT_SALT, a.seed32 ^ SEED_SALT, skey,
         a.sweep, a.q, a.seeded, a.forced, a.cap, a.flag);

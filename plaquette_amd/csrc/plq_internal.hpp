@@ -227,4 +227,37 @@ void launch_potts_measure(const uint8_t *labels, const CCGeom &cg, int q, long l
 void launch_cluster_resolve(int32_t *parent, int32_t *roots, const CCGeom &cg, uint32_t cap, int *flag,
                             int32_t *sizes, unsigned long long *hist, hipStream_t s);
 
+// --- per-configuration measurements (plq_measure.hip) ----------------------
+// Bit planes of a lattice-order configuration: row R = the axis-0 line with
+// coordinates x1..x_{D-1} (R = x1 + L1 * (x2 + L2 * x3)), W words per row,
+// bit x0 % 32 of word x0 / 32, bits past L0 zero; plane k at k * R * W.
+struct MeasGeom {
+    int32_t D;
+    int32_t L[MAX_D];        // extents; unused axes 1
+    int32_t rstride[MAX_D];  // row stride of axis a >= 1
+    int32_t R;               // rows, N / L0
+    int32_t W;               // words per row, ceil(L0 / 32)
+    int64_t N;
+};
+MeasGeom meas_geom(int D, const int64_t *shape);
+// Bit planes that hold a state 0..q-1.
+int meas_state_planes(int q);
+// Plane v * bits_per_var + p <- bit p of variable v (vars_per_site variables
+// per site, lat[site * vars_per_site + v]); vars_per_site 1 with
+// bits_per_var 1..4, or 2..4 with bits_per_var 1.
+void launch_meas_pack(const uint8_t *lat, const MeasGeom &m, int vars_per_site, int bits_per_var, uint32_t *bits,
+                      hipStream_t s);
+// agree[k] += sites where s(x) == s(x + r e_mu), pair k numbered axis after
+// axis over r = 0..L_mu/2; agree must start at zero.
+void launch_meas_pairs(const uint32_t *bits, const MeasGeom &m, int planes, int npairs, unsigned long long *agree,
+                       hipStream_t s);
+// *odd += lines along dir whose link product is -1; plane = links along dir.
+int64_t meas_polyakov_items(const MeasGeom &m, int dir);
+void launch_meas_polyakov(const uint32_t *plane, const MeasGeom &m, int dir, unsigned long long *odd, hipStream_t s);
+// D * max line-product planes from the D link planes (max < every extent).
+void launch_meas_lines(const uint32_t *links, const MeasGeom &m, int max, uint32_t *lines, hipStream_t s);
+// odd[((r-1) * max + t-1) * D(D-1) + o] += r-by-t loops with product -1 in
+// ordered direction pair o; 2 * max <= every extent.
+void launch_meas_wilson(const uint32_t *lines, const MeasGeom &m, int max, unsigned long long *odd, hipStream_t s);
+
 }  // namespace plq
