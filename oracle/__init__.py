@@ -109,6 +109,8 @@ def lib():
             "orc_plaquette_links": (None, [C.c_int32, P(C.c_int64), C.c_int64, P(C.c_int64)]),
             "orc_staple_table": (None, [C.c_int32, P(C.c_int64), P(C.c_int64)]),
             "orc_local_sweeps": (C.c_int, [P(LocalDesc), P(C.c_uint8), C.c_uint32, C.c_uint32]),
+            "orc_max_threads": (C.c_int32, []),
+            "orc_local_sweeps_mt": (C.c_double, [P(LocalDesc), P(C.c_uint8), C.c_uint32, C.c_uint32, C.c_int32]),
             "orc_ising_metropolis_accept": (C.c_int, [P(LocalDesc), C.c_int32, C.c_int64, C.c_uint32]),
             "orc_ising_heat_bath_choice": (C.c_int, [P(LocalDesc), C.c_int32, C.c_int64, C.c_uint32]),
             "orc_cluster_sweeps": (C.c_int, [P(ClusterDesc), P(C.c_uint8), C.c_uint32, C.c_uint32, P(C.c_int64)]),
@@ -247,6 +249,20 @@ def local_sweeps(desc: LocalDesc, state: np.ndarray, sweep0: int, n_sweeps: int)
     if rc != 0:
         raise ValueError("orc_local_sweeps rejected the description")
     return st
+
+
+def max_threads() -> int:
+    """Threads an OpenMP region of the oracle uses (OMP_NUM_THREADS, else the CPUs)."""
+    return int(lib().orc_max_threads())
+
+
+def local_sweeps_mt(desc: LocalDesc, state: np.ndarray, sweep0: int, n_sweeps: int, threads: int = 0):
+    """(state, seconds): the same sweeps over OpenMP threads (0 = all)."""
+    st = np.ascontiguousarray(state, dtype=np.uint8).copy()
+    secs = lib().orc_local_sweeps_mt(C.byref(desc), _u8(st), sweep0, n_sweeps, threads)
+    if secs < 0:
+        raise ValueError("bad descriptor")
+    return st, secs
 
 
 def ising_metropolis_accept(desc: LocalDesc, cur: int, nsum: int, m: int) -> int:

@@ -7,6 +7,8 @@
  * SSE2 doubles so that each f32/f64 operation rounds exactly once, as in the
  * Rust and WGSL sources.
  */
+#include <omp.h>
+
 #include "plq_oracle.h"
 
 #include <math.h>
@@ -355,6 +357,59 @@ int orc_local_sweeps(const orc_local_desc *d, uint8_t *state, uint32_t sweep0, u
     free(nbr);
     free(staples);
     return 0;
+}
+
+/* OpenMP's thread count for a parallel region (OMP_NUM_THREADS, else the CPUs). */
+int32_t orc_max_threads(void) { return omp_get_max_threads(); }
+
+/* The same sweeps with every colour pass spread over OpenMP threads.  In
+ * the keyed stream a variable's decision reads only variables of the other
+ * colour (or, for links, other directions) and its own draw, so the result is
+ * the sequential one whatever the thread count.  Returns the wall time of
+ * the sweeps in seconds (tables excluded), or -1 on a bad descriptor. */
+double orc_local_sweeps_mt(const orc_local_desc *d, uint8_t *state, uint32_t sweep0, uint32_t n_sweeps,
+                           int32_t threads) {
+    int dim = d->dim;
+    if (dim < 1 || dim > ORC_MAX_DIM || d->q < 2 || d->q > ORC_MAX_Q) return -1.0;
+    if (d->model == ORC_MODEL_Z2 && dim < 2) return -1.0;
+    if (threads < 1) threads = omp_get_max_threads();
+    int64_t n = orc_n_sites(dim, d->shape);
+    uint8_t *parity = (uint8_t *)malloc((size_t)n);
+    orc_parity_table(dim, d->shape, parity);
+    ctx_t c;
+    c.d = d;
+    c.state = state;
+    c.nn = 2 * dim;
+    int64_t *nbr = NULL, *staples = NULL;
+    if (d->model == ORC_MODEL_Z2) {
+        staples = (int64_t *)malloc((size_t)(n * dim * 6 * (dim - 1)) * sizeof(int64_t));
+        orc_staple_table(dim, d->shape, staples);
+    } else {
+        nbr = (int64_t *)malloc((size_t)(n * 2 * dim) * sizeof(int64_t));
+        orc_neighbor_table(dim, d->shape, nbr);
+    }
+    c.nbr = nbr;
+    c.staples = staples;
+    const double t0 = omp_get_wtime();
+    for (uint32_t t = 0; t < n_sweeps; ++t) {
+        uint32_t sweep = sweep0 + t;
+        const int ndir = d->model == ORC_MODEL_Z2 ? dim : 1;
+        for (int dir = 0; dir < ndir; ++dir)
+            for (int color = 0; color < 2; ++color) {
+#pragma omp parallel for schedule(static) num_threads(threads)
+                for (int64_t site = 0; site < n; ++site)
+                    if (parity[site] == color) {
+                        int64_t var = d->model == ORC_MODEL_Z2 ? site * dim + dir : site;
+                        if (d->kernel == ORC_KERNEL_METROPOLIS) metropolis(&c, var, sweep);
+                        else heat_bath(&c, var, sweep);
+                    }
+            }
+    }
+    const double secs = omp_get_wtime() - t0;
+    free(parity);
+    free(nbr);
+    free(staples);
+    return secs;
 }
 
 /* Single-variable decisions for an Ising site with a given neighbour sum,

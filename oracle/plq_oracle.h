@@ -100,6 +100,11 @@ void orc_staple_table(int32_t dim, const int64_t *shape, int64_t *out);
 
 /* K-mode checkerboard sweeps; state is lattice-ordered (site, or link=site*D+dir) */
 int orc_local_sweeps(const orc_local_desc *d, uint8_t *state, uint32_t sweep0, uint32_t n_sweeps);
+/* orc_local_sweeps with each colour pass over `threads` OpenMP threads (0: all);
+ * bit-identical to it.  Returns the seconds spent sweeping. */
+int32_t orc_max_threads(void);
+double orc_local_sweeps_mt(const orc_local_desc *d, uint8_t *state, uint32_t sweep0, uint32_t n_sweeps,
+                           int32_t threads);
 /* The acceptance table entries the device uses are checked against the literal
  * per-site arithmetic through these: the Metropolis / heat bath decision of a
  * single variable with a given 24-bit draw. */

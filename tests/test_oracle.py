@@ -464,3 +464,23 @@ def test_pcg64_step_and_output_match_numpy():
         got = [O.lib().orc_pcg64_next_u64(C.byref(g)) for _ in range(64)]
         assert [int(x) for x in want] == got
     assert O.lib().orc_pcg64_next_f64(C.byref(O.pcg64(3))) < 1.0
+
+
+@pytest.mark.parametrize("shape,model,q,kernel", [
+    ([16, 12], O.MODEL_ISING, 2, O.KERNEL_METROPOLIS),
+    ([8, 6, 10], O.MODEL_ISING, 2, O.KERNEL_HEAT_BATH),
+    ([10, 8], O.MODEL_POTTS, 4, O.KERNEL_METROPOLIS),
+    ([6, 4, 4, 6], O.MODEL_Z2, 2, O.KERNEL_HEAT_BATH),
+    ([8, 8, 8], O.MODEL_Z2, 2, O.KERNEL_METROPOLIS),
+])
+def test_threaded_sweeps_equal_the_sequential_ones(shape, model, q, kernel):
+    # The all-core CPU baseline: colour passes split over threads give the
+    # sequential sweep's bits, since a pass reads only the other colour.
+    d = O.local_desc(shape, model=model, q=q, kernel=kernel, beta=0.7, j=1.0, h=0.2, seed=99)
+    nv = int(np.prod(shape)) * (len(shape) if model == O.MODEL_Z2 else 1)
+    st = np.random.default_rng(4).integers(0, q, nv).astype(np.uint8)
+    want = O.local_sweeps(d, st, 3, 5)
+    for threads in (1, 3, 8):
+        got, secs = O.local_sweeps_mt(d, st, 3, 5, threads)
+        assert np.array_equal(got, want), threads
+        assert secs >= 0.0
