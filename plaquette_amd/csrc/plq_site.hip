@@ -423,10 +423,13 @@ __device__ __forceinline__ void x16_load(const uint4 *__restrict__ oth, const ui
     const uint32_t rc = r << lh16;  // chunk index of the row start
     k.p = p;
     k.site0 = (r << (sg.lh4 + 3u)) + (ch << 5) + p;
-    k.a = ld32(oth, w);
     // p = 1 needs the first byte of the next chunk, p = 0 the last byte of the
-    // previous one (wrapping at the row ends)
-    const uint32_t adj_dw = p ? ((rc + ((ch + 1u) & cmask)) << 2) : (((rc + ((ch - 1u) & cmask)) << 2) + 3u);
+    // previous one (wrapping at the row ends).  Computed without a branch: a
+    // branch here let the compiler wait for the first load inside it, before
+    // the other loads were issued.
+    const uint32_t nch = (ch + (p ? 1u : cmask)) & cmask;
+    const uint32_t adj_dw = ((rc + nch) << 2) + (p ^ 1u) * 3u;
+    k.a = ld32(oth, w);
     k.adj = ld32(reinterpret_cast<const uint32_t *>(oth), adj_dw);
     if (D >= 2) {
         k.rows[0] = ld32(oth, ((((x1 + 1u == sg.L1) ? r - (sg.L1 - 1u) : r + 1u)) << lh16) + ch);
@@ -490,6 +493,21 @@ __global__ __launch_bounds__(256) void ising_pass_x16(const uint4 *__restrict__ 
     for (uint32_t w = begin + threadIdx.x; w < end; w += blockDim.x) {
         X16Chunk<D> cur;
         x16_load<D>(oth, own, sg, uc, lh16, cmask, w, cur);
+        // The draws depend on the site and the sweep only, not on the
+        // lattice: all sixteen are computed here, between issuing the loads
+        // and the first use of their data, so the load latency is covered by
+        // about 60% of the chunk's VALU work.  The scheduling barriers keep
+        // the compiler from sinking the loads below the hashes or hoisting
+        // the waits above them.
+        uint32_t mm[16];
+        __builtin_amdgcn_sched_barrier(0);
+        if (MODE != 2) {
+            const uint32_t bx = seed ^ cur.site0;
+            const uint32_t v0 = bx ^ (bx >> 16);
+#pragma unroll
+            for (int k = 0; k < 16; ++k) mm[k] = lowbias32(lowbias32_tail(v0 ^ (2u * (uint32_t)k)) ^ skey) >> 8;
+        }
+        __builtin_amdgcn_sched_barrier(0);
         const uint4 a = cur.a;
         const uint32_t p = cur.p;
         // axis-0 neighbours: the same bytes shifted one site forward (p = 1)
@@ -511,8 +529,6 @@ __global__ __launch_bounds__(256) void ising_pass_x16(const uint4 *__restrict__ 
         const uint4 s = cur.s;
         uint4 out = s;
         if (MODE != 2) {
-            const uint32_t bx = seed ^ cur.site0;
-            const uint32_t v0 = bx ^ (bx >> 16);
             // m < t <=> m - t is negative (both below 2^31); the four sign
             // bytes are gathered into one dword with two byte permutes, and
             // their sign bits become the per-site hit bytes.
@@ -523,9 +539,7 @@ __global__ __launch_bounds__(256) void ising_pass_x16(const uint4 *__restrict__ 
                 for (int k = 0; k < 4; ++k) {
                     uint32_t addr = (addr4 >> (8 * k)) & 0xffu;
                     uint32_t t = *reinterpret_cast<const uint32_t *>(Tb + addr);
-                    uint32_t h1 = lowbias32_tail(v0 ^ (2u * (kbase + (uint32_t)k)));
-                    uint32_t mm = lowbias32(h1 ^ skey) >> 8;
-                    diff[k] = mm - t;
+                    diff[k] = mm[kbase + k] - t;
                 }
                 // byte 3 of diff[0..3] -> bytes 0..3
                 uint32_t lo = __builtin_amdgcn_perm(diff[1], diff[0], 0x0c0c0703u);
