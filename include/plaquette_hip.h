@@ -136,6 +136,11 @@ int plq_chain_advance(plq_chain *chain, uint64_t sweeps);
  * sweeps_between sweeps and copy the configuration out. */
 int plq_chain_next(plq_chain *chain, uint8_t *out, size_t n);
 int plq_chain_read(plq_chain *chain, uint8_t *out, size_t n);
+/* `count` samples, each after sweeps_between sweeps, copied on the device into a
+ * staging buffer and read back in one transfer: the sample batch of
+ * GpuChain::run_batch (device.rs:431-488; GPU_BATCH = 64 at device.rs:497-501).
+ * out holds count * n_vars states, sample after sample, in lattice order. */
+int plq_chain_next_batch(plq_chain *chain, uint64_t count, uint8_t *out, size_t n);
 int plq_chain_write(plq_chain *chain, const uint8_t *in, size_t n);
 /* On-device measurement of the current configuration (replaces the host
  * readback + measure of device.rs:435-494 and ising/mod.rs:177-186). */
@@ -164,6 +169,13 @@ int plq_chain_destroy(plq_chain *chain);
  * rand_pcg's Lcg128Xsl64 -- read, or replaced to continue a CPU run's stream. */
 int plq_chain_pcg_state(plq_chain *chain, uint64_t *out4);
 int plq_chain_set_pcg_state(plq_chain *chain, const uint64_t *in4);
+/* Configuration::hot (configuration.rs:80-87) as a run config builds it
+ * (ising/run_config.rs:237-251): RandRng::seed_from_u64(seed), then one
+ * random_range(0..q) (rng.rs:46-49) per variable in index order.  state4 (may
+ * be NULL) receives the stream state after the draws, which a
+ * PLQ_PROFILE_CPU_STREAM chain continues from (plq_chain_set_pcg_state).
+ * Host-only; no device is needed. */
+int plq_hot_start(uint64_t seed, int32_t q, uint8_t *out, size_t n, uint64_t *state4);
 
 /* ---------------------------------------------------------------------------
  * Per-configuration measurements, computed where the configuration lives (the

@@ -140,6 +140,9 @@ def lib():
                                               C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double]),
             "orc_pmode_ising_sweeps": (C.c_double, [C.c_int32, P(C.c_int64), C.c_int32, C.c_double, C.c_double,
                                                     C.c_double, C.c_uint64, P(C.c_uint8), C.c_uint32]),
+            "orc_pmode_ising_sweeps_from": (C.c_double, [C.c_int32, P(C.c_int64), C.c_int32, C.c_double,
+                                                         C.c_double, C.c_double, P(Pcg64), P(C.c_uint8),
+                                                         C.c_uint32]),
             "orc_axis_pair_sums": (None, [C.c_int32, P(C.c_int64), P(C.c_uint8), C.c_int32, P(C.c_int64)]),
             "orc_polyakov_sum": (C.c_int64, [C.c_int32, P(C.c_int64), P(C.c_uint8), C.c_int32, P(C.c_int64)]),
             "orc_wilson_sums": (C.c_int32, [C.c_int32, P(C.c_int64), P(C.c_uint8), C.c_int32, P(C.c_int64),
@@ -412,6 +415,13 @@ def pcg64(seed: int) -> Pcg64:
     g = Pcg64()
     lib().orc_pcg64_seed_from_u64(C.byref(g), seed)
     return g
+
+
+def pmode_ising_sweeps_from(shape, kernel, beta, j, h, gen: Pcg64, state, n_sweeps):
+    """The CPU loop continuing `gen` (advanced in place)."""
+    st = np.ascontiguousarray(state, dtype=np.uint8).copy()
+    lib().orc_pmode_ising_sweeps_from(len(shape), _shape(shape), kernel, beta, j, h, C.byref(gen), _u8(st), n_sweeps)
+    return st
 
 
 def pmode_ising_sweeps(shape, kernel, beta, j, h, seed, state, n_sweeps):

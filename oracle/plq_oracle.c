@@ -846,14 +846,23 @@ double orc_pcg64_next_f64(orc_pcg64 *g) {
 double orc_pmode_ising_sweeps(int32_t dim, const int64_t *shape, int32_t kernel,
                               double beta, double j, double h, uint64_t seed,
                               uint8_t *state, uint32_t n_sweeps) {
+    orc_pcg64 g;
+    orc_pcg64_seed_from_u64(&g, seed);
+    return orc_pmode_ising_sweeps_from(dim, shape, kernel, beta, j, h, &g, state, n_sweeps);
+}
+
+/* The same sweeps continuing a given stream (a run's generator after its hot
+ * start draws, ising/run_config.rs:237-251); *gen is advanced. */
+double orc_pmode_ising_sweeps_from(int32_t dim, const int64_t *shape, int32_t kernel,
+                                   double beta, double j, double h, orc_pcg64 *gen,
+                                   uint8_t *state, uint32_t n_sweeps) {
     int64_t n = orc_n_sites(dim, shape);
     int nn = 2 * dim;
     int64_t *nbr = (int64_t *)malloc((size_t)(n * nn) * sizeof(int64_t));
     uint8_t *parity = (uint8_t *)malloc((size_t)n);
     orc_neighbor_table(dim, shape, nbr);
     orc_parity_table(dim, shape, parity);
-    orc_pcg64 g;
-    orc_pcg64_seed_from_u64(&g, seed);
+    orc_pcg64 g = *gen;
     struct timespec t0, t1;
     clock_gettime(CLOCK_MONOTONIC, &t0);
     for (uint32_t t = 0; t < n_sweeps; ++t)
@@ -879,6 +888,7 @@ double orc_pmode_ising_sweeps(int32_t dim, const int64_t *shape, int32_t kernel,
                 }
             }
     clock_gettime(CLOCK_MONOTONIC, &t1);
+    *gen = g;
     free(nbr);
     free(parity);
     return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);

@@ -168,6 +168,9 @@ double orc_pcg64_next_f64(orc_pcg64 *g);
 double orc_pmode_ising_sweeps(int32_t dim, const int64_t *shape, int32_t kernel,
                               double beta, double j, double h, uint64_t seed,
                               uint8_t *state, uint32_t n_sweeps);
+double orc_pmode_ising_sweeps_from(int32_t dim, const int64_t *shape, int32_t kernel,
+                                   double beta, double j, double h, orc_pcg64 *gen,
+                                   uint8_t *state, uint32_t n_sweeps);
 
 /* ---- learned block proposal (plq_oracle_block.c) ----------------------------
  * 2D Ising, 8x8 blocks, block b = by * (L0/8) + bx, block colour (bx + by) & 1,

@@ -167,6 +167,8 @@ SIGNATURES = {
     "plq_chain_advance": (C.c_int, [_HANDLE, C.c_uint64]),
     "plq_chain_next": (C.c_int, [_HANDLE, _P(C.c_uint8), C.c_size_t]),
     "plq_chain_read": (C.c_int, [_HANDLE, _P(C.c_uint8), C.c_size_t]),
+    "plq_chain_next_batch": (C.c_int, [_HANDLE, C.c_uint64, _P(C.c_uint8), C.c_size_t]),
+    "plq_hot_start": (C.c_int, [C.c_uint64, C.c_int32, _P(C.c_uint8), C.c_size_t, _P(C.c_uint64)]),
     "plq_chain_write": (C.c_int, [_HANDLE, _P(C.c_uint8), C.c_size_t]),
     "plq_chain_measure": (C.c_int, [_HANDLE, _P(Obs)]),
     "plq_chain_record": (C.c_int, [_HANDLE, C.c_uint64, C.c_uint64]),
@@ -243,6 +245,16 @@ def gpu_available() -> bool:
     """Gpu::new().is_some() (device.rs:42-77)."""
     n = C.c_int32(0)
     return lib().plq_device_count(C.byref(n)) == PLQ_OK and n.value > 0
+
+
+def hot_start(seed: int, q: int, n: int):
+    """Configuration::hot from RandRng::seed_from_u64(seed) (configuration.rs:80-87,
+    rng.rs:30-50): (states, stream state after the draws as (state, increment))."""
+    out = np.empty(int(n), dtype=np.uint8)
+    st = (C.c_uint64 * 4)()
+    _check(lib().plq_hot_start(int(seed) & 0xFFFFFFFFFFFFFFFF, int(q), out.ctypes.data_as(_P(C.c_uint8)), out.size,
+                               st))
+    return out, (st[0] | (st[1] << 64), st[2] | (st[3] << 64))
 
 
 def fold_seed(seed: int) -> int:
@@ -323,6 +335,14 @@ class _Chain:
     def __next__(self) -> np.ndarray:
         out = np.empty(self.n_vars, dtype=np.uint8)
         _check(lib().plq_chain_next(self._h, out.ctypes.data_as(_P(C.c_uint8)), out.size))
+        return out
+
+    def next_batch(self, count: int) -> np.ndarray:
+        """`count` samples as GpuChain::run_batch yields them (device.rs:431-488):
+        each after sweeps_between sweeps, read back in one transfer; shape
+        (count, n_vars)."""
+        out = np.empty((int(count), self.n_vars), dtype=np.uint8)
+        _check(lib().plq_chain_next_batch(self._h, int(count), out.ctypes.data_as(_P(C.c_uint8)), out.size))
         return out
 
     def read(self) -> np.ndarray:

@@ -8,6 +8,8 @@
 // draws with the same value.
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -653,17 +655,59 @@ int plq_chain_destroy(plq_chain *c) {
     return PLQ_OK;
 }
 
-int plq_chain_advance(plq_chain *c, uint64_t sweeps) {
-    if (!c) return fail(PLQ_EINVAL, "chain is null");
-    std::lock_guard<std::mutex> lk(c->mu);
-    PLQ_HIP(hipSetDevice(c->device));
+namespace {
+
+// Enqueue `sweeps` sweeps of any chain kind (no synchronisation).
+int enqueue_sweeps(plq_chain *c, uint64_t sweeps) {
     if (c->kind == KIND_SITE && c->small) return enqueue_small_sweeps(c, sweeps);
     for (uint64_t t = 0; t < sweeps; ++t) {
         int rc = c->kind == KIND_CLUSTER ? enqueue_cluster_sweep(c) : enqueue_local_sweep(c);
         if (rc != PLQ_OK) return rc;
     }
+    return PLQ_OK;
+}
+
+}  // namespace
+
+int plq_chain_advance(plq_chain *c, uint64_t sweeps) {
+    if (!c) return fail(PLQ_EINVAL, "chain is null");
+    std::lock_guard<std::mutex> lk(c->mu);
+    PLQ_HIP(hipSetDevice(c->device));
+    PLQ_TRY(enqueue_sweeps(c, sweeps));
     if (c->kind == KIND_CLUSTER && sweeps > 0) return check_flag(c);
     return PLQ_OK;
+}
+
+int plq_chain_next_batch(plq_chain *c, uint64_t count, uint8_t *out, size_t n) {
+    if (!c) return fail(PLQ_EINVAL, "chain is null");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (count == 0) return PLQ_OK;
+    if (!out) return fail(PLQ_EINVAL, "output buffer is null");
+    const size_t nv = (size_t)c->n_vars;
+    if (count > SIZE_MAX / nv || n != (size_t)count * nv)
+        return fail(PLQ_EINVAL, "the output holds count * n_vars states");
+    PLQ_HIP(hipSetDevice(c->device));
+    const size_t bytes = (size_t)count * nv;
+    if (bytes > c->batch_cap) {
+        PLQ_HIP(hipStreamSynchronize(c->stream));
+        if (c->d_batch) (void)hipFree(c->d_batch);
+        c->d_batch = nullptr;
+        c->batch_cap = 0;
+        PLQ_TRY(alloc((void **)&c->d_batch, bytes));
+        c->batch_cap = bytes;
+    }
+    // device.rs:431-488: every sample after sweeps_between sweeps, copied into
+    // its slot of the staging buffer, and the batch read back in one transfer.
+    for (uint64_t k = 0; k < count; ++k) {
+        PLQ_TRY(enqueue_sweeps(c, c->sweeps_between));
+        uint8_t *slot = c->d_batch + k * nv;
+        if (c->kind == KIND_SITE) plq::launch_unpack_sites(slot, c->col[0], c->col[1], c->g, c->stream);
+        else if (c->kind == KIND_LINK) plq::launch_unpack_links(slot, c->d_links, c->g, c->stream);
+        else PLQ_HIP(hipMemcpyAsync(slot, c->labels, nv, hipMemcpyDeviceToDevice, c->stream));
+    }
+    PLQ_HIP(hipMemcpyAsync(out, c->d_batch, bytes, hipMemcpyDeviceToHost, c->stream));
+    PLQ_HIP(hipStreamSynchronize(c->stream));
+    return check_flag(c);
 }
 
 int plq_chain_next(plq_chain *c, uint8_t *out, size_t n) {
@@ -671,6 +715,15 @@ int plq_chain_next(plq_chain *c, uint8_t *out, size_t n) {
     PLQ_TRY(plq_chain_advance(c, c->sweeps_between));
     std::lock_guard<std::mutex> lk(c->mu);
     return read_lattice(c, out, n);
+}
+
+int plq_hot_start(uint64_t seed, int32_t q, uint8_t *out, size_t n, uint64_t *state4) {
+    if (!out && n) return fail(PLQ_EINVAL, "output buffer is null");
+    if (q < 2 || q > PLQ_MAX_Q) return fail(PLQ_EINVAL, "a hot start needs 2 <= q <= 16 states");
+    uint64_t st[4];
+    plq::pm_hot_start(seed, (uint32_t)q, out, n, st);
+    if (state4) std::memcpy(state4, st, sizeof st);
+    return PLQ_OK;
 }
 
 int plq_chain_read(plq_chain *c, uint8_t *out, size_t n) {

@@ -76,6 +76,9 @@ struct plq_chain {
     size_t meas_bits_cap = 0;                  // words
     unsigned long long *d_meas_counts = nullptr;
     size_t meas_counts_cap = 0;
+    // sample batches (plq_chain_next_batch)
+    uint8_t *d_batch = nullptr;
+    size_t batch_cap = 0;
 
     ~plq_chain() {
         (void)hipSetDevice(device);
@@ -85,7 +88,7 @@ struct plq_chain {
                        (void *)d_partials, (void *)d_meas, (void *)d_rec, (void *)d_block_counters, (void *)rootlist,
                        (void *)d_nroots, (void *)newlab,
                        (void *)d_block_logits, (void *)d_pm_cls, (void *)d_pm_state, (void *)d_pm_jt, (void *)d_pm_blk,
-                       (void *)d_pm_off, (void *)d_meas_bits, (void *)d_meas_counts})
+                       (void *)d_pm_off, (void *)d_meas_bits, (void *)d_meas_counts, (void *)d_batch})
             if (p) (void)hipFree(p);
         for (int k = 0; k < 2 * plq::MAX_D; ++k)
             if (links[k]) (void)hipFree(links[k]);

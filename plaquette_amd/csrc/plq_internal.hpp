@@ -154,6 +154,9 @@ struct PmodeArgs {
 };
 // rand_core seed_from_u64 + rand_pcg Lcg128Xsl64 (state lo, hi, increment lo, hi).
 void pm_seed(uint64_t seed, uint64_t out[4]);
+// Configuration::hot (configuration.rs:80-87) from RandRng::seed_from_u64(seed):
+// n states by random_range(0..q) (rng.rs:46-49), and the stream state after them.
+void pm_hot_start(uint64_t seed, uint32_t q, uint8_t *out, size_t n, uint64_t state_out[4]);
 void pm_jump_table(const uint64_t state[4], std::vector<uint64_t> &jt);
 bool pm_class_table(int D, int kernel, double beta, double j, double h, std::vector<uint64_t> &cls,
                     std::string &err);

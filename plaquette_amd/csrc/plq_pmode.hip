@@ -243,6 +243,38 @@ void pm_seed(uint64_t seed, uint64_t out[4]) {
     out[3] = (uint64_t)(inc >> 64);
 }
 
+// One Lcg128Xsl64 output (rand_pcg): step, then XSL-RR of the new state.
+static uint64_t pm_host_next(u128 &s, u128 inc) {
+    s = s * PCG_MUL + inc;
+    const uint64_t x = (uint64_t)(s >> 64) ^ (uint64_t)s;
+    const unsigned rot = (unsigned)(s >> 122);
+    return (x >> rot) | (x << ((64u - rot) & 63u));
+}
+
+void pm_hot_start(uint64_t seed, uint32_t q, uint8_t *out, size_t n, uint64_t state_out[4]) {
+    uint64_t st[4];
+    pm_seed(seed, st);
+    u128 s = ((u128)st[1] << 64) | st[0];
+    const u128 inc = ((u128)st[3] << 64) | st[2];
+    // rand's random_range(0..q) for a usize bound below 2^32 samples a u32
+    // (the low half of next_u64) by Canon's method: hi(x * q), and when the
+    // low half exceeds 2^32 - q, one more draw decides a carry into it.
+    for (size_t i = 0; i < n; ++i) {
+        const uint64_t w = (uint64_t)(uint32_t)pm_host_next(s, inc) * q;
+        uint32_t result = (uint32_t)(w >> 32);
+        const uint32_t lo = (uint32_t)w;
+        if (lo > (uint32_t)(0u - q)) {
+            const uint32_t hi2 = (uint32_t)(((uint64_t)(uint32_t)pm_host_next(s, inc) * q) >> 32);
+            if ((uint64_t)lo + hi2 > 0xffffffffull) result += 1;
+        }
+        out[i] = (uint8_t)result;
+    }
+    state_out[0] = (uint64_t)s;
+    state_out[1] = (uint64_t)(s >> 64);
+    state_out[2] = (uint64_t)inc;
+    state_out[3] = (uint64_t)(inc >> 64);
+}
+
 void pm_jump_table(const uint64_t state[4], std::vector<uint64_t> &jt) {
     jt.assign(4 * 64, 0);
     u128 A = PCG_MUL, C = ((u128)state[3] << 64) | state[2];
