@@ -108,6 +108,21 @@ __host__ __device__ __forceinline__ int32_t row_step(const Geom &g, int32_t r, c
     return r + delta * g.rstride[mu];
 }
 
+// Sum of v over the 64 lanes of a wave, returned in every lane; every lane
+// must be active.  DPP butterflies give each lane its quad's sum
+// (quad_perm [1,0,3,2] then [2,3,0,1]) and then its row's sum (row_ror 4 and
+// 8: a lane adds the quads four and eight lanes round its row of 16), and the
+// four row sums are read into a scalar.  A few cycles, where a shuffle
+// reduction waits on six LDS-crossbar round trips.
+__device__ __forceinline__ int wave_sum_i32(int v) {
+    v += __builtin_amdgcn_mov_dpp(v, 0xb1, 0xf, 0xf, false);
+    v += __builtin_amdgcn_mov_dpp(v, 0x4e, 0xf, 0xf, false);
+    v += __builtin_amdgcn_mov_dpp(v, 0x124, 0xf, 0xf, false);
+    v += __builtin_amdgcn_mov_dpp(v, 0x128, 0xf, 0xf, false);
+    return __builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16) + __builtin_amdgcn_readlane(v, 32) +
+           __builtin_amdgcn_readlane(v, 48);
+}
+
 // Deterministic f32 exponential: Cody-Waite range reduction (Cody & Waite,
 // "Software Manual for the Elementary Functions", 1980) with a split ln 2 and
 // a degree-7 Taylor polynomial on |r| <= ln2/2, scaled by 2^n through the

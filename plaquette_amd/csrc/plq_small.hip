@@ -29,6 +29,7 @@ namespace {
 
 constexpr int SMALL_THREADS = 1024;
 constexpr uint32_t NO_SITE = 0xffffffffu;
+constexpr uint32_t SMALL_RING = 64;
 
 struct SmallGeom {
     uint32_t lg[MAX_D];     // log2 of each extent (0 for unused axes)
@@ -47,7 +48,9 @@ __global__ __launch_bounds__(SMALL_THREADS) void small_chain_kernel(uint8_t *__r
     constexpr int NN = 2 * D;
     __shared__ uint8_t lat[2][SMALL_MAX_SITES / 2];
     __shared__ uint32_t tab[2 * (NN + 1)];
-    __shared__ int red[SMALL_THREADS / 64][2];
+    // Per-wave (bond, down) sums of up to SMALL_RING recorded sweeps, turned
+    // into records in one go by wave 0 (flush below).
+    __shared__ int ring[SMALL_RING][SMALL_THREADS / 64][2];
     const uint32_t tid = threadIdx.x;
     const uint32_t half = sg.half;
 
@@ -97,7 +100,29 @@ __global__ __launch_bounds__(SMALL_THREADS) void small_chain_kernel(uint8_t *__r
     __syncthreads();
 
     uint64_t togo = every ? every - 1 - phase % every : 0;  // sweeps before the next record
-    uint64_t nrec = 0;
+    uint64_t slot = every ? rec_first % rec_cap : 0;         // record-ring slot of the first pending record
+    uint32_t pend = 0;                                       // recorded sweeps held in ring[]
+    // Wave 0 writes the pending records, one lane per record; when more are
+    // pending than the record ring holds, only the newest rec_cap are kept,
+    // as sequential writes would leave them.
+    auto flush = [&]() {
+        if (tid < pend && tid + rec_cap >= pend) {
+            int b = 0, nd = 0;
+#pragma unroll
+            for (int wv = 0; wv < SMALL_THREADS / 64; ++wv) {
+                b += ring[tid][wv][0];
+                nd += ring[tid][wv][1];
+            }
+            long long *out = rec + ((slot + tid) % rec_cap) * REC_K;
+            out[0] = b;
+            out[1] = 0;
+            out[2] = 2 * (long long)half - nd;
+            out[3] = nd;
+            for (int q = 4; q < REC_K; ++q) out[q] = 0;
+        }
+        slot = (slot + pend) % rec_cap;
+        pend = 0;
+    };
     for (uint32_t t = 0; t < nsweeps; ++t) {
         const uint32_t skey = (sweep0 + t) * GOLDEN;
         const bool record = every != 0 && togo == 0;
@@ -130,37 +155,30 @@ __global__ __launch_bounds__(SMALL_THREADS) void small_chain_kernel(uint8_t *__r
                     if (c == 1) bond += (1 - 2 * nxt) * (NN - 2 * d);
                 }
             }
+            if (c == 1 && record) {
+                // per-wave sums into the LDS ring, before the pass's barrier
+                const int bw = wave_sum_i32(bond), nw = wave_sum_i32(ndown);
+                if ((tid & 63) == 0) {
+                    ring[pend][tid >> 6][0] = bw;
+                    ring[pend][tid >> 6][1] = nw;
+                }
+            }
             __syncthreads();
         }
         if (record) {
-            for (int o = 32; o > 0; o >>= 1) {
-                bond += __shfl_xor(bond, o, 64);
-                ndown += __shfl_xor(ndown, o, 64);
-            }
-            if ((tid & 63) == 0) {
-                red[tid >> 6][0] = bond;
-                red[tid >> 6][1] = ndown;
-            }
-            __syncthreads();
-            if (tid == 0) {
-                long long b = 0, nd = 0;
-                for (int wv = 0; wv < SMALL_THREADS / 64; ++wv) {
-                    b += red[wv][0];
-                    nd += red[wv][1];
-                }
-                long long *out = rec + ((rec_first + nrec) % rec_cap) * REC_K;
-                out[0] = b;
-                out[1] = 0;
-                out[2] = 2 * (long long)half - nd;
-                out[3] = nd;
-                for (int q = 4; q < REC_K; ++q) out[q] = 0;
-            }
-            nrec += 1;
+            // A full ring is written out right after the barrier that
+            // published its last entry.  The next writes to ring[] come after
+            // the next sweep's colour-0 barrier, which wave 0 reaches only
+            // after these reads.
+            pend += 1;
+            if (pend == SMALL_RING) flush();
             togo = every - 1;
         } else if (every != 0) {
             togo -= 1;
         }
     }
+
+    if (pend) flush();  // the last recorded sweep's barrier has published its entries
 
     for (uint32_t i = tid; i < half; i += SMALL_THREADS) {
         c0g[i] = lat[0][i];
