@@ -27,6 +27,15 @@ __host__ __device__ __forceinline__ uint32_t lowbias32(uint32_t v) {
     return v;
 }
 
+// lowbias32 after its first xor-shift (the caller supplies v ^ (v >> 16)).
+__host__ __device__ __forceinline__ uint32_t lowbias32_tail(uint32_t v) {
+    v *= 0x7feb352du;
+    v ^= v >> 15;
+    v *= 0x846ca68bu;
+    v ^= v >> 16;
+    return v;
+}
+
 // The 24-bit integer behind keyed_uniform (rng.wgsl:47-52):
 // keyed_uniform = m * 2^-24 exactly, so every comparison of it against a
 // threshold is an integer comparison of m.

@@ -369,14 +369,6 @@ __global__ __launch_bounds__(256) void ising_pass_swar(const uint32_t *__restric
 
 // ----------------------------------------------------- 16-site Ising pass ---
 
-// lowbias32 after its first xor-shift (the caller supplies v ^ (v >> 16)).
-__device__ __forceinline__ uint32_t lowbias32_tail(uint32_t v) {
-    v *= 0x7feb352du;
-    v ^= v >> 15;
-    v *= 0x846ca68bu;
-    v ^= v >> 16;
-    return v;
-}
 
 // Spread the low four bits of x to bit 0 of each byte: bit k of x lands at
 // bit 8k of x * (1 + 2^7 + 2^14 + 2^21), and no other partial product falls
