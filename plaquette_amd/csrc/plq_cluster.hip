@@ -303,15 +303,25 @@ __global__ __launch_bounds__(256) void cc_boundary(const uint8_t *__restrict__ l
 
 // -------------------------------------------------------------- relabel ---
 
-__device__ __forceinline__ long long blk_sum(long long v, long long *lds) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    __syncthreads();
-    if (lane == 0) lds[wave] = v;
+// Block totals of the first nv (uniform) of NV per-lane values, for blocks
+// of at most 4 waves with every lane active: DPP wave sums into LDS, one
+// barrier, then thread k adds value k over the waves and returns it (threads
+// k >= nv return 0).  Each lane's value and each wave's sum fit in an int.
+constexpr int BLK_WAVES = 4;
+template <int NV>
+__device__ __forceinline__ long long blk_sums(const int (&v)[NV], int nv, int (*wsum)[NV]) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        if (k < nv) {
+            const int s = wave_sum_i32(v[k]);
+            if (lane == 0) wsum[wave][k] = s;
+        }
+    }
     __syncthreads();
     long long t = 0;
-    if (threadIdx.x == 0)
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += lds[w];
+    if ((int)threadIdx.x < nv)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += wsum[w][threadIdx.x];
     return t;
 }
 
@@ -382,7 +392,6 @@ __global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, 
                                                   const uint8_t *__restrict__ newlab, int64_t N,
                                                   long long *__restrict__ partials, int64_t span,
                                                   int *__restrict__ nroots) {
-    __shared__ long long lds[16];
     long long moved = 0;
     const int64_t nd = N >> 2;
     int64_t begin = (int64_t)blockIdx.x * span;
@@ -410,7 +419,9 @@ __global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, 
             }
         }
     }
-    long long t = blk_sum(moved, lds);
+    __shared__ int wsum[BLK_WAVES][1];
+    const int mv[1] = {(int)moved};
+    const long long t = blk_sums<1>(mv, 1, wsum);
     if (threadIdx.x == 0) {
         long long *dst = partials + (int64_t)blockIdx.x * REC_K;
         dst[0] = 0;
@@ -427,7 +438,6 @@ __global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, 
 template <int D>
 __global__ __launch_bounds__(256) void potts_measure_kernel(const uint8_t *__restrict__ labels, CCGeom cg, int q,
                                                             long long *__restrict__ partials, int64_t span) {
-    __shared__ long long lds[16];
     long long agree = 0;
     int counts[MAX_Q_DEV];
     for (int a = 0; a < MAX_Q_DEV; ++a) counts[a] = 0;
@@ -449,16 +459,14 @@ __global__ __launch_bounds__(256) void potts_measure_kernel(const uint8_t *__res
             agree += labels[gp] == lab;
         }
     }
-    long long *dst = partials + (int64_t)blockIdx.x * REC_K;
-    long long t = blk_sum(agree, lds);
-    if (threadIdx.x == 0) {
-        dst[0] = t;
-        dst[1] = 0;
-    }
-    for (int a = 0; a < MAX_Q_DEV; ++a) {
-        t = blk_sum((long long)(a < q ? counts[a] : 0), lds);
-        if (threadIdx.x == 0) dst[2 + a] = t;
-    }
+    __shared__ int wsum[BLK_WAVES][2 + MAX_Q_DEV];
+    int vals[2 + MAX_Q_DEV];
+    vals[0] = (int)agree;
+    vals[1] = 0;
+#pragma unroll
+    for (int a = 0; a < MAX_Q_DEV; ++a) vals[2 + a] = counts[a];
+    const long long t = blk_sums<2 + MAX_Q_DEV>(vals, 2 + q, wsum);
+    if (threadIdx.x < (unsigned)REC_K) partials[(int64_t)blockIdx.x * REC_K + threadIdx.x] = t;
 }
 
 // The same sums four sites per lane, for L0 a multiple of 4: a dword of
@@ -469,7 +477,6 @@ __global__ __launch_bounds__(256) void potts_measure_kernel(const uint8_t *__res
 template <int D>
 __global__ __launch_bounds__(256) void potts_measure_vec(const uint8_t *__restrict__ labels, CCGeom cg, int q,
                                                          long long *__restrict__ part, int64_t span) {
-    __shared__ long long lds[16];
     const uint32_t *w32 = reinterpret_cast<const uint32_t *>(labels);
     const int64_t nd = cg.N >> 2;
     long long agree = 0;
@@ -505,15 +512,14 @@ __global__ __launch_bounds__(256) void potts_measure_vec(const uint8_t *__restri
             if (a < q) counts[a] += zero_bytes(w ^ (0x01010101u * (uint32_t)a));
     }
     const int64_t G = gridDim.x;
-    long long t = blk_sum(agree, lds);
-    if (threadIdx.x == 0) {
-        part[blockIdx.x] = t;
-        part[G + blockIdx.x] = 0;
-    }
-    for (int a = 0; a < q; ++a) {
-        t = blk_sum((long long)counts[a], lds);
-        if (threadIdx.x == 0) part[(2 + a) * G + blockIdx.x] = t;
-    }
+    __shared__ int wsum[BLK_WAVES][2 + MAX_Q_DEV];
+    int vals[2 + MAX_Q_DEV];
+    vals[0] = (int)agree;
+    vals[1] = 0;
+#pragma unroll
+    for (int a = 0; a < MAX_Q_DEV; ++a) vals[2 + a] = counts[a];
+    const long long t = blk_sums<2 + MAX_Q_DEV>(vals, 2 + q, wsum);
+    if ((int)threadIdx.x < 2 + q) part[(int64_t)threadIdx.x * G + blockIdx.x] = t;
 }
 
 // ---------------------------------------------------- roots and sizes ---
