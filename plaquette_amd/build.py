@@ -24,6 +24,7 @@ SOURCES = [
     "plq_site.hip",
     "plq_link.hip",
     "plq_cluster.hip",
+    "plq_cluster_small.hip",
     "plq_ensemble.hip",
     "plq_ensemble_abi.hip",
     "plq_small.hip",

@@ -608,6 +608,12 @@ int plq_chain_create_cluster(const plq_cluster_desc *d, const uint8_t *start, si
     // here every find and union walks strictly decreasing indices, so the
     // site count bounds any walk.
     c->pass_cap = (uint64_t)ns + 64;
+    // Small lattices run whole cluster sweeps in one workgroup;
+    // PLQ_SMALL_CLUSTER=0 keeps them on the four-kernel sweep (tests compare
+    // the two).
+    c->small = ns <= plq::small_cluster_max_sites(d->dim);
+    if (const char *sm = getenv("PLQ_SMALL_CLUSTER"))
+        if (atoi(sm) == 0) c->small = false;
     int rc = PLQ_OK;
     auto bail = [&](int code) {
         delete c;
@@ -657,9 +663,59 @@ int plq_chain_destroy(plq_chain *c) {
 
 namespace {
 
+// Cluster sweeps of a small lattice, up to 4,096 per launch of one workgroup
+// (plq_cluster_small.hip), with the record-ring bookkeeping of
+// enqueue_small_sweeps.
+int enqueue_small_cluster_sweeps(plq_chain *c, uint64_t sweeps) {
+    while (sweeps > 0) {
+        const uint32_t n = sweeps > 4096 ? 4096u : (uint32_t)sweeps;
+        plq::SmallClusterArgs a{};
+        a.labels = c->labels;
+        a.parent = c->parent;
+        a.partials = c->d_partials + plq::MAX_PASS_BLOCKS * plq::REC_K;
+        a.D = c->D;
+        for (int mu = 0; mu < plq::MAX_D; ++mu) a.L[mu] = mu < c->D ? (int32_t)c->shape[mu] : 1;
+        a.N = (uint32_t)c->n_sites;
+        a.q = c->q;
+        a.seeded = c->seeded;
+        a.forced = c->forced;
+        a.bond_seed = c->seed32 ^ plq::BOND_KEY;
+        a.root_seed = c->seed32 ^ plq::ROOT_SALT;
+        a.seed_seed = c->seed32 ^ plq::SEED_SALT;
+        a.bond_t = c->bond_t;
+        a.sweep0 = (uint32_t)c->sweeps_done;
+        a.nsweeps = n;
+        a.rec = c->d_rec;
+        a.rec_cap = c->rec_cap;
+        a.rec_first = c->rec_cap ? c->rec_count % c->rec_cap : 0;
+        a.every = c->rec_every;
+        a.phase = c->rec_every ? c->sweeps_done % c->rec_every : 0;
+        a.stream = c->stream;
+        if (c->rec_every)
+            for (uint64_t s = c->sweeps_done; s < c->sweeps_done + n; ++s)
+                if ((s + 1) % c->rec_every == 0) {
+                    c->rec_sweeps[c->rec_count % c->rec_cap] = s + 1;
+                    c->rec_count += 1;
+                }
+        plq::launch_small_cluster(a);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail(PLQ_EDEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
+        c->sweeps_done += n;
+        sweeps -= n;
+    }
+    c->last_relabel_blocks = 1;
+    c->have_partition = true;
+    return PLQ_OK;
+}
+
 // Enqueue `sweeps` sweeps of any chain kind (no synchronisation).
 int enqueue_sweeps(plq_chain *c, uint64_t sweeps) {
     if (c->kind == KIND_SITE && c->small) return enqueue_small_sweeps(c, sweeps);
+    // The one-workgroup cluster sweep cannot overrun the default hop budget
+    // (every walk is shorter than the site count); a lowered budget takes the
+    // four-kernel path, which counts hops.
+    if (c->kind == KIND_CLUSTER && c->small && c->pass_cap >= (uint64_t)c->n_sites + 64 && sweeps > 0)
+        return enqueue_small_cluster_sweeps(c, sweeps);
     for (uint64_t t = 0; t < sweeps; ++t) {
         int rc = c->kind == KIND_CLUSTER ? enqueue_cluster_sweep(c) : enqueue_local_sweep(c);
         if (rc != PLQ_OK) return rc;

@@ -230,6 +230,30 @@ void launch_potts_measure(const uint8_t *labels, const CCGeom &cg, int q, long l
 void launch_cluster_resolve(int32_t *parent, int32_t *roots, const CCGeom &cg, uint32_t cap, int *flag,
                             int32_t *sizes, unsigned long long *hist, hipStream_t s);
 
+// --- small-lattice cluster chains (plq_cluster_small.hip) ------------------
+// Swendsen-Wang / Wolff sweeps of a lattice of at most
+// small_cluster_max_sites(D) sites in one workgroup, up to 4,096 sweeps per
+// launch.
+constexpr int SMALL_CC_MAX_SITES = 8192;
+struct SmallClusterArgs {
+    uint8_t *labels;        // lattice order, read at the start and written back at the end
+    int32_t *parent;        // receives the last sweep's forest (every site -> its root)
+    long long *partials;    // REC_K entries: [0, sites relabelled by the last sweep, 0, ...]
+    int D;
+    int32_t L[MAX_D];
+    uint32_t N;
+    int q;
+    int seeded, forced;     // Wolff: Extent::Seeded, Relabel::ForcedChange
+    uint32_t bond_seed, root_seed, seed_seed;
+    uint32_t bond_t;        // open iff agree && m < bond_t
+    uint32_t sweep0, nsweeps;
+    long long *rec;         // record ring: [agree, 0, counts[0..q)] per recorded sweep
+    uint64_t rec_cap, rec_first, every, phase;  // as SmallChainArgs
+    hipStream_t stream;
+};
+void launch_small_cluster(const SmallClusterArgs &a);
+int64_t small_cluster_max_sites(int D);
+
 // --- per-configuration measurements (plq_measure.hip) ----------------------
 // Bit planes of a lattice-order configuration: row R = the axis-0 line with
 // coordinates x1..x_{D-1} (R = x1 + L1 * (x2 + L2 * x3)), W words per row,

@@ -181,6 +181,41 @@ def test_small_chain_matches_per_pass_kernels(gpu, monkeypatch, shape, kernel):
     assert [s for s, _, _ in outs[0][1]] == list(range(4113 - 3 * 40, 4113, 3))
 
 
+SMALL_CLUSTER_CASES = [([64, 64], 3, "sw"), ([7, 5], 2, "wolff"), ([16], 2, "sw"), ([9, 9, 9], 4, "wolff"),
+                       ([4, 6, 4, 8], 3, "sw"), ([90, 91], 2, "sw"), ([32, 32], 5, "wolff")]
+
+
+@pytest.mark.parametrize("shape,q,kind", SMALL_CLUSTER_CASES,
+                         ids=[f"{k}-q{q}-{'x'.join(map(str, s))}" for s, q, k in SMALL_CLUSTER_CASES])
+def test_small_cluster_chain_matches_the_four_kernel_sweep(gpu, monkeypatch, shape, q, kind):
+    # Lattices of up to 8,192 sites run whole cluster sweeps in one
+    # workgroup; the four-kernel sweep is an independent implementation.
+    # Advances of 5, 4100 and 7 sweeps cross the 4096-sweep launch limit, the
+    # ring keeps the last 40 of 1370 records, and the last sweep's partition
+    # (roots, size histogram, relabelled sites) must agree as well.
+    n = int(np.prod(shape))
+    start = start_config(n, q, True, 31)
+    beta = 1.0 if kind == "sw" else 0.6
+    outs = []
+    for small in ("1", "0"):
+        monkeypatch.setenv("PLQ_SMALL_CLUSTER", small)
+        make = gpu.GpuClusterChain.swendsen_wang if kind == "sw" else gpu.GpuClusterChain.wolff
+        c = make(shape, q, 1.0, beta, 77, start)
+        c.record(3, 40)
+        for k in (5, 4100, 7):
+            c.advance(k)
+        st = c.stats()
+        outs.append((c.read(), [(o.sweep, o.bond_sum, tuple(o.counts[a] for a in range(q))) for o in c.series()],
+                     c.roots(), (st.moved, st.n_clusters, st.largest, tuple(st.size_hist)), c.sweeps_done))
+        c.close()
+    assert outs[0][4] == outs[1][4] == 4112
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert outs[0][1] == outs[1][1]
+    assert np.array_equal(outs[0][2], outs[1][2])
+    assert outs[0][3] == outs[1][3]
+    assert [s for s, _, _ in outs[0][1]] == list(range(4113 - 3 * 40, 4113, 3))
+
+
 def test_recorded_series_matches_per_sweep_measurement(gpu):
     shape = [64, 64]
     n = 4096
