@@ -336,6 +336,69 @@ def bench_ising2d_64(P, ev, steps, warmup):
     }
 
 
+def bench_cluster_small(P, ev, steps, warmup):
+    """Small-lattice cluster chains: 2D q=3 Potts 64^2 Swendsen-Wang at beta_c
+    and 2D Ising 64^2 Wolff at beta_c, hot starts, a record every sweep (one
+    workgroup per chain, the whole lattice in LDS)."""
+    import numpy as np
+
+    shape, n = [64, 64], 4096
+    out = {}
+    for name, q, gap, beta, make in (("sw_potts3", 3, 1.0, 1.0050525, P.GpuClusterChain.swendsen_wang),
+                                     ("wolff_ising", 2, 2.0, 0.4406868, P.GpuClusterChain.wolff)):
+        start = np.random.default_rng(SEED).integers(0, q, n).astype(np.uint8)
+        c = make(shape, q, gap, beta, SEED, start)
+        c.record(1, steps + 1)
+        c.advance(warmup)
+        c.sync()
+        wall, ms = timed(P, ev, c.stream, lambda: (c.advance(steps), c.sync()))
+        c.close()
+        out[name] = {"us_per_sweep": round(wall * 1e6 / steps, 3), "site_updates_per_ns": round(steps * n / (wall * 1e9), 4)}
+    return {
+        "metric": "site-updates/ns, small-lattice cluster chains", "value": out["sw_potts3"]["site_updates_per_ns"],
+        "unit": "site-updates/ns", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": round(out["sw_potts3"]["us_per_sweep"] / 1e3, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": "64^2 q=3 Potts Swendsen-Wang (value) and 64^2 Ising Wolff, beta_c, hot start, "
+                               "a record every sweep", "lattice": shape},
+        "chains": out,
+    }
+
+
+def bench_measure(P, ev, reps):
+    """The device measurements: the Ising correlator of a 256^3 chain (387
+    axis-displacement pair sums) and the Wilson table of a 4D Z2 chain at
+    32^4 (sides 1..16, 3,072 rectangle classes), each timed per call with the
+    pack, launches and readback."""
+    import numpy as np
+
+    def per_call(f):
+        f()
+        t = time.perf_counter()
+        for _ in range(reps):
+            f()
+        return (time.perf_counter() - t) / reps
+
+    n3 = 256 ** 3
+    c = P.ising_gpu_chain([256] * 3, 1.0, 0.0, BETA, SEED, np.random.default_rng(SEED).integers(0, 2, n3).astype(np.uint8))
+    corr_s = per_call(c.correlator)
+    c.close()
+    n4 = 32 ** 4
+    g = P.gauge_gpu_chain([32] * 4, 1.0, 0.5, SEED, np.random.default_rng(SEED).integers(0, 2, 4 * n4).astype(np.uint8))
+    wil_s = per_call(lambda: g.wilson_rectangles(16))
+    pol_s = per_call(lambda: g.polyakov_loop(3))
+    g.close()
+    pairs = 3 * (256 // 2 + 1) * n3
+    return {
+        "metric": "site pairs/ns, device correlator", "value": round(pairs / (corr_s * 1e9), 2),
+        "unit": "site pairs/ns", "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": round(corr_s * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": "Ising correlator of a 256^3 random configuration; Wilson table (side 16) and a "
+                               "Polyakov loop of a 32^4 Z2 configuration"},
+        "correlator_ms": round(corr_s * 1e3, 4), "wilson_ms": round(wil_s * 1e3, 4), "polyakov_ms": round(pol_s * 1e3, 4),
+    }
+
+
 BLOCK_FLOP = 1253376  # network FLOPs per 8x8 block: 64 x 32 x 9 x 2 (conv1) + 64 x 32 x 288 x 2 (conv2) + 64 x 288 x 2 (conv3)
 
 
@@ -473,7 +536,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--profile", action="store_true",
                     help="run the fixed counter-collection workload instead of the benchmark")
-    ap.add_argument("--workload", default="ising3d", choices=["ising3d", "potts-sw", "z2-batch", "ising2d-64", "learned", "cpu-stream"],
+    ap.add_argument("--workload", default="ising3d",
+                    choices=["ising3d", "potts-sw", "z2-batch", "ising2d-64", "learned", "cpu-stream", "cluster-small",
+                             "measure"],
                     help="ising3d is the headline line; the others time the remaining configurations")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
@@ -512,6 +577,12 @@ def main():
         return
     if args.workload == "ising2d-64":
         print(json.dumps(bench_ising2d_64(P, ev, 10000, 1000)))
+        return
+    if args.workload == "cluster-small":
+        print(json.dumps(bench_cluster_small(P, ev, 10000, 500)))
+        return
+    if args.workload == "measure":
+        print(json.dumps(bench_measure(P, ev, 20)))
         return
     n = int(np.prod(SHAPE))
     metro = run_kernel(P, ev, P.Kernel.METROPOLIS, args.steps, args.warmup, args.kernel_sweeps)
