@@ -51,8 +51,10 @@ class ConfigError(ValueError):
     """config.rs:257-275: `kind` is "parse" (not valid TOML for the schema),
     "invalid" (parsed, but not runnable) or "io"."""
 
+    _PREFIX = {"parse": "could not parse config", "invalid": "invalid config", "io": "could not read config file"}
+
     def __init__(self, kind: str, message: str):
-        super().__init__(f"{'could not parse config' if kind == 'parse' else 'invalid config'}: {message}")
+        super().__init__(f"{self._PREFIX[kind]}: {message}")
         self.kind = kind
 
 
@@ -116,7 +118,15 @@ def _toml_value(v) -> str:
     if isinstance(v, float):
         return repr(v) if math.isfinite(v) else ("nan" if v != v else ("inf" if v > 0 else "-inf"))
     if isinstance(v, str):
-        return '"' + v.replace("\\", "\\\\").replace('"', '\\"') + '"'
+        out = []
+        for ch in v:  # a TOML basic string: escape quotes, backslashes and control characters
+            if ch in ('"', "\\"):
+                out.append("\\" + ch)
+            elif ord(ch) < 0x20 or ord(ch) == 0x7F:
+                out.append("\\u%04x" % ord(ch))
+            else:
+                out.append(ch)
+        return '"' + "".join(out) + '"'
     if isinstance(v, (list, tuple)):
         return "[" + ", ".join(_toml_value(x) for x in v) + "]"
     raise TypeError(f"cannot render {v!r}")
@@ -181,8 +191,7 @@ class _RunConfig:
             with open(path, "r", encoding="utf-8") as f:
                 text = f.read()
         except OSError as e:
-            err = ConfigError("io", f"could not read config file: {e}")
-            raise err from e
+            raise ConfigError("io", str(e)) from e
         return cls.parse(text)
 
     def dimension(self) -> int:
