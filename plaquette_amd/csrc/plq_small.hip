@@ -62,12 +62,16 @@ struct SmallRecorder {
     __device__ bool recording() const { return every != 0 && togo == 0; }
 
     // Publish this wave's sums (every lane of every wave calls this, before
-    // the colour-1 barrier of a recorded sweep).
-    __device__ void publish(SmallRing &ring, int bond, int ndown) const {
-        const int bw = wave_sum_i32(bond), nw = wave_sum_i32(ndown);
+    // the colour-1 barrier of a recorded sweep).  A wave that owns no site
+    // publishes zeros without reducing.  The two sums travel in one word,
+    // down * 2^16 + bond: a wave's |bond| and down count are below 2^15.
+    __device__ void publish(SmallRing &ring, int bond, int ndown, bool wave_owns_sites) const {
+        int packed = 0;
+        if (wave_owns_sites) packed = wave_sum_i32(ndown * 65536 + bond);
         if ((threadIdx.x & 63) == 0) {
-            ring[pend][threadIdx.x >> 6][0] = bw;
-            ring[pend][threadIdx.x >> 6][1] = nw;
+            const int b = ((packed + 32768) & 0xffff) - 32768;
+            ring[pend][threadIdx.x >> 6][0] = b;
+            ring[pend][threadIdx.x >> 6][1] = (packed - b) >> 16;
         }
     }
 
@@ -200,7 +204,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void small_chain_kernel(uint8_t *__r
                     if (c == 1) bond += (1 - 2 * nxt) * (NN - 2 * d);
                 }
             }
-            if (c == 1 && record) records.publish(ring, bond, ndown);
+            if (c == 1 && record) records.publish(ring, bond, ndown, (tid & ~63u) < half);
             __syncthreads();
         }
         records.end_sweep(ring);
@@ -317,7 +321,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void small_chain_x4(uint8_t *__restr
                     }
                 }
             }
-            if (c == 1 && record) records.publish(ring, bond, ndown);
+            if (c == 1 && record) records.publish(ring, bond, ndown, (tid & ~63u) < nw);
             __syncthreads();
         }
         records.end_sweep(ring);
