@@ -27,15 +27,6 @@ __host__ __device__ __forceinline__ uint32_t lowbias32(uint32_t v) {
     return v;
 }
 
-// lowbias32 after its first xor-shift (the caller supplies v ^ (v >> 16)).
-__host__ __device__ __forceinline__ uint32_t lowbias32_tail(uint32_t v) {
-    v *= 0x7feb352du;
-    v ^= v >> 15;
-    v *= 0x846ca68bu;
-    v ^= v >> 16;
-    return v;
-}
-
 // The 24-bit integer behind keyed_uniform (rng.wgsl:47-52):
 // keyed_uniform = m * 2^-24 exactly, so every comparison of it against a
 // threshold is an integer comparison of m.
@@ -45,11 +36,34 @@ __host__ __device__ __forceinline__ uint32_t keyed_bits(uint32_t seed, uint32_t 
     return k >> 8;
 }
 
-// The same with the sweep term hoisted: sweep_key = sweep * GOLDEN.
+// lowbias32 after its first xor-shift (the caller supplies v ^ (v >> 16)).
+__host__ __device__ __forceinline__ uint32_t lowbias32_tail(uint32_t v) {
+    v *= 0x7feb352du;
+    v ^= v >> 15;
+    v *= 0x846ca68bu;
+    v ^= v >> 16;
+    return v;
+}
+
+// lowbias32 between its first and its last xor-shift.
+__host__ __device__ __forceinline__ uint32_t lowbias32_mid(uint32_t v) {
+    v *= 0x7feb352du;
+    v ^= v >> 15;
+    v *= 0x846ca68bu;
+    return v;
+}
+
+// The sweep's key as the merged draw below consumes it.
+__host__ __device__ __forceinline__ uint32_t sweep_mix(uint32_t sweep_key) { return sweep_key ^ (sweep_key >> 16); }
+
+// The same draw with the sweep term hoisted: sweep_key = sweep * GOLDEN.
+// With u the first round before its last xor-shift (h = u ^ u >> 16), the
+// second round starts from (h ^ k) ^ ((h ^ k) >> 16) = u ^ (k ^ k >> 16),
+// since h >> 16 = u >> 16: the two xor-shifts between the rounds collapse
+// into one xor with sweep_mix(k), two operations fewer per draw.
 __host__ __device__ __forceinline__ uint32_t keyed_bits_k(uint32_t seed, uint32_t var, uint32_t sweep_key) {
-    uint32_t k = lowbias32(seed ^ var);
-    k = lowbias32(k ^ sweep_key);
-    return k >> 8;
+    const uint32_t x = seed ^ var;
+    return lowbias32_tail(lowbias32_mid(x ^ (x >> 16)) ^ sweep_mix(sweep_key)) >> 8;
 }
 
 // f32 uniform for the draws the reference scales in f32 (proposal, relabel,

@@ -492,12 +492,13 @@ __global__ __launch_bounds__(256) void ising_pass_x16(const uint4 *__restrict__ 
         // the compiler from sinking the loads below the hashes or hoisting
         // the waits above them.
         uint32_t mm[16];
+        const uint32_t kmix = sweep_mix(skey);  // keyed_bits_k's merged inter-round xor-shift
         __builtin_amdgcn_sched_barrier(0);
         if (MODE != 2) {
             const uint32_t bx = seed ^ cur.site0;
             const uint32_t v0 = bx ^ (bx >> 16);
 #pragma unroll
-            for (int k = 0; k < 16; ++k) mm[k] = lowbias32(lowbias32_tail(v0 ^ (2u * (uint32_t)k)) ^ skey) >> 8;
+            for (int k = 0; k < 16; ++k) mm[k] = lowbias32_tail(lowbias32_mid(v0 ^ (2u * (uint32_t)k)) ^ kmix) >> 8;
         }
         __builtin_amdgcn_sched_barrier(0);
         const uint4 a = cur.a;

@@ -283,7 +283,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void small_chain_x4(uint8_t *__restr
 
     SmallRecorder records(rec, rec_cap, rec_first, every, phase, half);
     for (uint32_t t = 0; t < nsweeps; ++t) {
-        const uint32_t skey = (sweep0 + t) * GOLDEN;
+        const uint32_t kmix = sweep_mix((sweep0 + t) * GOLDEN);  // keyed_bits_k's merged xor-shift
         const bool record = records.recording();
         int bond = 0, ndown = 0;
 #pragma unroll
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void small_chain_x4(uint8_t *__restr
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const uint32_t tk = *reinterpret_cast<const uint32_t *>(Tb + ((addr4 >> (8 * k)) & 0xffu));
-                    const uint32_t m = lowbias32(lowbias32_tail(v0[c] ^ (2u * (uint32_t)k)) ^ skey) >> 8;
+                    const uint32_t m = lowbias32_tail(lowbias32_mid(v0[c] ^ (2u * (uint32_t)k)) ^ kmix) >> 8;
                     diff[k] = m - tk;  // negative iff m < t (both below 2^31)
                 }
                 const uint32_t lo = __builtin_amdgcn_perm(diff[1], diff[0], 0x0c0c0703u);
