@@ -338,9 +338,6 @@ int enqueue_cluster_sweep(plq_chain *c) {
     a.forced = c->forced;
     a.cap = (uint32_t)(c->pass_cap > 0xffffffffull ? 0xffffffffull : c->pass_cap);
     a.flag = c->d_flag;
-    a.rootlist = c->rootlist;
-    a.nroots = c->d_nroots;
-    a.newlab = c->newlab;
     a.partials = c->d_partials + plq::MAX_PASS_BLOCKS * plq::REC_K;  // the measure kernels use the first half
     a.max_blocks = plq::MAX_PASS_BLOCKS;
     a.stream = c->stream;
@@ -624,11 +621,8 @@ int plq_chain_create_cluster(const plq_cluster_desc *d, const uint8_t *start, si
     if ((rc = alloc_scratch(c)) != PLQ_OK) return bail(rc);
     if ((rc = alloc((void **)&c->labels, (size_t)ns)) != PLQ_OK) return bail(rc);
     if ((rc = alloc((void **)&c->parent, (size_t)ns * sizeof(int32_t))) != PLQ_OK) return bail(rc);
-    if ((rc = alloc((void **)&c->rootlist, (size_t)ns * sizeof(int32_t))) != PLQ_OK) return bail(rc);
-    if ((rc = alloc((void **)&c->newlab, (size_t)ns)) != PLQ_OK) return bail(rc);
-    if ((rc = alloc((void **)&c->d_nroots, sizeof(int))) != PLQ_OK) return bail(rc);
     if ((rc = alloc((void **)&c->d_flag, sizeof(int))) != PLQ_OK) return bail(rc);
-    if (hipMemset(c->d_flag, 0, sizeof(int)) != hipSuccess || hipMemset(c->d_nroots, 0, sizeof(int)) != hipSuccess)
+    if (hipMemset(c->d_flag, 0, sizeof(int)) != hipSuccess)
         return bail(fail(PLQ_EDEVICE, "hipMemset failed"));
     if ((rc = write_lattice(c, start, n)) != PLQ_OK) return bail(rc);
     *out = c;

@@ -41,9 +41,6 @@ struct plq_chain {
     plq::CCGeom cg{};
     uint8_t *labels = nullptr;
     int32_t *parent = nullptr;
-    int32_t *rootlist = nullptr;           // box-local roots of the current sweep
-    int *d_nroots = nullptr;               // their count
-    uint8_t *newlab = nullptr;             // new label per local root
     int32_t *roots = nullptr;
     int32_t *sizes = nullptr;
     unsigned long long *hist = nullptr;
@@ -85,8 +82,7 @@ struct plq_chain {
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto p : {(void *)col[0], (void *)col[1], (void *)d_links, (void *)d_table, (void *)labels,
                        (void *)parent, (void *)roots, (void *)sizes, (void *)hist, (void *)d_flag, (void *)d_lat,
-                       (void *)d_partials, (void *)d_meas, (void *)d_rec, (void *)d_block_counters, (void *)rootlist,
-                       (void *)d_nroots, (void *)newlab,
+                       (void *)d_partials, (void *)d_meas, (void *)d_rec, (void *)d_block_counters,
                        (void *)d_block_logits, (void *)d_pm_cls, (void *)d_pm_state, (void *)d_pm_jt, (void *)d_pm_blk,
                        (void *)d_pm_off, (void *)d_meas_bits, (void *)d_meas_counts, (void *)d_batch})
             if (p) (void)hipFree(p);

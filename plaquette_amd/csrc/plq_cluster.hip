@@ -16,12 +16,10 @@
 //   2. cc_boundary one lane per box-face site: draw the bonds that leave a
 //                 box and union their ends in the global forest with
 //                 compare-and-swap.
-//   3. cc_flatten one lane per box-local root (cc_local lists them): find
-//                 the global root, draw the cluster's new label keyed on it
-//                 (Redraw or ForcedChange; for Wolff only the seed site's
-//                 cluster) and store it at the local root.
-//   4. cc_relabel one lane per site: the site's parent is a local root, whose
-//                 stored label is the site's new label.
+//   3. cc_relabel  one lane per four sites: walk from the site's box-local
+//                 root to the cluster root, draw the cluster's new label
+//                 keyed on it (Redraw or ForcedChange; for Wolff only the
+//                 seed site's cluster) and store it.
 //
 // Every union links the larger root under the smaller one, so parent[x] <= x
 // always holds and the root of a tree is the minimum site index in it: the
@@ -89,10 +87,8 @@ __device__ __forceinline__ uint32_t cdiv(uint32_t n, uint64_t M) { return (uint3
 // into box coordinates by shifts and masks instead of reciprocal divisions.
 template <int D, bool POW2>
 __global__ __launch_bounds__(1024) void cc_local(const uint8_t *__restrict__ labels, int32_t *__restrict__ parent,
-                                                 CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t,
-                                                 int32_t *__restrict__ rootlist, int *__restrict__ nroots) {
+                                                 CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t) {
     extern __shared__ int32_t P[];
-    __shared__ uint32_t box_roots, box_base, box_next;
     uint32_t o[MAX_D], e[MAX_D], ls[MAX_D];
     uint64_t me[MAX_D];
     uint32_t rest = blockIdx.x;
@@ -131,10 +127,6 @@ __global__ __launch_bounds__(1024) void cc_local(const uint8_t *__restrict__ lab
         }
         return g;
     };
-    if (threadIdx.x == 0) {
-        box_roots = 0;
-        box_next = 0;
-    }
     // Axis 0 by a wave scan: lanes hold consecutive local indices, so a run of
     // open axis-0 bonds is a run of lanes; every site points at the first site
     // of its run (parent <= child, as every union keeps), with no atomics.
@@ -193,25 +185,185 @@ __global__ __launch_bounds__(1024) void cc_local(const uint8_t *__restrict__ lab
         const uint32_t gsite = locate(l, y);
         const uint32_t groot = root == l ? gsite : locate(root, yr);
         parent[gsite] = (int32_t)groot;
-        const uint64_t rm = __ballot(root == l);  // one LDS atomic per wave
-        if (lane == (uint32_t)(__ffsll((unsigned long long)__ballot(1)) - 1)) atomicAdd(&box_roots, (uint32_t)__popcll(rm));
+    }
+}
+
+// Bit k (k < 4) set where byte k of a equals byte k of b.  Zero bytes of
+// z = a ^ b have bit 7 clear in ((z & 0x7f..) + 0x7f..) | z; the flags at
+// bits 0, 8, 16, 24 of f are gathered to bits 24..27 by one multiply with
+// 2^24 + 2^17 + 2^10 + 2^3 (every other partial product lands below bit 24
+// or above bit 31, on a position no other one takes, so nothing carries).
+__device__ __forceinline__ uint32_t equal_bytes4(uint32_t a, uint32_t b) {
+    const uint32_t z = a ^ b;
+    const uint32_t f = (~(((z & 0x7f7f7f7fu) + 0x7f7f7f7fu) | z) >> 7) & 0x01010101u;
+    return (f * 0x01020408u) >> 24;
+}
+
+__device__ __forceinline__ uint32_t equal_bytes16(const uint4 &a, const uint4 &b) {
+    return equal_bytes4(a.x, b.x) | (equal_bytes4(a.y, b.y) << 4) | (equal_bytes4(a.z, b.z) << 8) |
+           (equal_bytes4(a.w, b.w) << 12);
+}
+
+// The same box labelling as cc_local, sixteen sites per lane: lane c owns
+// the box-local indices 16c .. 16c+15, one segment of an axis-0 row (every
+// extent a power of two, box extent along axis 0 a multiple of 16).
+//   1. Bond bits.  The lane reads its 16 labels and, per axis, the 16 labels
+//      one step up as one 16-byte load each; equal bytes and the keyed draws
+//      give a 16-bit mask of open bonds per axis.  P[16c + k] is the first
+//      site of k's run of open axis-0 bonds inside the segment, and the
+//      segment's axis-0 mask goes to H[c].
+//   2. Unions, after a barrier, between run starts: the axis-0 bond out of
+//      the segment's last site, and each open bond of the other axes unless
+//      an earlier bond of the segment joins the same two runs (the runs of
+//      the row one step up are read from its segment's H).
+//   3. After a barrier, the roots of the run starts; every site of a run
+//      stores the run's root as a global index (four 16-byte stores).
+// The draws are the reference's (bond = site * D + axis); for D = 1, 2, 4
+// the site index of a segment's first site times D is a multiple of 16 D,
+// so the bond index is that base or-ed with k D + axis, and the first
+// xor-shift of the hash is done once per segment.
+template <int D>
+__global__ __launch_bounds__(1024) void cc_local_x16(const uint8_t *__restrict__ labels, int32_t *__restrict__ parent,
+                                                     CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t) {
+    extern __shared__ int32_t P[];
+    __shared__ uint16_t H[1024];
+    uint32_t o[MAX_D], e[MAX_D], lsh[MAX_D];
+    uint32_t rest = blockIdx.x;
+    uint32_t lg = 0;
+#pragma unroll
+    for (int mu = 0; mu < D; ++mu) {
+        const uint32_t nb = (uint32_t)cg.nbox[mu];
+        const uint32_t q = rest / nb;  // once per block
+        const uint32_t b = rest - q * nb;
+        rest = q;
+        e[mu] = (uint32_t)cg.B[mu];
+        o[mu] = b * e[mu];
+        lsh[mu] = lg;
+        lg += (uint32_t)(31 - __clz((int)e[mu]));
+    }
+    const uint32_t nloc = 1u << lg;
+    auto locate = [&](uint32_t l) -> uint32_t {
+        uint32_t g = 0;
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) g += (o[mu] + ((l >> lsh[mu]) & (e[mu] - 1u))) * (uint32_t)cg.stride[mu];
+        return g;
+    };
+    const uint32_t c = threadIdx.x;
+    const bool act = c < (nloc >> 4);
+    const uint32_t l0 = c << 4;
+    uint32_t y[MAX_D];
+    uint32_t gsite0 = 0;
+#pragma unroll
+    for (int mu = 0; mu < D; ++mu) {
+        y[mu] = (l0 >> lsh[mu]) & (e[mu] - 1u);
+        gsite0 += (o[mu] + y[mu]) * (uint32_t)cg.stride[mu];
+    }
+    const uint32_t kmix = sweep_mix(skey);
+    constexpr bool XOR_BONDS = (D & (D - 1)) == 0;
+    const uint32_t bx = bond_seed ^ (gsite0 * (uint32_t)D);
+    const uint32_t v0 = bx ^ (bx >> 16);
+    // the 16 bonds of axis mu: bit k set iff open (labels agree and m < t)
+    auto draw16 = [&](uint32_t eq, int mu) -> uint32_t {
+        uint32_t open = 0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const uint32_t m = XOR_BONDS
+                                   ? lowbias32_tail(lowbias32_mid(v0 ^ (uint32_t)(k * D + mu)) ^ kmix) >> 8
+                                   : keyed_bits_k(bond_seed, (gsite0 + (uint32_t)k) * (uint32_t)D + (uint32_t)mu, skey);
+            open |= (m < bond_t ? 1u : 0u) << k;
+        }
+        return open & eq;
+    };
+    uint32_t h = 0, starts = 1;
+    uint32_t vmask[MAX_D];
+    uint32_t nbr_l0[MAX_D];
+    bool x_out = false;
+    uint32_t x_to = 0;
+    if (act) {
+        const uint4 me = *reinterpret_cast<const uint4 *>(labels + gsite0);
+        // axis 0: site k to k + 1; site 15 to the next segment (or the row's
+        // first segment when the box spans axis 0), when that stays in the box
+        const bool x_in = y[0] + 16u < e[0];
+        const bool x_wrap = !x_in && e[0] == (uint32_t)cg.L[0];
+        x_out = x_in || x_wrap;
+        x_to = x_in ? l0 + 16u : l0 + 16u - e[0];
+        const uint32_t nb = x_out ? (uint32_t)labels[x_in ? gsite0 + 16u : gsite0 + 16u - (uint32_t)cg.L[0]] : 0u;
+        uint4 nx;
+        nx.x = __builtin_amdgcn_alignbyte(me.y, me.x, 1);
+        nx.y = __builtin_amdgcn_alignbyte(me.z, me.y, 1);
+        nx.z = __builtin_amdgcn_alignbyte(me.w, me.z, 1);
+        nx.w = __builtin_amdgcn_alignbyte(nb, me.w, 1);
+        uint32_t eq0 = equal_bytes16(me, nx);
+        if (!x_out) eq0 &= 0x7fffu;
+        h = draw16(eq0, 0);
+        starts = ((~h << 1) | 1u) & 0xffffu;  // bit k: site k begins a run inside the segment
+#pragma unroll
+        for (int mu = 1; mu < D; ++mu) {
+            vmask[mu] = 0;
+            uint32_t gn;
+            if (y[mu] + 1u < e[mu]) {
+                nbr_l0[mu] = l0 + (1u << lsh[mu]);
+                gn = gsite0 + (uint32_t)cg.stride[mu];
+            } else if (e[mu] == (uint32_t)cg.L[mu]) {  // the box spans the axis: the wrap bond is internal
+                nbr_l0[mu] = l0 - ((e[mu] - 1u) << lsh[mu]);
+                gn = gsite0 - (uint32_t)(cg.L[mu] - 1) * (uint32_t)cg.stride[mu];
+            } else {
+                continue;  // bonds leaving the box (cc_boundary)
+            }
+            const uint4 up = *reinterpret_cast<const uint4 *>(labels + gn);
+            vmask[mu] = draw16(equal_bytes16(me, up), mu);
+        }
+        H[c] = (uint16_t)(h & 0x7fffu);
+        int32_t run = (int32_t)l0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if ((starts >> k) & 1u) run = (int32_t)(l0 + (uint32_t)k);
+            P[l0 + (uint32_t)k] = run;
+        }
     }
     __syncthreads();
-    // Append the box's local roots to the sweep's root list.
-    if (threadIdx.x == 0) box_base = (uint32_t)atomicAdd(nroots, (int)box_roots);
-    __syncthreads();
-    const uint64_t lower = (1ull << lane) - 1ull;
-    for (uint32_t l = threadIdx.x; l < nloc; l += blockDim.x) {
-        const bool is_root = lds_load(&P[l]) == (int32_t)l;  // roots keep P[r] = r; path halving moved only non-roots
-        const uint64_t rm = __ballot(is_root);
-        const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)__ballot(1)) - 1);
-        uint32_t wbase = 0;
-        if (lane == leader) wbase = atomicAdd(&box_next, (uint32_t)__popcll(rm));
-        wbase = __shfl(wbase, (int)leader, 64);
-        if (is_root) {
-            uint32_t y[MAX_D];
-            rootlist[box_base + wbase + (uint32_t)__popcll(rm & lower)] = (int32_t)locate(l, y);
+    if (act) {
+        if (x_out && ((h >> 15) & 1u)) lds_union(P, (int32_t)(l0 + 31u - (uint32_t)__clz((int)starts)), (int32_t)x_to);
+#pragma unroll
+        for (int mu = 1; mu < D; ++mu) {
+            uint32_t v = vmask[mu];
+            if (!v) continue;
+            const uint32_t sn = ((~(uint32_t)H[nbr_l0[mu] >> 4] << 1) | 1u) & 0xffffu;  // the neighbour's run starts
+            // keep the first bond of the segment between each pair of runs
+            uint32_t need = 0;
+            for (uint32_t w = v; w; w &= w - 1u) {
+                const uint32_t k = (uint32_t)__builtin_ctz(w);
+                const uint32_t le = (2u << k) - 1u;
+                const uint32_t m = max(31u - (uint32_t)__clz((int)(starts & le)), 31u - (uint32_t)__clz((int)(sn & le)));
+                need |= (v & ((1u << k) - (1u << m))) ? 0u : 1u << k;
+            }
+            while (need) {
+                const uint32_t k = (uint32_t)__builtin_ctz(need);
+                need &= need - 1u;
+                const uint32_t le = (2u << k) - 1u;
+                const uint32_t a = 31u - (uint32_t)__clz((int)(starts & le));
+                const uint32_t b = 31u - (uint32_t)__clz((int)(sn & le));
+                lds_union(P, (int32_t)(l0 + a), (int32_t)(nbr_l0[mu] + b));
+            }
         }
+    }
+    __syncthreads();
+    if (act) {
+        int32_t out[16];
+        int32_t g = 0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if ((starts >> k) & 1u) {
+                const uint32_t r = (uint32_t)lds_find(P, (int32_t)(l0 + (uint32_t)k));
+                g = (int32_t)(r == l0 + (uint32_t)k ? gsite0 + (uint32_t)k : locate(r));
+            }
+            out[k] = g;
+        }
+        int4 *dst = reinterpret_cast<int4 *>(parent + gsite0);
+        dst[0] = make_int4(out[0], out[1], out[2], out[3]);
+        dst[1] = make_int4(out[4], out[5], out[6], out[7]);
+        dst[2] = make_int4(out[8], out[9], out[10], out[11]);
+        dst[3] = make_int4(out[12], out[13], out[14], out[15]);
     }
 }
 
@@ -257,28 +409,39 @@ __global__ __launch_bounds__(256) void cc_boundary(const uint8_t *__restrict__ l
         if (t < count) break;
         t -= count;
     }
-    if (mu >= D) return;
-    int32_t k = (int32_t)(t / layer_sites);
-    int64_t rem = t - (int64_t)k * layer_sites;  // index over the other axes
-    int32_t xm = (k + 1) * cg.B[mu];
-    if (xm > cg.L[mu]) xm = cg.L[mu];
-    xm -= 1;
-    // rebuild the site: rem enumerates the other axes in lattice order
-    int64_t gsite = 0;
+    bool open = false;
+    int32_t pa = -1, pb = -1;
+    int64_t gsite = 0, gp = 0;
+    if (mu < D) {
+        int32_t k = (int32_t)(t / layer_sites);
+        int64_t rem = t - (int64_t)k * layer_sites;  // index over the other axes
+        int32_t xm = (k + 1) * cg.B[mu];
+        if (xm > cg.L[mu]) xm = cg.L[mu];
+        xm -= 1;
+        // rebuild the site: rem enumerates the other axes in lattice order
 #pragma unroll
-    for (int a = 0; a < D; ++a) {
-        if (a == mu) {
-            gsite += (int64_t)xm * cg.stride[a];
-        } else {
-            int32_t ya = (int32_t)(rem % cg.L[a]);
-            rem /= cg.L[a];
-            gsite += (int64_t)ya * cg.stride[a];
+        for (int a = 0; a < D; ++a) {
+            if (a == mu) {
+                gsite += (int64_t)xm * cg.stride[a];
+            } else {
+                int32_t ya = (int32_t)(rem % cg.L[a]);
+                rem /= cg.L[a];
+                gsite += (int64_t)ya * cg.stride[a];
+            }
+        }
+        gp = (xm + 1 == cg.L[mu]) ? gsite - (int64_t)xm * cg.stride[mu] : gsite + cg.stride[mu];
+        open = labels[gsite] == labels[gp] && keyed_bits_k(bond_seed, (uint32_t)(gsite * D + mu), skey) < bond_t;
+        if (open) {
+            pa = load_relaxed(&parent[gsite]);
+            pb = load_relaxed(&parent[gp]);
         }
     }
-    int64_t gp = (xm + 1 == cg.L[mu]) ? gsite - (int64_t)xm * cg.stride[mu] : gsite + cg.stride[mu];
-    if (labels[gsite] != labels[gp]) return;
-    uint32_t bond = (uint32_t)(gsite * D + mu);
-    if (keyed_bits_k(bond_seed, bond, skey) >= bond_t) return;
+    // Consecutive face sites mostly pair the same two box-local roots: a
+    // bond whose ends had the same parents as the previous lane's open bond
+    // joins the same two sets, which that lane's union already joins.
+    const bool prev_open = __shfl_up(open ? 1 : 0, 1, 64) != 0;
+    const int32_t prev_pa = __shfl_up(pa, 1, 64), prev_pb = __shfl_up(pb, 1, 64);
+    if (!open || ((threadIdx.x & 63u) != 0u && prev_open && prev_pa == pa && prev_pb == pb)) return;
     uint32_t hops = 0;
     int32_t a = (int32_t)gsite, b = (int32_t)gp;
     for (;;) {
@@ -325,56 +488,18 @@ __device__ __forceinline__ long long blk_sums(const int (&v)[NV], int nv, int (*
     return t;
 }
 
-// One lane per local root of this sweep (rootlist[0 .. *nroots)): the new
-// label of its cluster, keyed on the global root (cluster_relabel.wgsl:39-90),
-// stored at the local root; NO_LABEL where the cluster keeps its label
-// (Wolff: every cluster but the seed site's).  The hop budget counts the hop
-// from a member site to its local root as well.
-constexpr uint8_t NO_LABEL = 0xff;
-
-__global__ __launch_bounds__(256) void cc_flatten(const uint8_t *__restrict__ labels,
-                                                  const int32_t *__restrict__ parent,
-                                                  const int32_t *__restrict__ rootlist, const int *__restrict__ nroots,
-                                                  uint8_t *__restrict__ newlab, int64_t N, uint32_t root_seed,
-                                                  uint32_t seed_seed, uint32_t skey, uint32_t sweep, int q,
-                                                  int seeded, int forced, uint32_t cap, int *__restrict__ flag) {
-    __shared__ int32_t seed_root;
-    if (seeded) {
-        if (threadIdx.x == 0) {
-            // cluster_relabel.wgsl:59-62: the seed site from the keyed draw,
-            // scaled in f32 and clamped.
-            float u = keyed_uniform_f32(keyed_bits(seed_seed, 0u, sweep));
-            uint32_t ss = (uint32_t)(u * (float)(uint32_t)N);
-            if (ss >= (uint32_t)N) ss = (uint32_t)N - 1u;
-            uint32_t hops = 0;
-            int32_t r = find_ro(parent, (int32_t)ss, cap, hops);
-            if (hops > cap) atomicOr(flag, 1);
-            seed_root = r;
-        }
-        __syncthreads();
-    }
-    const int n = *nroots;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int32_t r = rootlist[i];
-        uint32_t hops = 1;
-        const int32_t root = find_ro(parent, r, cap, hops);
-        if (hops > cap) atomicOr(flag, 1);
-        uint8_t nl = NO_LABEL;
-        if (!seeded || root == seed_root) {
-            float u = keyed_uniform_f32(keyed_bits_k(root_seed, (uint32_t)root, skey));
-            if (forced) {
-                uint32_t step = (uint32_t)(u * (float)(uint32_t)(q - 1));
-                if (step >= (uint32_t)(q - 1)) step = (uint32_t)(q - 2);
-                nl = (uint8_t)((labels[r] + 1u + step) % (uint32_t)q);
-            } else {
-                uint32_t draw = (uint32_t)(u * (float)(uint32_t)q);
-                if (draw >= (uint32_t)q) draw = (uint32_t)(q - 1);
-                nl = (uint8_t)draw;
-            }
-        }
-        newlab[r] = nl;
-    }
-}
+// The new label of every site, four sites per lane (one dword of labels, one
+// int4 of parents).  A site's parent is its box-local root (cc_local stores
+// it) or, where cc_boundary split a path through the site, an ancestor of
+// that root; the walk from there ends at the cluster root, the minimum site of the cluster, on which the label draw is keyed
+// (cluster_relabel.wgsl:39-90): Redraw takes floor(u q), ForcedChange adds
+// 1 + floor(u (q - 1)) to the cluster's label (every site of a cluster holds
+// that label, since only agreeing sites are bonded).  Seeded (Wolff): only
+// the cluster of the seed site, drawn from the sweep's seed key
+// (cluster_relabel.wgsl:59-62), changes.  The hop budget counts the hop
+// from a site to its local root as well.  The last block also takes the
+// N mod 4 tail.  Partials per block: [0, relabelled sites, 0 ...].
+constexpr uint32_t NO_LABEL = 0xffu;
 
 // Number of zero bytes of x (exact: no borrow between bytes).
 __device__ __forceinline__ int zero_bytes(uint32_t x) {
@@ -382,16 +507,43 @@ __device__ __forceinline__ int zero_bytes(uint32_t x) {
     return __popc(~(t | x | 0x7f7f7f7fu));
 }
 
-// The site's parent is a local root of this sweep (cc_local stores local
-// roots, cc_boundary links local roots and halves paths through them), whose
-// stored label is the site's new label.  Four sites per lane (one dword of
-// labels, one int4 of parents); the last block also takes the N mod 4 tail
-// and resets the root count for the next sweep.  Partials per block:
-// [0, moved, 0 ...].
+__device__ __forceinline__ uint32_t relabel_draw(uint32_t u_bits, uint32_t lab, int q, int forced) {
+    const float u = keyed_uniform_f32(u_bits);
+    if (forced) {
+        uint32_t step = (uint32_t)(u * (float)(uint32_t)(q - 1));
+        if (step >= (uint32_t)(q - 1)) step = (uint32_t)(q - 2);
+        return (lab + 1u + step) % (uint32_t)q;
+    }
+    uint32_t draw = (uint32_t)(u * (float)(uint32_t)q);
+    return draw >= (uint32_t)q ? (uint32_t)(q - 1) : draw;
+}
+
 __global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, const int32_t *__restrict__ parent,
-                                                  const uint8_t *__restrict__ newlab, int64_t N,
-                                                  long long *__restrict__ partials, int64_t span,
-                                                  int *__restrict__ nroots) {
+                                                  int64_t N, uint32_t root_seed, uint32_t seed_seed, uint32_t skey,
+                                                  uint32_t sweep, int q, int seeded, int forced, uint32_t cap,
+                                                  int *__restrict__ flag, long long *__restrict__ partials,
+                                                  int64_t span) {
+    __shared__ int32_t seed_root;
+    if (seeded) {
+        if (threadIdx.x == 0) {
+            // the seed site from the keyed draw, scaled in f32 and clamped
+            float u = keyed_uniform_f32(keyed_bits(seed_seed, 0u, sweep));
+            uint32_t ss = (uint32_t)(u * (float)(uint32_t)N);
+            if (ss >= (uint32_t)N) ss = (uint32_t)N - 1u;
+            uint32_t hops = 0;
+            const int32_t r = find_ro(parent, (int32_t)ss, cap, hops);
+            if (hops > cap) atomicOr(flag, 1);
+            seed_root = r;
+        }
+        __syncthreads();
+    }
+    const int32_t sroot = seeded ? seed_root : -1;
+    bool over = false;
+    // new label of a site of label lab in the cluster of root r (NO_LABEL: unchanged)
+    auto relabel = [&](int32_t r, uint32_t lab) -> uint32_t {
+        if (seeded && r != sroot) return NO_LABEL;
+        return relabel_draw(keyed_bits_k(root_seed, (uint32_t)r, skey), lab, q, forced);
+    };
     long long moved = 0;
     const int64_t nd = N >> 2;
     int64_t begin = (int64_t)blockIdx.x * span;
@@ -399,26 +551,52 @@ __global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, 
     if (end > nd) end = nd;
     uint32_t *w32 = reinterpret_cast<uint32_t *>(labels);
     const int4 *p4 = reinterpret_cast<const int4 *>(parent);
+    // The four walks of a dword go side by side, one level per step, so that
+    // their loads overlap.
     for (int64_t v = begin + threadIdx.x; v < end; v += blockDim.x) {
         const int4 p = p4[v];
-        const uint32_t n0 = newlab[p.x], n1 = newlab[p.y], n2 = newlab[p.z], n3 = newlab[p.w];
-        const uint32_t nw = n0 | (n1 << 8) | (n2 << 16) | (n3 << 24);
-        const uint32_t keep = (n0 == NO_LABEL ? 0xffu : 0u) | (n1 == NO_LABEL ? 0xff00u : 0u) |
-                              (n2 == NO_LABEL ? 0xff0000u : 0u) | (n3 == NO_LABEL ? 0xff000000u : 0u);
-        if (keep == 0xffffffffu) continue;
-        const uint32_t old = keep ? w32[v] : 0u;
-        w32[v] = (old & keep) | (nw & ~keep);
-        moved += 4 - (__popc(keep) >> 3);
+        const uint32_t w = w32[v];
+        int32_t r[4] = {p.x, p.y, p.z, p.w};
+        uint32_t hops = 1;
+        for (;;) {
+            int32_t nx[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) nx[k] = parent[r[k]];
+            bool up = false;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                up = up || nx[k] != r[k];
+                r[k] = nx[k];
+            }
+            if (!up || ++hops > cap) break;
+        }
+        over = over || hops > cap;
+        uint32_t out = w;
+        int cnt = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t nl = relabel(r[k], (w >> (8 * k)) & 0xffu);
+            if (nl != NO_LABEL) {
+                out = (out & ~(0xffu << (8 * k))) | (nl << (8 * k));
+                ++cnt;
+            }
+        }
+        if (cnt) w32[v] = out;
+        moved += cnt;
     }
     if (blockIdx.x == gridDim.x - 1) {
         for (int64_t s = (nd << 2) + threadIdx.x; s < N; s += blockDim.x) {
-            const uint8_t nl = newlab[parent[s]];
+            uint32_t hops = 1;
+            const int32_t r = find_ro(parent, parent[s], cap, hops);
+            over = over || hops > cap;
+            const uint32_t nl = relabel(r, labels[s]);
             if (nl != NO_LABEL) {
-                labels[s] = nl;
+                labels[s] = (uint8_t)nl;
                 moved += 1;
             }
         }
     }
+    if (over) atomicOr(flag, 1);
     __shared__ int wsum[BLK_WAVES][1];
     const int mv[1] = {(int)moved};
     const long long t = blk_sums<1>(mv, 1, wsum);
@@ -427,7 +605,6 @@ __global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, 
         dst[0] = 0;
         dst[1] = t;
         for (int k = 2; k < REC_K; ++k) dst[k] = 0;
-        if (blockIdx.x == gridDim.x - 1) *nroots = 0;
     }
 }
 
@@ -641,13 +818,16 @@ int64_t launch_cluster_sweep(const ClusterSweepArgs &a) {
     for (int mu = 0; mu < cg.D; ++mu)
         pow2 = pow2 && (cg.L[mu] & (cg.L[mu] - 1)) == 0 && (cg.B[mu] & (cg.B[mu] - 1)) == 0;
     dim3 gb((uint32_t)((faces + 255) / 256)), bb(256);
+    // sixteen sites per lane: whole 16-site segments of axis-0 rows, one per lane
+    const bool x16 = pow2 && cg.B[0] % 16 == 0 && cg.box_sites <= 16 * 1024 &&
+                     (reinterpret_cast<uintptr_t>(a.labels) & 15u) == 0;
 #define PLQ_CC(DD)                                                                                          \
-    if (pow2)                                                                                               \
-        cc_local<DD, true><<<gl, bl, lds, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t,    \
-                                                      a.rootlist, a.nroots);                                \
+    if (x16)                                                                                                \
+        cc_local_x16<DD><<<gl, bl, lds, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t);     \
+    else if (pow2)                                                                                          \
+        cc_local<DD, true><<<gl, bl, lds, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t);   \
     else                                                                                                    \
-        cc_local<DD, false><<<gl, bl, lds, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t,   \
-                                                       a.rootlist, a.nroots);                               \
+        cc_local<DD, false><<<gl, bl, lds, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t);  \
     if (faces > 0)                                                                                          \
         cc_boundary<DD><<<gb, bb, 0, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t, a.cap, \
                                                  a.flag)
@@ -660,13 +840,11 @@ int64_t launch_cluster_sweep(const ClusterSweepArgs &a) {
 #undef PLQ_CC
     int64_t need = (cg.N + 255) / 256;
     int64_t G = need < a.max_blocks ? need : a.max_blocks;
-    cc_flatten<<<dim3((uint32_t)G), dim3(256), 0, a.stream>>>(
-        a.labels, a.parent, a.rootlist, a.nroots, a.newlab, cg.N, a.seed32 ^ ROOT_SALT, a.seed32 ^ SEED_SALT, skey,
-        a.sweep, a.q, a.seeded, a.forced, a.cap, a.flag);
     const int64_t nd = cg.N >> 2;
     const int64_t span4 = nd > 0 ? (nd + G - 1) / G : 1;
-    cc_relabel<<<dim3((uint32_t)G), dim3(256), 0, a.stream>>>(a.labels, a.parent, a.newlab, cg.N, a.partials, span4,
-                                                              a.nroots);
+    cc_relabel<<<dim3((uint32_t)G), dim3(256), 0, a.stream>>>(a.labels, a.parent, cg.N, a.seed32 ^ ROOT_SALT,
+                                                              a.seed32 ^ SEED_SALT, skey, a.sweep, a.q, a.seeded,
+                                                              a.forced, a.cap, a.flag, a.partials, span4);
     return G;
 }
 

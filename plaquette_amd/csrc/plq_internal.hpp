@@ -212,9 +212,6 @@ struct ClusterSweepArgs {
     int forced;
     uint32_t cap;         // hop budget of every find / union
     int *flag;            // set to 1 when a budget is exceeded
-    int32_t *rootlist;    // N entries: the sweep's box-local roots
-    int *nroots;          // their count; zero at the start of a sweep (the relabel resets it)
-    uint8_t *newlab;      // N entries: new label of each local root's cluster
     long long *partials;  // REC_K per relabel block: [0, moved, counts...]
     int64_t max_blocks;
     hipStream_t stream;

@@ -303,6 +303,10 @@ CLUSTER_CASES = [
     ([1000], 3, 1.0, 2.0, False, False, 0, 10),
     ([256, 256], 3, 1.0, 1.0050525, False, False, 0, 4),   # power-of-two boxes, 4 of 128x128
     ([32, 32, 32], 2, 2.0, 0.2216546, True, True, 0, 6),   # power-of-two boxes, 2 of 32x32x16
+    ([16, 2048], 2, 2.0, 0.4406868, False, False, 0, 4),  # 16x1024 boxes: one segment per row, axis-0 wrap
+    ([512, 64], 3, 1.0, 1.0050525, True, True, 0, 6),     # 256x64 boxes spanning axis 1
+    ([16, 8, 8, 16], 3, 1.0, 0.9, False, False, 0, 5),    # one box spanning every axis
+    ([65536], 2, 2.0, 1.5, False, False, 0, 4),           # four 1D boxes
 ]
 
 
