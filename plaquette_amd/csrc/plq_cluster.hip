@@ -551,26 +551,34 @@ __global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, 
     if (end > nd) end = nd;
     uint32_t *w32 = reinterpret_cast<uint32_t *>(labels);
     const int4 *p4 = reinterpret_cast<const int4 *>(parent);
-    // The four walks of a dword go side by side, one level per step, so that
-    // their loads overlap.
+    // The walks of a dword go side by side, one level per step, so that their
+    // loads overlap.  A site whose parent equals its predecessor's takes that
+    // site's root at the end, and a walk stops loading once it has reached
+    // its root: the loads are what bounds this kernel.
     for (int64_t v = begin + threadIdx.x; v < end; v += blockDim.x) {
         const int4 p = p4[v];
         const uint32_t w = w32[v];
         int32_t r[4] = {p.x, p.y, p.z, p.w};
+        const uint32_t own = 1u | (p.y != p.x ? 2u : 0u) | (p.z != p.y ? 4u : 0u) | (p.w != p.z ? 8u : 0u);
+        uint32_t live = own;  // walks still climbing
         uint32_t hops = 1;
         for (;;) {
             int32_t nx[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) nx[k] = parent[r[k]];
-            bool up = false;
+            for (int k = 0; k < 4; ++k) nx[k] = ((live >> k) & 1u) ? parent[r[k]] : r[k];
+            uint32_t still = 0;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                up = up || nx[k] != r[k];
+                still |= (nx[k] != r[k] ? 1u : 0u) << k;
                 r[k] = nx[k];
             }
-            if (!up || ++hops > cap) break;
+            live = still;
+            if (!live || ++hops > cap) break;
         }
         over = over || hops > cap;
+#pragma unroll
+        for (int k = 1; k < 4; ++k)
+            if (!((own >> k) & 1u)) r[k] = r[k - 1];
         uint32_t out = w;
         int cnt = 0;
 #pragma unroll
