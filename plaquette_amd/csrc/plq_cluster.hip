@@ -719,33 +719,68 @@ __global__ void cc_resolve(int32_t *__restrict__ parent, int32_t *__restrict__ r
     roots[s] = r;
 }
 
-// Cluster sizes: each lane walks CC_RUN consecutive sites and adds one count
-// per run of equal roots, so a large cluster takes one atomic per run rather
-// than one per site.
-constexpr int CC_RUN = 64;
+// Cluster sizes.  A block takes CC_SPAN consecutive sites, a wave 64 at a
+// time (coalesced); each run of equal roots among a wave's 64 sites is one
+// add into an LDS table keyed by root (open addressing, linear probing), and
+// the table goes to the global counts with one atomic per root it holds.  A
+// cluster spanning many blocks so costs one global atomic per block instead
+// of one per run; a run whose probe finds the table crowded goes straight to
+// the global count.
+constexpr int CC_SLOTS = 4096;
+constexpr int64_t CC_SPAN = 16384;
+constexpr int CC_PROBES = 8;
 
-__global__ void cc_count(const int32_t *__restrict__ roots, int32_t *__restrict__ sizes, int64_t N) {
-    const int64_t s0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * CC_RUN;
-    if (s0 >= N) return;
-    const int64_t s1 = s0 + CC_RUN < N ? s0 + CC_RUN : N;
-    int32_t cur = roots[s0];
-    int run = 1;
-    for (int64_t s = s0 + 1; s < s1; ++s) {
-        const int32_t r = roots[s];
-        if (r == cur) {
-            ++run;
-        } else {
-            atomicAdd(&sizes[cur], run);
-            cur = r;
-            run = 1;
-        }
+__global__ __launch_bounds__(256) void cc_count(const int32_t *__restrict__ roots, int32_t *__restrict__ sizes,
+                                                int64_t N) {
+    __shared__ int32_t key[CC_SLOTS];
+    __shared__ int32_t cnt[CC_SLOTS];
+    for (int i = threadIdx.x; i < CC_SLOTS; i += blockDim.x) {
+        key[i] = -1;
+        cnt[i] = 0;
     }
-    atomicAdd(&sizes[cur], run);
+    __syncthreads();
+    const int64_t b0 = (int64_t)blockIdx.x * CC_SPAN;
+    const int64_t b1 = b0 + CC_SPAN < N ? b0 + CC_SPAN : N;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t above = lane == 63u ? 0ull : (~0ull << (lane + 1));
+    for (int64_t base = b0 + (threadIdx.x - lane); base < b1; base += blockDim.x) {
+        const int64_t s = base + lane;
+        const bool in = s < b1;
+        const int32_t r = in ? roots[s] : -1;
+        const int32_t prev = __shfl_up(r, 1, 64);
+        const bool head = in && (lane == 0u || prev != r);
+        // run length: up to the next head, or the end of the wave's sites
+        const uint64_t heads = __ballot(head) | (__ballot(!in) & ~0ull);
+        if (!head) continue;
+        const uint64_t later = heads & above;
+        const uint32_t stop = later ? (uint32_t)__ffsll((unsigned long long)later) - 1u : 64u;
+        const int run = (int)(stop - lane);
+        uint32_t h = ((uint32_t)r * 0x9e3779b1u) >> (32 - 12);
+        bool done = false;
+        for (int p = 0; p < CC_PROBES && !done; ++p, h = (h + 1u) & (CC_SLOTS - 1)) {
+            int32_t k = key[h];
+            if (k == -1) {
+                const int32_t old = atomicCAS(&key[h], -1, r);
+                k = old == -1 ? r : old;
+            }
+            if (k == r) {
+                atomicAdd(&cnt[h], run);
+                done = true;
+            }
+        }
+        if (!done) atomicAdd(&sizes[r], run);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < CC_SLOTS; i += blockDim.x)
+        if (key[i] != -1) atomicAdd(&sizes[key[i]], cnt[i]);
 }
 
-// hist[0..39] clusters by floor(log2 size); hist[40] clusters; hist[41] largest
-__global__ void cc_hist(const int32_t *__restrict__ roots, const int32_t *__restrict__ sizes, int64_t N,
-                        unsigned long long *__restrict__ hist) {
+// hist[0..39] clusters by floor(log2 size); hist[40] clusters; hist[41]
+// largest.  A grid-stride loop over the roots; per block, LDS bins, wave
+// sums for the count and a wave maximum for the largest size, then one
+// global atomic per bin.
+__global__ __launch_bounds__(256) void cc_hist(const int32_t *__restrict__ roots, const int32_t *__restrict__ sizes,
+                                               int64_t N, unsigned long long *__restrict__ hist) {
     __shared__ unsigned int h[40];
     __shared__ unsigned int nclu, largest;
     if (threadIdx.x < 40) h[threadIdx.x] = 0;
@@ -754,12 +789,24 @@ __global__ void cc_hist(const int32_t *__restrict__ roots, const int32_t *__rest
         largest = 0;
     }
     __syncthreads();
-    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < N && roots[s] == (int32_t)s) {
-        uint32_t sz = (uint32_t)sizes[s];
-        atomicAdd(&h[31 - __clz(sz)], 1u);
-        atomicAdd(&nclu, 1u);
-        atomicMax(&largest, sz);
+    unsigned int mine = 0, big = 0;
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < N; s += (int64_t)gridDim.x * blockDim.x) {
+        if (roots[s] == (int32_t)s) {
+            const uint32_t sz = (uint32_t)sizes[s];
+            atomicAdd(&h[31 - __clz(sz)], 1u);
+            ++mine;
+            big = sz > big ? sz : big;
+        }
+    }
+    const int tot = wave_sum_i32((int)mine);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const unsigned int other = (unsigned int)__shfl_xor((int)big, o, 64);
+        big = other > big ? other : big;
+    }
+    if ((threadIdx.x & 63u) == 0u) {
+        atomicAdd(&nclu, (unsigned int)tot);
+        atomicMax(&largest, big);
     }
     __syncthreads();
     if (threadIdx.x < 40 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
@@ -891,8 +938,9 @@ void launch_cluster_resolve(int32_t *parent, int32_t *roots, const CCGeom &cg, u
     dim3 g((uint32_t)((cg.N + 255) / 256)), b(256);
     cc_resolve<<<g, b, 0, s>>>(parent, roots, cg.N, cap, flag);
     if (sizes && hist) {
-        cc_count<<<dim3((uint32_t)((cg.N + 256LL * CC_RUN - 1) / (256LL * CC_RUN))), b, 0, s>>>(roots, sizes, cg.N);
-        cc_hist<<<g, b, 0, s>>>(roots, sizes, cg.N, hist);
+        cc_count<<<dim3((uint32_t)((cg.N + CC_SPAN - 1) / CC_SPAN)), b, 0, s>>>(roots, sizes, cg.N);
+        const int64_t hb = (cg.N + 255) / 256 < 2048 ? (cg.N + 255) / 256 : 2048;
+        cc_hist<<<dim3((uint32_t)hb), b, 0, s>>>(roots, sizes, cg.N, hist);
     }
 }
 

@@ -307,6 +307,7 @@ CLUSTER_CASES = [
     ([512, 64], 3, 1.0, 1.0050525, True, True, 0, 6),     # 256x64 boxes spanning axis 1
     ([16, 8, 8, 16], 3, 1.0, 0.9, False, False, 0, 5),    # one box spanning every axis
     ([65536], 2, 2.0, 1.5, False, False, 0, 4),           # four 1D boxes
+    ([256, 256], 4, 1.0, 0.01, False, False, 0, 2),       # nearly all singletons: the size table overflows
 ]
 
 
