@@ -16,10 +16,12 @@
 //   2. cc_boundary one lane per box-face site: draw the bonds that leave a
 //                 box and union their ends in the global forest with
 //                 compare-and-swap.
-//   3. cc_relabel  one lane per four sites: walk from the site's box-local
-//                 root to the cluster root, draw the cluster's new label
-//                 keyed on it (Redraw or ForcedChange; for Wolff only the
-//                 seed site's cluster) and store it.
+//   3. cc_compress one lane per box-face site again: point every local root
+//                 a face bond joined straight at its cluster root.
+//   4. cc_relabel  one lane per four sites: the cluster root is two loads
+//                 away; draw the cluster's new label keyed on it (Redraw or
+//                 ForcedChange; for Wolff only the seed site's cluster) and
+//                 store it.
 //
 // Every union links the larger root under the smaller one, so parent[x] <= x
 // always holds and the root of a tree is the minimum site index in it: the
@@ -393,13 +395,14 @@ __device__ __forceinline__ int32_t find_ro(const int32_t *__restrict__ P, int32_
     return x;
 }
 
+// Face site t of the box faces: along axis mu, the last coordinate of each
+// box (only where the box does not span the axis), each layer holding
+// N / L[mu] sites.  Sets the site, its forward neighbour across the face and
+// whether their bond is open; false past the last face site.
 template <int D>
-__global__ __launch_bounds__(256) void cc_boundary(const uint8_t *__restrict__ labels, int32_t *__restrict__ parent,
-                                                   CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t,
-                                                   uint32_t cap, int *__restrict__ flag) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    // Face layers: along axis mu, the last coordinate of each box (only when
-    // the box does not span the axis), each layer holding N / L[mu] sites.
+__device__ __forceinline__ bool face_bond(const uint8_t *__restrict__ labels, const CCGeom &cg, int64_t t,
+                                          uint32_t bond_seed, uint32_t skey, uint32_t bond_t, int64_t &gsite,
+                                          int64_t &gp) {
     int mu = 0;
     int64_t layer_sites = 0, count = 0;
     for (; mu < D; ++mu) {
@@ -409,41 +412,49 @@ __global__ __launch_bounds__(256) void cc_boundary(const uint8_t *__restrict__ l
         if (t < count) break;
         t -= count;
     }
-    bool open = false;
-    int32_t pa = -1, pb = -1;
-    int64_t gsite = 0, gp = 0;
-    if (mu < D) {
-        int32_t k = (int32_t)(t / layer_sites);
-        int64_t rem = t - (int64_t)k * layer_sites;  // index over the other axes
-        int32_t xm = (k + 1) * cg.B[mu];
-        if (xm > cg.L[mu]) xm = cg.L[mu];
-        xm -= 1;
-        // rebuild the site: rem enumerates the other axes in lattice order
+    if (mu >= D) return false;
+    int32_t k = (int32_t)(t / layer_sites);
+    int64_t rem = t - (int64_t)k * layer_sites;  // index over the other axes
+    int32_t xm = (k + 1) * cg.B[mu];
+    if (xm > cg.L[mu]) xm = cg.L[mu];
+    xm -= 1;
+    // rebuild the site: rem enumerates the other axes in lattice order
+    gsite = 0;
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-            if (a == mu) {
-                gsite += (int64_t)xm * cg.stride[a];
-            } else {
-                int32_t ya = (int32_t)(rem % cg.L[a]);
-                rem /= cg.L[a];
-                gsite += (int64_t)ya * cg.stride[a];
-            }
-        }
-        gp = (xm + 1 == cg.L[mu]) ? gsite - (int64_t)xm * cg.stride[mu] : gsite + cg.stride[mu];
-        open = labels[gsite] == labels[gp] && keyed_bits_k(bond_seed, (uint32_t)(gsite * D + mu), skey) < bond_t;
-        if (open) {
-            pa = load_relaxed(&parent[gsite]);
-            pb = load_relaxed(&parent[gp]);
+    for (int a = 0; a < D; ++a) {
+        if (a == mu) {
+            gsite += (int64_t)xm * cg.stride[a];
+        } else {
+            int32_t ya = (int32_t)(rem % cg.L[a]);
+            rem /= cg.L[a];
+            gsite += (int64_t)ya * cg.stride[a];
         }
     }
+    gp = (xm + 1 == cg.L[mu]) ? gsite - (int64_t)xm * cg.stride[mu] : gsite + cg.stride[mu];
+    return labels[gsite] == labels[gp] && keyed_bits_k(bond_seed, (uint32_t)(gsite * D + mu), skey) < bond_t;
+}
+
+// One lane per face site: the bonds that leave a box, joined in the global
+// forest.  The walks start from the two box-local roots (the sites' parents
+// as cc_local stored them), so a face site's own parent is never re-pointed
+// and stays its local root, which cc_compress relies on.
+template <int D>
+__global__ __launch_bounds__(256) void cc_boundary(const uint8_t *__restrict__ labels, int32_t *__restrict__ parent,
+                                                   CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t,
+                                                   uint32_t cap, int *__restrict__ flag) {
+    int64_t gsite = 0, gp = 0;
+    const bool open = face_bond<D>(labels, cg, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, bond_seed, skey,
+                                   bond_t, gsite, gp);
+    const int32_t pa = open ? load_relaxed(&parent[gsite]) : -1;
+    const int32_t pb = open ? load_relaxed(&parent[gp]) : -1;
     // Consecutive face sites mostly pair the same two box-local roots: a
     // bond whose ends had the same parents as the previous lane's open bond
     // joins the same two sets, which that lane's union already joins.
     const bool prev_open = __shfl_up(open ? 1 : 0, 1, 64) != 0;
     const int32_t prev_pa = __shfl_up(pa, 1, 64), prev_pb = __shfl_up(pb, 1, 64);
     if (!open || ((threadIdx.x & 63u) != 0u && prev_open && prev_pa == pa && prev_pb == pb)) return;
-    uint32_t hops = 0;
-    int32_t a = (int32_t)gsite, b = (int32_t)gp;
+    uint32_t hops = 1;  // the hop from each site to its local root
+    int32_t a = pa, b = pb;
     for (;;) {
         a = g_find(parent, a, cap, hops);
         b = g_find(parent, b, cap, hops);
@@ -461,6 +472,46 @@ __global__ __launch_bounds__(256) void cc_boundary(const uint8_t *__restrict__ l
         }
         a = old;
         b = lo;
+    }
+}
+
+// After cc_boundary: every box-local root that a face bond joined is pointed
+// straight at its cluster root, so that every site is two loads from its
+// cluster root (parent[parent[s]]).  Such a local root x first met the
+// forest through an open face bond with an end s whose parent was x (s = x,
+// or a site of x's component).  Each open face bond's lane reads the parents
+// of its two ends, walks to the root (read only: the forest no longer
+// changes, and a path split could store a non-root ancestor over another
+// lane's store) and stores the root into both ends and both parents.  Every
+// store writes the final root of that cluster.  Of the lanes storing into an
+// end s, the first to read reads s's original parent, so x is reached: for
+// s = x directly, otherwise through s's parent.
+template <int D>
+__global__ __launch_bounds__(256) void cc_compress(const uint8_t *__restrict__ labels, int32_t *__restrict__ parent,
+                                                   CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t,
+                                                   uint32_t cap, int *__restrict__ flag) {
+    int64_t gsite = 0, gp = 0;
+    if (!face_bond<D>(labels, cg, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, bond_seed, skey, bond_t, gsite,
+                      gp))
+        return;
+    const int32_t pa = load_relaxed(&parent[gsite]);
+    const int32_t pb = load_relaxed(&parent[gp]);
+    uint32_t hops = 1;
+    int32_t root = pa;
+    for (int32_t up = load_relaxed(&parent[root]); up != root; up = load_relaxed(&parent[root])) {
+        if (++hops > cap) {
+            atomicOr(flag, 1);
+            return;
+        }
+        root = up;
+    }
+    if (pa != root) {
+        store_relaxed(&parent[gsite], root);
+        store_relaxed(&parent[pa], root);
+    }
+    if (pb != root) {
+        store_relaxed(&parent[gp], root);
+        store_relaxed(&parent[pb], root);
     }
 }
 
@@ -490,15 +541,16 @@ __device__ __forceinline__ long long blk_sums(const int (&v)[NV], int nv, int (*
 
 // The new label of every site, four sites per lane (one dword of labels, one
 // int4 of parents).  A site's parent is its box-local root (cc_local stores
-// it) or, where cc_boundary split a path through the site, an ancestor of
-// that root; the walk from there ends at the cluster root, the minimum site of the cluster, on which the label draw is keyed
-// (cluster_relabel.wgsl:39-90): Redraw takes floor(u q), ForcedChange adds
-// 1 + floor(u (q - 1)) to the cluster's label (every site of a cluster holds
-// that label, since only agreeing sites are bonded).  Seeded (Wolff): only
-// the cluster of the seed site, drawn from the sweep's seed key
-// (cluster_relabel.wgsl:59-62), changes.  The hop budget counts the hop
-// from a site to its local root as well.  The last block also takes the
-// N mod 4 tail.  Partials per block: [0, relabelled sites, 0 ...].
+// it; cc_boundary and cc_compress re-point only local roots), and after
+// cc_compress the local root's parent is the cluster root, the minimum site
+// of the cluster, on which the label draw is keyed (cluster_relabel.wgsl:
+// 39-90): Redraw takes floor(u q), ForcedChange adds 1 + floor(u (q - 1)) to
+// the cluster's label (every site of a cluster holds that label, since only
+// agreeing sites are bonded).  Seeded (Wolff): only the cluster of the seed
+// site, drawn from the sweep's seed key (cluster_relabel.wgsl:59-62),
+// changes.  The hop budget counts the hop from a site to its local root and
+// the one above it.  The last block also takes the N mod 4 tail.  Partials
+// per block: [0, relabelled sites, 0 ...].
 constexpr uint32_t NO_LABEL = 0xffu;
 
 // Number of zero bytes of x (exact: no borrow between bytes).
@@ -551,34 +603,22 @@ __global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, 
     if (end > nd) end = nd;
     uint32_t *w32 = reinterpret_cast<uint32_t *>(labels);
     const int4 *p4 = reinterpret_cast<const int4 *>(parent);
-    // The walks of a dword go side by side, one level per step, so that their
-    // loads overlap.  A site whose parent equals its predecessor's takes that
-    // site's root at the end, and a walk stops loading once it has reached
-    // its root: the loads are what bounds this kernel.
+    // After cc_compress a site's cluster root is parent[parent[s]]: one
+    // gather per distinct parent in the dword.
     for (int64_t v = begin + threadIdx.x; v < end; v += blockDim.x) {
         const int4 p = p4[v];
         const uint32_t w = w32[v];
-        int32_t r[4] = {p.x, p.y, p.z, p.w};
-        const uint32_t own = 1u | (p.y != p.x ? 2u : 0u) | (p.z != p.y ? 4u : 0u) | (p.w != p.z ? 8u : 0u);
-        uint32_t live = own;  // walks still climbing
-        uint32_t hops = 1;
-        for (;;) {
-            int32_t nx[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) nx[k] = ((live >> k) & 1u) ? parent[r[k]] : r[k];
-            uint32_t still = 0;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                still |= (nx[k] != r[k] ? 1u : 0u) << k;
-                r[k] = nx[k];
-            }
-            live = still;
-            if (!live || ++hops > cap) break;
-        }
-        over = over || hops > cap;
-#pragma unroll
-        for (int k = 1; k < 4; ++k)
-            if (!((own >> k) & 1u)) r[k] = r[k - 1];
+        int32_t r[4];
+        r[0] = parent[p.x];
+        r[1] = p.y != p.x ? parent[p.y] : -1;
+        r[2] = p.z != p.y ? parent[p.z] : -1;
+        r[3] = p.w != p.z ? parent[p.w] : -1;
+        if (p.y == p.x) r[1] = r[0];
+        if (p.z == p.y) r[2] = r[1];
+        if (p.w == p.z) r[3] = r[2];
+        // the hop budget: one hop to the local root, one more above it
+        const bool deeper = r[0] != p.x || r[1] != p.y || r[2] != p.z || r[3] != p.w;
+        over = over || (deeper ? 2u : 1u) > cap;
         uint32_t out = w;
         int cnt = 0;
 #pragma unroll
@@ -594,9 +634,9 @@ __global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, 
     }
     if (blockIdx.x == gridDim.x - 1) {
         for (int64_t s = (nd << 2) + threadIdx.x; s < N; s += blockDim.x) {
-            uint32_t hops = 1;
-            const int32_t r = find_ro(parent, parent[s], cap, hops);
-            over = over || hops > cap;
+            const int32_t lr = parent[s];
+            const int32_t r = parent[lr];
+            over = over || (r != lr ? 2u : 1u) > cap;
             const uint32_t nl = relabel(r, labels[s]);
             if (nl != NO_LABEL) {
                 labels[s] = (uint8_t)nl;
@@ -609,6 +649,129 @@ __global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, 
     const int mv[1] = {(int)moved};
     const long long t = blk_sums<1>(mv, 1, wsum);
     if (threadIdx.x == 0) {
+        long long *dst = partials + (int64_t)blockIdx.x * REC_K;
+        dst[0] = 0;
+        dst[1] = t;
+        for (int k = 2; k < REC_K; ++k) dst[k] = 0;
+    }
+}
+
+// The relabel one box per workgroup, for the boxes cc_local_x16 labels
+// (power-of-two extents, box extent along axis 0 a multiple of 16): the
+// box's parents are loaded into LDS with coalesced 16-byte loads, and a
+// site's parent p is then either a site of the box, whose own parent (from
+// LDS) is the cluster root after cc_compress, or a cluster root outside the
+// box that cc_compress stored into a face site.  No global gathers.  Lane c
+// takes the 16 sites of one axis-0 row segment; consecutive sites with the
+// same parent share the draw.  A workgroup walks boxes blockIdx.x,
+// blockIdx.x + gridDim.x, ...  Partials per block: [0, relabelled sites,
+// 0 ...].
+template <int D>
+__global__ __launch_bounds__(1024, 8) void cc_relabel_box(uint8_t *__restrict__ labels,
+                                                       const int32_t *__restrict__ parent, CCGeom cg,
+                                                       uint32_t root_seed, uint32_t seed_seed, uint32_t skey,
+                                                       uint32_t sweep, int q, int seeded, int forced, uint32_t cap,
+                                                       int *__restrict__ flag, long long *__restrict__ partials) {
+    extern __shared__ int32_t P[];
+    __shared__ int32_t seed_root;
+    __shared__ int wsum[16];
+    uint32_t e[MAX_D], lsh[MAX_D], Lsh[MAX_D];
+    uint32_t lg = 0, Lg = 0;
+#pragma unroll
+    for (int mu = 0; mu < D; ++mu) {
+        e[mu] = (uint32_t)cg.B[mu];
+        lsh[mu] = lg;
+        lg += (uint32_t)(31 - __clz((int)e[mu]));
+        Lsh[mu] = Lg;
+        Lg += (uint32_t)(31 - __clz(cg.L[mu]));
+    }
+    const uint32_t nloc = 1u << lg;
+    if (seeded && threadIdx.x == 0) {
+        // the seed site from the keyed draw, scaled in f32 and clamped
+        const float u = keyed_uniform_f32(keyed_bits(seed_seed, 0u, sweep));
+        uint32_t ss = (uint32_t)(u * (float)(uint32_t)cg.N);
+        if (ss >= (uint32_t)cg.N) ss = (uint32_t)cg.N - 1u;
+        uint32_t hops = 0;
+        const int32_t r = find_ro(parent, (int32_t)ss, cap, hops);
+        if (hops > cap) atomicOr(flag, 1);
+        seed_root = r;
+    }
+    const uint32_t c = threadIdx.x;
+    const bool act = c < (nloc >> 4);
+    const uint32_t l0 = c << 4;
+    long long moved = 0;
+    bool over = false;
+    for (int64_t box = blockIdx.x; box < cg.nboxes; box += gridDim.x) {
+        uint32_t o[MAX_D];
+        uint32_t rest = (uint32_t)box;
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) {
+            const uint32_t nb = (uint32_t)cg.nbox[mu];
+            const uint32_t qt = rest / nb;
+            o[mu] = (rest - qt * nb) * e[mu];
+            rest = qt;
+        }
+        uint32_t gsite0 = 0;
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) gsite0 += (o[mu] + ((l0 >> lsh[mu]) & (e[mu] - 1u))) * (uint32_t)cg.stride[mu];
+        __syncthreads();  // the previous box's readers are done with P
+        if (act) {
+            const int4 *src = reinterpret_cast<const int4 *>(parent + gsite0);
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                const int4 v = src[d];
+                P[l0 + 4 * d] = v.x;
+                P[l0 + 4 * d + 1] = v.y;
+                P[l0 + 4 * d + 2] = v.z;
+                P[l0 + 4 * d + 3] = v.w;
+            }
+        }
+        __syncthreads();
+        if (act) {
+            uint4 *dst = reinterpret_cast<uint4 *>(labels + gsite0);
+            const uint4 l4 = *dst;
+            uint32_t w[4] = {l4.x, l4.y, l4.z, l4.w};
+            int32_t prev = -1;
+            uint32_t nl = NO_LABEL;
+            int cnt = 0;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int32_t p = P[l0 + (uint32_t)k];
+                if (p != prev) {
+                    prev = p;
+                    // p inside the box: its parent, from LDS; else p is a root
+                    uint32_t local = 0;
+                    bool inside = true;
+#pragma unroll
+                    for (int mu = 0; mu < D; ++mu) {
+                        const uint32_t y = (((uint32_t)p >> Lsh[mu]) & ((uint32_t)cg.L[mu] - 1u)) - o[mu];
+                        inside = inside && y < e[mu];
+                        local |= y << lsh[mu];
+                    }
+                    const int32_t root = inside ? P[local] : p;
+                    over = over || (root != p ? 2u : 1u) > cap;
+                    const uint32_t lab = (w[k >> 2] >> (8 * (k & 3))) & 0xffu;
+                    nl = (seeded && root != seed_root)
+                             ? NO_LABEL
+                             : relabel_draw(keyed_bits_k(root_seed, (uint32_t)root, skey), lab, q, forced);
+                }
+                if (nl != NO_LABEL) {
+                    const int sh = 8 * (k & 3);
+                    w[k >> 2] = (w[k >> 2] & ~(0xffu << sh)) | (nl << sh);
+                    ++cnt;
+                }
+            }
+            if (cnt) *dst = make_uint4(w[0], w[1], w[2], w[3]);
+            moved += cnt;
+        }
+    }
+    if (over) atomicOr(flag, 1);
+    const int wv = wave_sum_i32((int)moved);
+    if ((threadIdx.x & 63u) == 0u) wsum[threadIdx.x >> 6] = wv;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += wsum[i];
         long long *dst = partials + (int64_t)blockIdx.x * REC_K;
         dst[0] = 0;
         dst[1] = t;
@@ -883,9 +1046,12 @@ int64_t launch_cluster_sweep(const ClusterSweepArgs &a) {
         cc_local<DD, true><<<gl, bl, lds, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t);   \
     else                                                                                                    \
         cc_local<DD, false><<<gl, bl, lds, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t);  \
-    if (faces > 0)                                                                                          \
+    if (faces > 0) {                                                                                        \
         cc_boundary<DD><<<gb, bb, 0, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t, a.cap, \
-                                                 a.flag)
+                                                 a.flag);                                                   \
+        cc_compress<DD><<<gb, bb, 0, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t, a.cap, \
+                                                 a.flag);                                                   \
+    }
     switch (cg.D) {
         case 1: PLQ_CC(1); break;
         case 2: PLQ_CC(2); break;
@@ -895,6 +1061,21 @@ int64_t launch_cluster_sweep(const ClusterSweepArgs &a) {
 #undef PLQ_CC
     int64_t need = (cg.N + 255) / 256;
     int64_t G = need < a.max_blocks ? need : a.max_blocks;
+    if (x16) {
+        const int64_t Gb = cg.nboxes < a.max_blocks ? cg.nboxes : a.max_blocks;
+        switch (cg.D) {
+#define PLQ_RB(DD)                                                                                            \
+    cc_relabel_box<DD><<<dim3((uint32_t)Gb), bl, lds, a.stream>>>(a.labels, a.parent, cg, a.seed32 ^ ROOT_SALT, \
+                                                                a.seed32 ^ SEED_SALT, skey, a.sweep, a.q,     \
+                                                                a.seeded, a.forced, a.cap, a.flag, a.partials)
+            case 1: PLQ_RB(1); break;
+            case 2: PLQ_RB(2); break;
+            case 3: PLQ_RB(3); break;
+            default: PLQ_RB(4); break;
+#undef PLQ_RB
+        }
+        return Gb;
+    }
     const int64_t nd = cg.N >> 2;
     const int64_t span4 = nd > 0 ? (nd + G - 1) / G : 1;
     cc_relabel<<<dim3((uint32_t)G), dim3(256), 0, a.stream>>>(a.labels, a.parent, cg.N, a.seed32 ^ ROOT_SALT,
