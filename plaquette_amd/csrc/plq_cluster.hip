@@ -817,27 +817,41 @@ __global__ __launch_bounds__(256) void potts_measure_kernel(const uint8_t *__res
     if (threadIdx.x < (unsigned)REC_K) partials[(int64_t)blockIdx.x * REC_K + threadIdx.x] = t;
 }
 
-// The same sums four sites per lane, for L0 a multiple of 4: a dword of
-// labels never straddles a row; agreement along axis 0 compares each byte
-// with the next (the row's first byte after its last), along the other axes
-// with the dword one stride on.  Column-major partials: column k of block b
-// at part[k * gridDim.x + b], columns [agree, 0, counts[0..q)].
-template <int D>
+// The same sums 4 W sites per lane and step, for L0 a multiple of 4 W (and,
+// for W = 4, 16-byte aligned labels): a step's labels never straddle a row;
+// agreement along axis 0 compares each byte with the next (the row's first
+// byte after its last), along the other axes with the labels one stride on.
+// Column-major partials: column k of block b at part[k * gridDim.x + b],
+// columns [agree, 0, counts[0..q)].
+template <int D, int W>
 __global__ __launch_bounds__(256) void potts_measure_vec(const uint8_t *__restrict__ labels, CCGeom cg, int q,
                                                          long long *__restrict__ part, int64_t span) {
-    const uint32_t *w32 = reinterpret_cast<const uint32_t *>(labels);
-    const int64_t nd = cg.N >> 2;
+    constexpr uint32_t S = 4u * W;
+    const int64_t nu = cg.N / S;
     long long agree = 0;
     int counts[MAX_Q_DEV];
 #pragma unroll
     for (int a = 0; a < MAX_Q_DEV; ++a) counts[a] = 0;
     int64_t begin = (int64_t)blockIdx.x * span;
     int64_t end = begin + span;
-    if (end > nd) end = nd;
+    if (end > nu) end = nu;
     const uint32_t L0 = (uint32_t)cg.L[0];
+    auto load = [&](uint32_t site, uint32_t (&w)[W]) {
+        if constexpr (W == 4) {
+            const uint4 l4 = *reinterpret_cast<const uint4 *>(labels + site);
+            w[0] = l4.x;
+            w[1] = l4.y;
+            w[2] = l4.z;
+            w[3] = l4.w;
+        } else {
+#pragma unroll
+            for (int d = 0; d < W; ++d) w[d] = reinterpret_cast<const uint32_t *>(labels)[(site >> 2) + d];
+        }
+    };
     for (int64_t v = begin + threadIdx.x; v < end; v += blockDim.x) {
-        const uint32_t s0 = (uint32_t)v << 2;
-        const uint32_t w = w32[v];
+        const uint32_t s0 = (uint32_t)v * S;
+        uint32_t w[W];
+        load(s0, w);
         uint32_t rest = s0;
         uint32_t x[MAX_D];
 #pragma unroll
@@ -846,18 +860,28 @@ __global__ __launch_bounds__(256) void potts_measure_vec(const uint8_t *__restri
             x[mu] = mu + 1 < D ? rest - qt * (uint32_t)cg.L[mu] : rest;
             rest = qt;
         }
-        const uint32_t nb = x[0] + 4u == L0 ? labels[s0 - x[0]] : labels[s0 + 4u];
-        int a0 = zero_bytes(w ^ ((w >> 8) | (nb << 24)));
+        const uint32_t nb = x[0] + S == L0 ? labels[s0 - x[0]] : labels[s0 + S];
+        int a0 = 0;
+#pragma unroll
+        for (int d = 0; d < W; ++d) {
+            const uint32_t nxt = d + 1 < W ? w[d + 1 < W ? d + 1 : 0] : nb;
+            a0 += zero_bytes(w[d] ^ ((w[d] >> 8) | (nxt << 24)));
+        }
 #pragma unroll
         for (int mu = 1; mu < D; ++mu) {
             const uint32_t st = (uint32_t)cg.stride[mu];
             const uint32_t gp = (x[mu] + 1u == (uint32_t)cg.L[mu]) ? s0 - x[mu] * st : s0 + st;
-            a0 += zero_bytes(w ^ w32[gp >> 2]);
+            uint32_t u[W];
+            load(gp, u);
+#pragma unroll
+            for (int d = 0; d < W; ++d) a0 += zero_bytes(w[d] ^ u[d]);
         }
         agree += a0;
 #pragma unroll
         for (int a = 0; a < MAX_Q_DEV; ++a)
-            if (a < q) counts[a] += zero_bytes(w ^ (0x01010101u * (uint32_t)a));
+            if (a < q)
+#pragma unroll
+                for (int d = 0; d < W; ++d) counts[a] += zero_bytes(w[d] ^ (0x01010101u * (uint32_t)a));
     }
     const int64_t G = gridDim.x;
     __shared__ int wsum[BLK_WAVES][2 + MAX_Q_DEV];
@@ -1087,17 +1111,25 @@ int64_t launch_cluster_sweep(const ClusterSweepArgs &a) {
 void launch_potts_measure(const uint8_t *labels, const CCGeom &cg, int q, long long *partials, int64_t max_blocks,
                           long long *rec, hipStream_t s) {
     if (cg.L[0] % 4 == 0) {
-        const int64_t nd = cg.N >> 2;
-        int64_t need = (nd + 255) / 256;
+        // sixteen sites per lane and step where rows hold whole 16-byte vectors
+        const bool wide = cg.L[0] % 16 == 0 && (reinterpret_cast<uintptr_t>(labels) & 15u) == 0;
+        const int64_t nu = cg.N / (wide ? 16 : 4);
+        int64_t need = (nu + 255) / 256;
         int64_t G = need < max_blocks ? need : max_blocks;
-        int64_t span = (nd + G - 1) / G;
+        int64_t span = (nu + G - 1) / G;
         dim3 g((uint32_t)G), b(256);
+#define PLQ_PM(DD)                                                                           \
+    if (wide)                                                                                \
+        potts_measure_vec<DD, 4><<<g, b, 0, s>>>(labels, cg, q, partials, span);           \
+    else                                                                                     \
+        potts_measure_vec<DD, 1><<<g, b, 0, s>>>(labels, cg, q, partials, span)
         switch (cg.D) {
-            case 1: potts_measure_vec<1><<<g, b, 0, s>>>(labels, cg, q, partials, span); break;
-            case 2: potts_measure_vec<2><<<g, b, 0, s>>>(labels, cg, q, partials, span); break;
-            case 3: potts_measure_vec<3><<<g, b, 0, s>>>(labels, cg, q, partials, span); break;
-            default: potts_measure_vec<4><<<g, b, 0, s>>>(labels, cg, q, partials, span); break;
+            case 1: PLQ_PM(1); break;
+            case 2: PLQ_PM(2); break;
+            case 3: PLQ_PM(3); break;
+            default: PLQ_PM(4); break;
         }
+#undef PLQ_PM
         launch_finalize_columns(partials, G, nullptr, 0, 2 + q, rec, s);
         return;
     }
