@@ -8,20 +8,21 @@
 // reaching the same partition and the same root reproduces the reference's
 // labels bit for bit, so this file uses union-find instead:
 //
-//   1. cc_local   one block per box of the lattice: draw the box's internal
-//                 bonds (cluster_prelude.wgsl:61-80; bond = site*D + axis,
-//                 open iff the labels agree and m < T_p) and union their ends
-//                 in LDS, then store each site's box-local root as a global
+//   1. cc_local_x16 (power-of-two boxes) or cc_local: one workgroup per box
+//                 of the lattice: draw the box's internal bonds
+//                 (cluster_prelude.wgsl:61-80; bond = site*D + axis, open
+//                 iff the labels agree and m < T_p) and union their ends in
+//                 LDS, then store each site's box-local root as a global
 //                 site index.
 //   2. cc_boundary one lane per box-face site: draw the bonds that leave a
 //                 box and union their ends in the global forest with
 //                 compare-and-swap.
 //   3. cc_compress one lane per box-face site again: point every local root
 //                 a face bond joined straight at its cluster root.
-//   4. cc_relabel  one lane per four sites: the cluster root is two loads
-//                 away; draw the cluster's new label keyed on it (Redraw or
-//                 ForcedChange; for Wolff only the seed site's cluster) and
-//                 store it.
+//   4. cc_relabel_box (the boxes of cc_local_x16) or cc_relabel: the
+//                 cluster root is two loads away; draw the cluster's new
+//                 label keyed on it (Redraw or ForcedChange; for Wolff only
+//                 the seed site's cluster) and store it.
 //
 // Every union links the larger root under the smaller one, so parent[x] <= x
 // always holds and the root of a tree is the minimum site index in it: the
