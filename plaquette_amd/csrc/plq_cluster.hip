@@ -207,14 +207,47 @@ __device__ __forceinline__ uint32_t equal_bytes16(const uint4 &a, const uint4 &b
            (equal_bytes4(a.w, b.w) << 12);
 }
 
-// The same box labelling as cc_local, sixteen sites per lane: lane c owns
-// the box-local indices 16c .. 16c+15, one segment of an axis-0 row (every
-// extent a power of two, box extent along axis 0 a multiple of 16).
-//   1. Bond bits.  The lane reads its 16 labels and, per axis, the 16 labels
-//      one step up as one 16-byte load each; equal bytes and the keyed draws
-//      give a 16-bit mask of open bonds per axis.  P[16c + k] is the first
-//      site of k's run of open axis-0 bonds inside the segment, and the
-//      segment's axis-0 mask goes to H[c].
+// Geometry of the boxes cc_local_x16 labels (every extent a power of two,
+// box extent along axis 0 a multiple of 16): box origin, extents, and the
+// bit offsets of the local coordinates (lsh) and of the lattice ones (Lsh).
+template <int D>
+struct X16Box {
+    uint32_t o[MAX_D], e[MAX_D], lsh[MAX_D], Lsh[MAX_D];
+    uint32_t nloc;
+    __device__ __forceinline__ X16Box(const CCGeom &cg, uint32_t box) {
+        uint32_t lg = 0, Lg = 0, rest = box;
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) {
+            const uint32_t nb = (uint32_t)cg.nbox[mu];
+            const uint32_t q = rest / nb;
+            e[mu] = (uint32_t)cg.B[mu];
+            o[mu] = (rest - q * nb) * e[mu];
+            rest = q;
+            lsh[mu] = lg;
+            lg += (uint32_t)(31 - __clz((int)e[mu]));
+            Lsh[mu] = Lg;
+            Lg += (uint32_t)(31 - __clz(cg.L[mu]));
+        }
+        nloc = 1u << lg;
+    }
+    // global site of box-local index l
+    __device__ __forceinline__ uint32_t locate(const CCGeom &cg, uint32_t l) const {
+        uint32_t g = 0;
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) g += (o[mu] + ((l >> lsh[mu]) & (e[mu] - 1u))) * (uint32_t)cg.stride[mu];
+        return g;
+    }
+};
+
+// The box labelling of cc_local, sixteen sites per lane: lane c owns the
+// box-local indices 16c .. 16c+15, one segment of an axis-0 row.  `me` holds
+// the lane's 16 labels; up(l, g) returns the 16 labels at box-local index l
+// (global site g) of the row one step up, next(l, g) the label after the
+// segment.  All threads of the workgroup call it.
+//   1. Bond bits: equal bytes and the keyed draws give a 16-bit mask of open
+//      bonds per axis.  P[16c + k] is the first site of k's run of open
+//      axis-0 bonds inside the segment, and the segment's axis-0 mask goes
+//      to H[c].
 //   2. Unions, after a barrier, between run starts: the axis-0 bond out of
 //      the segment's last site, and each open bond of the other axes unless
 //      an earlier bond of the segment joins the same two runs (the runs of
@@ -225,46 +258,24 @@ __device__ __forceinline__ uint32_t equal_bytes16(const uint4 &a, const uint4 &b
 // the site index of a segment's first site times D is a multiple of 16 D,
 // so the bond index is that base or-ed with k D + axis, and the first
 // xor-shift of the hash is done once per segment.
-template <int D>
-__global__ __launch_bounds__(1024) void cc_local_x16(const uint8_t *__restrict__ labels, int32_t *__restrict__ parent,
-                                                     CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t) {
-    extern __shared__ int32_t P[];
-    __shared__ uint16_t H[1024];
-    uint32_t o[MAX_D], e[MAX_D], lsh[MAX_D];
-    uint32_t rest = blockIdx.x;
-    uint32_t lg = 0;
-#pragma unroll
-    for (int mu = 0; mu < D; ++mu) {
-        const uint32_t nb = (uint32_t)cg.nbox[mu];
-        const uint32_t q = rest / nb;  // once per block
-        const uint32_t b = rest - q * nb;
-        rest = q;
-        e[mu] = (uint32_t)cg.B[mu];
-        o[mu] = b * e[mu];
-        lsh[mu] = lg;
-        lg += (uint32_t)(31 - __clz((int)e[mu]));
-    }
-    const uint32_t nloc = 1u << lg;
-    auto locate = [&](uint32_t l) -> uint32_t {
-        uint32_t g = 0;
-#pragma unroll
-        for (int mu = 0; mu < D; ++mu) g += (o[mu] + ((l >> lsh[mu]) & (e[mu] - 1u))) * (uint32_t)cg.stride[mu];
-        return g;
-    };
+template <int D, class Up, class Next>
+__device__ __forceinline__ void x16_label(int32_t *P, uint16_t *H, int32_t *__restrict__ parent, const CCGeom &cg,
+                                          const X16Box<D> &bx, uint32_t bond_seed, uint32_t skey, uint32_t bond_t,
+                                          const uint4 &me, Up up, Next next) {
     const uint32_t c = threadIdx.x;
-    const bool act = c < (nloc >> 4);
+    const bool act = c < (bx.nloc >> 4);
     const uint32_t l0 = c << 4;
     uint32_t y[MAX_D];
     uint32_t gsite0 = 0;
 #pragma unroll
     for (int mu = 0; mu < D; ++mu) {
-        y[mu] = (l0 >> lsh[mu]) & (e[mu] - 1u);
-        gsite0 += (o[mu] + y[mu]) * (uint32_t)cg.stride[mu];
+        y[mu] = (l0 >> bx.lsh[mu]) & (bx.e[mu] - 1u);
+        gsite0 += (bx.o[mu] + y[mu]) * (uint32_t)cg.stride[mu];
     }
     const uint32_t kmix = sweep_mix(skey);
     constexpr bool XOR_BONDS = (D & (D - 1)) == 0;
-    const uint32_t bx = bond_seed ^ (gsite0 * (uint32_t)D);
-    const uint32_t v0 = bx ^ (bx >> 16);
+    const uint32_t sb = bond_seed ^ (gsite0 * (uint32_t)D);
+    const uint32_t v0 = sb ^ (sb >> 16);
     // the 16 bonds of axis mu: bit k set iff open (labels agree and m < t)
     auto draw16 = [&](uint32_t eq, int mu) -> uint32_t {
         uint32_t open = 0;
@@ -283,14 +294,13 @@ __global__ __launch_bounds__(1024) void cc_local_x16(const uint8_t *__restrict__
     bool x_out = false;
     uint32_t x_to = 0;
     if (act) {
-        const uint4 me = *reinterpret_cast<const uint4 *>(labels + gsite0);
         // axis 0: site k to k + 1; site 15 to the next segment (or the row's
         // first segment when the box spans axis 0), when that stays in the box
-        const bool x_in = y[0] + 16u < e[0];
-        const bool x_wrap = !x_in && e[0] == (uint32_t)cg.L[0];
+        const bool x_in = y[0] + 16u < bx.e[0];
+        const bool x_wrap = !x_in && bx.e[0] == (uint32_t)cg.L[0];
         x_out = x_in || x_wrap;
-        x_to = x_in ? l0 + 16u : l0 + 16u - e[0];
-        const uint32_t nb = x_out ? (uint32_t)labels[x_in ? gsite0 + 16u : gsite0 + 16u - (uint32_t)cg.L[0]] : 0u;
+        x_to = x_in ? l0 + 16u : l0 + 16u - bx.e[0];
+        const uint32_t nb = x_out ? next(x_to, x_in ? gsite0 + 16u : gsite0 + 16u - (uint32_t)cg.L[0]) : 0u;
         uint4 nx;
         nx.x = __builtin_amdgcn_alignbyte(me.y, me.x, 1);
         nx.y = __builtin_amdgcn_alignbyte(me.z, me.y, 1);
@@ -304,18 +314,19 @@ __global__ __launch_bounds__(1024) void cc_local_x16(const uint8_t *__restrict__
         for (int mu = 1; mu < D; ++mu) {
             vmask[mu] = 0;
             uint32_t gn;
-            if (y[mu] + 1u < e[mu]) {
-                nbr_l0[mu] = l0 + (1u << lsh[mu]);
+            if (y[mu] + 1u < bx.e[mu]) {
+                nbr_l0[mu] = l0 + (1u << bx.lsh[mu]);
                 gn = gsite0 + (uint32_t)cg.stride[mu];
-            } else if (e[mu] == (uint32_t)cg.L[mu]) {  // the box spans the axis: the wrap bond is internal
-                nbr_l0[mu] = l0 - ((e[mu] - 1u) << lsh[mu]);
+            } else if (bx.e[mu] == (uint32_t)cg.L[mu]) {  // the box spans the axis: the wrap bond is internal
+                nbr_l0[mu] = l0 - ((bx.e[mu] - 1u) << bx.lsh[mu]);
                 gn = gsite0 - (uint32_t)(cg.L[mu] - 1) * (uint32_t)cg.stride[mu];
             } else {
                 continue;  // bonds leaving the box (cc_boundary)
             }
-            const uint4 up = *reinterpret_cast<const uint4 *>(labels + gn);
-            vmask[mu] = draw16(equal_bytes16(me, up), mu);
+            vmask[mu] = draw16(equal_bytes16(me, up(nbr_l0[mu], gn)), mu);
         }
+    }
+    if (act) {
         H[c] = (uint16_t)(h & 0x7fffu);
         int32_t run = (int32_t)l0;
 #pragma unroll
@@ -358,7 +369,7 @@ __global__ __launch_bounds__(1024) void cc_local_x16(const uint8_t *__restrict__
         for (int k = 0; k < 16; ++k) {
             if ((starts >> k) & 1u) {
                 const uint32_t r = (uint32_t)lds_find(P, (int32_t)(l0 + (uint32_t)k));
-                g = (int32_t)(r == l0 + (uint32_t)k ? gsite0 + (uint32_t)k : locate(r));
+                g = (int32_t)(r == l0 + (uint32_t)k ? gsite0 + (uint32_t)k : bx.locate(cg, r));
             }
             out[k] = g;
         }
@@ -368,6 +379,22 @@ __global__ __launch_bounds__(1024) void cc_local_x16(const uint8_t *__restrict__
         dst[2] = make_int4(out[8], out[9], out[10], out[11]);
         dst[3] = make_int4(out[12], out[13], out[14], out[15]);
     }
+}
+
+// cc_local for power-of-two boxes whose extent along axis 0 is a multiple of
+// 16: x16_label with the labels read from global memory.
+template <int D>
+__global__ __launch_bounds__(1024) void cc_local_x16(const uint8_t *__restrict__ labels, int32_t *__restrict__ parent,
+                                                     CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t) {
+    extern __shared__ int32_t P[];
+    __shared__ uint16_t H[1024];
+    const X16Box<D> bx(cg, blockIdx.x);
+    uint4 me = make_uint4(0u, 0u, 0u, 0u);
+    if (threadIdx.x < (bx.nloc >> 4)) me = *reinterpret_cast<const uint4 *>(labels + bx.locate(cg, threadIdx.x << 4));
+    x16_label<D>(
+        P, H, parent, cg, bx, bond_seed, skey, bond_t, me,
+        [&](uint32_t, uint32_t g) { return *reinterpret_cast<const uint4 *>(labels + g); },
+        [&](uint32_t, uint32_t g) { return (uint32_t)labels[g]; });
 }
 
 // ------------------------------------------------------------- boundary ---
@@ -560,6 +587,20 @@ __device__ __forceinline__ int zero_bytes(uint32_t x) {
     return __popc(~(t | x | 0x7f7f7f7fu));
 }
 
+// The seed site's cluster root (Wolff), for the relabel kernels: the seed
+// site from the keyed draw, scaled in f32 and clamped
+// (cluster_relabel.wgsl:59-62).
+__device__ __forceinline__ int32_t seed_cluster_root(const int32_t *__restrict__ parent, int64_t N, uint32_t seed_seed,
+                                                     uint32_t sweep, uint32_t cap, int *__restrict__ flag) {
+    const float u = keyed_uniform_f32(keyed_bits(seed_seed, 0u, sweep));
+    uint32_t ss = (uint32_t)(u * (float)(uint32_t)N);
+    if (ss >= (uint32_t)N) ss = (uint32_t)N - 1u;
+    uint32_t hops = 0;
+    const int32_t r = find_ro(parent, (int32_t)ss, cap, hops);
+    if (hops > cap) atomicOr(flag, 1);
+    return r;
+}
+
 __device__ __forceinline__ uint32_t relabel_draw(uint32_t u_bits, uint32_t lab, int q, int forced) {
     const float u = keyed_uniform_f32(u_bits);
     if (forced) {
@@ -578,16 +619,7 @@ __global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, 
                                                   int64_t span) {
     __shared__ int32_t seed_root;
     if (seeded) {
-        if (threadIdx.x == 0) {
-            // the seed site from the keyed draw, scaled in f32 and clamped
-            float u = keyed_uniform_f32(keyed_bits(seed_seed, 0u, sweep));
-            uint32_t ss = (uint32_t)(u * (float)(uint32_t)N);
-            if (ss >= (uint32_t)N) ss = (uint32_t)N - 1u;
-            uint32_t hops = 0;
-            const int32_t r = find_ro(parent, (int32_t)ss, cap, hops);
-            if (hops > cap) atomicOr(flag, 1);
-            seed_root = r;
-        }
+        if (threadIdx.x == 0) seed_root = seed_cluster_root(parent, N, seed_seed, sweep, cap, flag);
         __syncthreads();
     }
     const int32_t sroot = seeded ? seed_root : -1;
@@ -657,116 +689,67 @@ __global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, 
     }
 }
 
-// The relabel one box per workgroup, for the boxes cc_local_x16 labels
-// (power-of-two extents, box extent along axis 0 a multiple of 16): the
-// box's parents are loaded into LDS with coalesced 16-byte loads, and a
-// site's parent p is then either a site of the box, whose own parent (from
-// LDS) is the cluster root after cc_compress, or a cluster root outside the
-// box that cc_compress stored into a face site.  No global gathers.  Lane c
-// takes the 16 sites of one axis-0 row segment; consecutive sites with the
-// same parent share the draw.  A workgroup walks boxes blockIdx.x,
-// blockIdx.x + gridDim.x, ...  Partials per block: [0, relabelled sites,
-// 0 ...].
+// Relabel of one box, for the boxes cc_local_x16 labels: the box's parents
+// are loaded into LDS (P) with coalesced 16-byte loads, and a site's parent
+// p is then either a site of the box, whose own parent (from LDS) is the
+// cluster root after cc_compress, or a cluster root outside the box that
+// cc_compress stored into a face site.  No global gathers.  Lane c takes the
+// 16 sites of one axis-0 row segment (its labels in w, updated in place);
+// consecutive sites with the same parent share the draw.  Returns the
+// relabelled sites; the caller synchronises before P is reused.
 template <int D>
-__global__ __launch_bounds__(1024, 8) void cc_relabel_box(uint8_t *__restrict__ labels,
-                                                       const int32_t *__restrict__ parent, CCGeom cg,
-                                                       uint32_t root_seed, uint32_t seed_seed, uint32_t skey,
-                                                       uint32_t sweep, int q, int seeded, int forced, uint32_t cap,
-                                                       int *__restrict__ flag, long long *__restrict__ partials) {
-    extern __shared__ int32_t P[];
-    __shared__ int32_t seed_root;
-    __shared__ int wsum[16];
-    uint32_t e[MAX_D], lsh[MAX_D], Lsh[MAX_D];
-    uint32_t lg = 0, Lg = 0;
+__device__ __forceinline__ void box_load_parents(int32_t *P, const int32_t *__restrict__ parent, uint32_t l0,
+                                                 uint32_t gsite0) {
+    const int4 *src = reinterpret_cast<const int4 *>(parent + gsite0);
 #pragma unroll
-    for (int mu = 0; mu < D; ++mu) {
-        e[mu] = (uint32_t)cg.B[mu];
-        lsh[mu] = lg;
-        lg += (uint32_t)(31 - __clz((int)e[mu]));
-        Lsh[mu] = Lg;
-        Lg += (uint32_t)(31 - __clz(cg.L[mu]));
+    for (int d = 0; d < 4; ++d) {
+        const int4 v = src[d];
+        P[l0 + 4 * d] = v.x;
+        P[l0 + 4 * d + 1] = v.y;
+        P[l0 + 4 * d + 2] = v.z;
+        P[l0 + 4 * d + 3] = v.w;
     }
-    const uint32_t nloc = 1u << lg;
-    if (seeded && threadIdx.x == 0) {
-        // the seed site from the keyed draw, scaled in f32 and clamped
-        const float u = keyed_uniform_f32(keyed_bits(seed_seed, 0u, sweep));
-        uint32_t ss = (uint32_t)(u * (float)(uint32_t)cg.N);
-        if (ss >= (uint32_t)cg.N) ss = (uint32_t)cg.N - 1u;
-        uint32_t hops = 0;
-        const int32_t r = find_ro(parent, (int32_t)ss, cap, hops);
-        if (hops > cap) atomicOr(flag, 1);
-        seed_root = r;
-    }
-    const uint32_t c = threadIdx.x;
-    const bool act = c < (nloc >> 4);
-    const uint32_t l0 = c << 4;
-    long long moved = 0;
-    bool over = false;
-    for (int64_t box = blockIdx.x; box < cg.nboxes; box += gridDim.x) {
-        uint32_t o[MAX_D];
-        uint32_t rest = (uint32_t)box;
+}
+
+template <int D>
+__device__ __forceinline__ int box_relabel16(const int32_t *P, const CCGeom &cg, const X16Box<D> &bx, uint32_t l0,
+                                             uint32_t (&w)[4], uint32_t root_seed, uint32_t skey, int q, int seeded,
+                                             int forced, int32_t seed_root, uint32_t cap, bool &over) {
+    int32_t prev = -1;
+    uint32_t nl = NO_LABEL;
+    int cnt = 0;
 #pragma unroll
-        for (int mu = 0; mu < D; ++mu) {
-            const uint32_t nb = (uint32_t)cg.nbox[mu];
-            const uint32_t qt = rest / nb;
-            o[mu] = (rest - qt * nb) * e[mu];
-            rest = qt;
-        }
-        uint32_t gsite0 = 0;
+    for (int k = 0; k < 16; ++k) {
+        const int32_t p = P[l0 + (uint32_t)k];
+        if (p != prev) {
+            prev = p;
+            // p inside the box: its parent, from LDS; else p is a root
+            uint32_t local = 0;
+            bool inside = true;
 #pragma unroll
-        for (int mu = 0; mu < D; ++mu) gsite0 += (o[mu] + ((l0 >> lsh[mu]) & (e[mu] - 1u))) * (uint32_t)cg.stride[mu];
-        __syncthreads();  // the previous box's readers are done with P
-        if (act) {
-            const int4 *src = reinterpret_cast<const int4 *>(parent + gsite0);
-#pragma unroll
-            for (int d = 0; d < 4; ++d) {
-                const int4 v = src[d];
-                P[l0 + 4 * d] = v.x;
-                P[l0 + 4 * d + 1] = v.y;
-                P[l0 + 4 * d + 2] = v.z;
-                P[l0 + 4 * d + 3] = v.w;
+            for (int mu = 0; mu < D; ++mu) {
+                const uint32_t y = (((uint32_t)p >> bx.Lsh[mu]) & ((uint32_t)cg.L[mu] - 1u)) - bx.o[mu];
+                inside = inside && y < bx.e[mu];
+                local |= y << bx.lsh[mu];
             }
+            const int32_t root = inside ? P[local] : p;
+            over = over || (root != p ? 2u : 1u) > cap;
+            const uint32_t lab = (w[k >> 2] >> (8 * (k & 3))) & 0xffu;
+            nl = (seeded && root != seed_root) ? NO_LABEL
+                                               : relabel_draw(keyed_bits_k(root_seed, (uint32_t)root, skey), lab, q, forced);
         }
-        __syncthreads();
-        if (act) {
-            uint4 *dst = reinterpret_cast<uint4 *>(labels + gsite0);
-            const uint4 l4 = *dst;
-            uint32_t w[4] = {l4.x, l4.y, l4.z, l4.w};
-            int32_t prev = -1;
-            uint32_t nl = NO_LABEL;
-            int cnt = 0;
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const int32_t p = P[l0 + (uint32_t)k];
-                if (p != prev) {
-                    prev = p;
-                    // p inside the box: its parent, from LDS; else p is a root
-                    uint32_t local = 0;
-                    bool inside = true;
-#pragma unroll
-                    for (int mu = 0; mu < D; ++mu) {
-                        const uint32_t y = (((uint32_t)p >> Lsh[mu]) & ((uint32_t)cg.L[mu] - 1u)) - o[mu];
-                        inside = inside && y < e[mu];
-                        local |= y << lsh[mu];
-                    }
-                    const int32_t root = inside ? P[local] : p;
-                    over = over || (root != p ? 2u : 1u) > cap;
-                    const uint32_t lab = (w[k >> 2] >> (8 * (k & 3))) & 0xffu;
-                    nl = (seeded && root != seed_root)
-                             ? NO_LABEL
-                             : relabel_draw(keyed_bits_k(root_seed, (uint32_t)root, skey), lab, q, forced);
-                }
-                if (nl != NO_LABEL) {
-                    const int sh = 8 * (k & 3);
-                    w[k >> 2] = (w[k >> 2] & ~(0xffu << sh)) | (nl << sh);
-                    ++cnt;
-                }
-            }
-            if (cnt) *dst = make_uint4(w[0], w[1], w[2], w[3]);
-            moved += cnt;
+        if (nl != NO_LABEL) {
+            const int sh = 8 * (k & 3);
+            w[k >> 2] = (w[k >> 2] & ~(0xffu << sh)) | (nl << sh);
+            ++cnt;
         }
     }
-    if (over) atomicOr(flag, 1);
+    return cnt;
+}
+
+
+// The moved count of the workgroup into its partials row [0, moved, 0 ...].
+__device__ __forceinline__ void box_partials(long long moved, int *wsum, long long *__restrict__ partials) {
     const int wv = wave_sum_i32((int)moved);
     if ((threadIdx.x & 63u) == 0u) wsum[threadIdx.x >> 6] = wv;
     __syncthreads();
@@ -779,6 +762,44 @@ __global__ __launch_bounds__(1024, 8) void cc_relabel_box(uint8_t *__restrict__ 
         for (int k = 2; k < REC_K; ++k) dst[k] = 0;
     }
 }
+
+// The relabel one box per workgroup (box_relabel16); a workgroup walks boxes
+// blockIdx.x, blockIdx.x + gridDim.x, ...  Partials per block: [0,
+// relabelled sites, 0 ...].
+template <int D>
+__global__ __launch_bounds__(1024, 8) void cc_relabel_box(uint8_t *__restrict__ labels,
+                                                       const int32_t *__restrict__ parent, CCGeom cg,
+                                                       uint32_t root_seed, uint32_t seed_seed, uint32_t skey,
+                                                       uint32_t sweep, int q, int seeded, int forced, uint32_t cap,
+                                                       int *__restrict__ flag, long long *__restrict__ partials) {
+    extern __shared__ int32_t P[];
+    __shared__ int32_t seed_root;
+    __shared__ int wsum[16];
+    if (seeded && threadIdx.x == 0) seed_root = seed_cluster_root(parent, cg.N, seed_seed, sweep, cap, flag);
+    const uint32_t c = threadIdx.x;
+    const uint32_t l0 = c << 4;
+    long long moved = 0;
+    bool over = false;
+    for (int64_t box = blockIdx.x; box < cg.nboxes; box += gridDim.x) {
+        const X16Box<D> bx(cg, (uint32_t)box);
+        const bool act = c < (bx.nloc >> 4);
+        const uint32_t gsite0 = bx.locate(cg, l0);
+        __syncthreads();  // the previous box's readers are done with P
+        if (act) box_load_parents<D>(P, parent, l0, gsite0);
+        __syncthreads();
+        if (act) {
+            uint4 *dst = reinterpret_cast<uint4 *>(labels + gsite0);
+            const uint4 l4 = *dst;
+            uint32_t w[4] = {l4.x, l4.y, l4.z, l4.w};
+            const int cnt = box_relabel16<D>(P, cg, bx, l0, w, root_seed, skey, q, seeded, forced, seed_root, cap, over);
+            if (cnt) *dst = make_uint4(w[0], w[1], w[2], w[3]);
+            moved += cnt;
+        }
+    }
+    if (over) atomicOr(flag, 1);
+    box_partials(moved, wsum, partials);
+}
+
 
 // --------------------------------------------------------- measurement ---
 

@@ -340,6 +340,35 @@ def test_cluster_sweeps_are_bit_exact(gpu, case):
     c.close()
 
 
+MULTI_SWEEP_CASES = [c for c in CLUSTER_CASES if all(e & (e - 1) == 0 for e in c[0]) and np.prod(c[0]) > 8192]
+
+
+@pytest.mark.parametrize("case", MULTI_SWEEP_CASES)
+def test_cluster_advance_of_many_sweeps_is_bit_exact(gpu, case):
+    # One advance of several sweeps fuses each sweep's relabel with the next
+    # sweep's box labelling; every recorded sweep must still match the
+    # oracle's sweep by sweep, and the partition of the last sweep its roots.
+    shape, q, gap, beta, seeded, forced, profile, sweeps = case
+    n = int(np.prod(shape))
+    start = start_config(n, q, True, 13)
+    extent = gpu.Extent.SEEDED if seeded else gpu.Extent.ALL
+    relabel = gpu.Relabel.FORCED_CHANGE if forced else gpu.Relabel.REDRAW
+    c = gpu.GpuClusterChain.new(shape, q, gap, extent, relabel, beta, 37, start, profile=profile)
+    d = O.cluster_desc(shape, q=q, seeded=seeded, forced=forced, profile=profile, beta=beta, gap=gap, seed=37)
+    k = sweeps + 3
+    c.record(1, k)
+    c.advance(k)
+    series = c.series()
+    st = start
+    for t in range(k):
+        st, roots = O.cluster_sweeps(d, st, t, 1)
+        agree, counts = O.potts_sums(shape, q, st)
+        assert series[t].bond_sum == agree and list(series[t].counts[:q]) == list(counts), f"sweep {t}"
+    assert np.array_equal(c.read(), st)
+    assert np.array_equal(c.roots(), roots)
+    c.close()
+
+
 def test_cluster_budget_exhaustion_is_reported(gpu):
     # gpu_cluster.rs:512-518: labelling past its cap is an error, forced from
     # the host with a zero hop budget on an ordered (percolating) lattice
