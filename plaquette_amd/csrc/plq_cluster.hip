@@ -462,6 +462,8 @@ __device__ __forceinline__ bool face_bond(const uint8_t *__restrict__ labels, co
     return labels[gsite] == labels[gp] && keyed_bits_k(bond_seed, (uint32_t)(gsite * D + mu), skey) < bond_t;
 }
 
+constexpr int BND_SLOTS = 512;  // LDS pair set of a 256-lane boundary workgroup
+
 // One lane per face site: the bonds that leave a box, joined in the global
 // forest.  The walks start from the two box-local roots (the sites' parents
 // as cc_local stored them), so a face site's own parent is never re-pointed
@@ -470,17 +472,26 @@ template <int D>
 __global__ __launch_bounds__(256) void cc_boundary(const uint8_t *__restrict__ labels, int32_t *__restrict__ parent,
                                                    CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t,
                                                    uint32_t cap, int *__restrict__ flag) {
+    // The workgroup's distinct (parent, parent) pairs: face sites along a
+    // face mostly pair the same two box-local roots, and a bond whose ends
+    // have the same parents as another open bond joins the same two sets.
+    // Only the lane that first enters a pair in this LDS set walks and links.
+    __shared__ unsigned long long pairs[BND_SLOTS];
+    for (int i = threadIdx.x; i < BND_SLOTS; i += blockDim.x) pairs[i] = ~0ull;
+    __syncthreads();
     int64_t gsite = 0, gp = 0;
     const bool open = face_bond<D>(labels, cg, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, bond_seed, skey,
                                    bond_t, gsite, gp);
-    const int32_t pa = open ? load_relaxed(&parent[gsite]) : -1;
-    const int32_t pb = open ? load_relaxed(&parent[gp]) : -1;
-    // Consecutive face sites mostly pair the same two box-local roots: a
-    // bond whose ends had the same parents as the previous lane's open bond
-    // joins the same two sets, which that lane's union already joins.
-    const bool prev_open = __shfl_up(open ? 1 : 0, 1, 64) != 0;
-    const int32_t prev_pa = __shfl_up(pa, 1, 64), prev_pb = __shfl_up(pb, 1, 64);
-    if (!open || ((threadIdx.x & 63u) != 0u && prev_open && prev_pa == pa && prev_pb == pb)) return;
+    if (!open) return;
+    const int32_t pa = load_relaxed(&parent[gsite]);
+    const int32_t pb = load_relaxed(&parent[gp]);
+    const unsigned long long key = ((unsigned long long)(uint32_t)pa << 32) | (uint32_t)pb;
+    uint32_t h = (uint32_t)((key * 0x9e3779b97f4a7c15ull) >> 55) & (BND_SLOTS - 1);
+    for (int probe = 0; probe < 8; ++probe, h = (h + 1u) & (BND_SLOTS - 1)) {
+        const unsigned long long old = atomicCAS(&pairs[h], ~0ull, key);
+        if (old == ~0ull) break;  // first: this lane links the pair
+        if (old == key) return;   // another lane links it
+    }                             // (a crowded probe links it again: harmless)
     uint32_t hops = 1;  // the hop from each site to its local root
     int32_t a = pa, b = pb;
     for (;;) {
