@@ -122,6 +122,7 @@ __global__ __launch_bounds__(CCS_THREADS) void cluster_small_kernel(SmallCluster
         //    from a wave's last lane to the next wave and the row's wrap bond
         //    are joined in phase 2 with the other axes' bonds.
         uint32_t edge = 0;  // bit k: lane 63's axis-0 bond to the next wave is open
+        int32_t rstart[CCS_K];  // the first site of each site's run: the unions start there, one hop up
 #pragma unroll
         for (int k = 0; k < CCS_K; ++k) {
             const uint32_t l = tid + (uint32_t)k * CCS_THREADS;
@@ -130,7 +131,8 @@ __global__ __launch_bounds__(CCS_THREADS) void cluster_small_kernel(SmallCluster
             const uint64_t m = __ballot(open);
             const uint64_t starts = (~m << 1) | 1ull;  // bit j: lane j begins a run
             const uint32_t first = 63u - (uint32_t)__clzll(starts & below);
-            if (l < N) P[l] = (int32_t)(l - lane + first);
+            rstart[k] = (int32_t)(l - lane + first);
+            if (l < N) P[l] = rstart[k];
             if (open && lane == 63u) edge |= 1u << k;
         }
         __syncthreads();
@@ -139,14 +141,14 @@ __global__ __launch_bounds__(CCS_THREADS) void cluster_small_kernel(SmallCluster
         for (int k = 0; k < CCS_K; ++k) {
             const uint32_t l = tid + (uint32_t)k * CCS_THREADS;
             if (l >= N) continue;
-            if (edge & (1u << k)) ccs_union(P, (int32_t)l, (int32_t)(l + 1));
+            if (edge & (1u << k)) ccs_union(P, rstart[k], (int32_t)(l + 1));
             const uint8_t s = lab[l];
 #pragma unroll
             for (int mu = 0; mu < D; ++mu) {
                 const uint32_t lp = fwd(k, mu);
                 if (mu == 0 && (lp == l + 1 || lp == l)) continue;  // inside the row (phase 1), or an extent of 1
                 if (lab[lp] == s && keyed_bits_k(a.bond_seed, l * (uint32_t)D + (uint32_t)mu, skey) < a.bond_t)
-                    ccs_union(P, (int32_t)l, (int32_t)lp);
+                    ccs_union(P, rstart[k], (int32_t)lp);
             }
         }
         __syncthreads();
