@@ -900,29 +900,35 @@ int plq_chain_cluster_stats(plq_chain *c, plq_cluster_stats *out) {
     if (!c->have_partition) return fail(PLQ_EINVAL, "no cluster sweep has run since the last write");
     std::lock_guard<std::mutex> lk(c->mu);
     PLQ_HIP(hipSetDevice(c->device));
-    if (!c->roots) PLQ_TRY(alloc((void **)&c->roots, sizeof(int32_t) * (size_t)c->n_sites));
     if (!c->sizes) PLQ_TRY(alloc((void **)&c->sizes, sizeof(int32_t) * (size_t)c->n_sites));
-    if (!c->hist) PLQ_TRY(alloc((void **)&c->hist, sizeof(unsigned long long) * 42));
-    PLQ_HIP(hipMemsetAsync(c->sizes, 0, sizeof(int32_t) * (size_t)c->n_sites, c->stream));
+    // hist[0..41]: the histogram; hist[42 ..]: the last relabel's partials summed on the device
+    constexpr size_t HN = 42 + plq::REC_K;
+    if (!c->hist) PLQ_TRY(alloc((void **)&c->hist, sizeof(unsigned long long) * HN));
     PLQ_HIP(hipMemsetAsync(c->hist, 0, sizeof(unsigned long long) * 42, c->stream));
-    uint32_t cap = (uint32_t)(c->pass_cap > 0xffffffffull ? 0xffffffffull : c->pass_cap);
-    plq::launch_cluster_resolve(c->parent, c->roots, c->cg, cap, c->d_flag, c->sizes, c->hist, c->stream);
-    unsigned long long h[42];
+    if (plq::cluster_boxes_x16(c->cg, c->labels)) {
+        // sizes counted per box from the finished forest; ext stays zero between calls
+        if (!c->ext) {
+            PLQ_TRY(alloc((void **)&c->ext, sizeof(int32_t) * (size_t)c->n_sites));
+            PLQ_HIP(hipMemsetAsync(c->ext, 0, sizeof(int32_t) * (size_t)c->n_sites, c->stream));
+        }
+        plq::launch_cluster_sizes_box(c->parent, c->cg, c->sizes, c->ext, c->hist, plq::MAX_PASS_BLOCKS, c->stream);
+    } else {
+        if (!c->roots) PLQ_TRY(alloc((void **)&c->roots, sizeof(int32_t) * (size_t)c->n_sites));
+        PLQ_HIP(hipMemsetAsync(c->sizes, 0, sizeof(int32_t) * (size_t)c->n_sites, c->stream));
+        uint32_t cap = (uint32_t)(c->pass_cap > 0xffffffffull ? 0xffffffffull : c->pass_cap);
+        plq::launch_cluster_resolve(c->parent, c->roots, c->cg, cap, c->d_flag, c->sizes, c->hist, c->stream);
+    }
+    plq::launch_finalize_record(c->d_partials + plq::MAX_PASS_BLOCKS * plq::REC_K, c->last_relabel_blocks, nullptr,
+                                0, reinterpret_cast<long long *>(c->hist + 42), c->stream);
+    unsigned long long h[HN];
     PLQ_HIP(hipMemcpyAsync(h, c->hist, sizeof h, hipMemcpyDeviceToHost, c->stream));
-    long long part[plq::REC_K];
-    std::vector<long long> parts((size_t)c->last_relabel_blocks * plq::REC_K);
-    PLQ_HIP(hipMemcpyAsync(parts.data(), c->d_partials + plq::MAX_PASS_BLOCKS * plq::REC_K,
-                           sizeof(long long) * parts.size(), hipMemcpyDeviceToHost, c->stream));
     PLQ_HIP(hipStreamSynchronize(c->stream));
-    std::memset(part, 0, sizeof part);
-    for (int64_t b = 0; b < c->last_relabel_blocks; ++b)
-        for (int k = 0; k < plq::REC_K; ++k) part[k] += parts[(size_t)b * plq::REC_K + k];
     std::memset(out, 0, sizeof *out);
     out->sweep = c->sweeps_done;
     for (int k = 0; k < 40; ++k) out->size_hist[k] = (int64_t)h[k];
     out->n_clusters = (int64_t)h[40];
     out->largest = (int64_t)h[41];
-    out->moved = part[1];
+    out->moved = (int64_t)h[42 + 1];
     return check_flag(c);
 }
 

@@ -43,6 +43,7 @@ struct plq_chain {
     int32_t *parent = nullptr;
     int32_t *roots = nullptr;
     int32_t *sizes = nullptr;
+    int32_t *ext = nullptr;                // cluster counts from other boxes (box-counted sizes), zero between calls
     unsigned long long *hist = nullptr;
     int *d_flag = nullptr;
     uint32_t bond_t = 0;
@@ -81,7 +82,8 @@ struct plq_chain {
         (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto p : {(void *)col[0], (void *)col[1], (void *)d_links, (void *)d_table, (void *)labels,
-                       (void *)parent, (void *)roots, (void *)sizes, (void *)hist, (void *)d_flag, (void *)d_lat,
+                       (void *)parent, (void *)roots, (void *)sizes, (void *)ext, (void *)hist, (void *)d_flag,
+                       (void *)d_lat,
                        (void *)d_partials, (void *)d_meas, (void *)d_rec, (void *)d_block_counters,
                        (void *)d_block_logits, (void *)d_pm_cls, (void *)d_pm_state, (void *)d_pm_jt, (void *)d_pm_blk,
                        (void *)d_pm_off, (void *)d_meas_bits, (void *)d_meas_counts, (void *)d_batch})

@@ -1036,6 +1036,198 @@ __global__ __launch_bounds__(256) void cc_hist(const int32_t *__restrict__ roots
     }
 }
 
+// Cluster sizes for the boxes cc_local_x16 labels, from the forest of a
+// finished sweep (every site two loads from its cluster root: the box's
+// parents in LDS as in cc_relabel_box).  Per box, the members of each run of
+// equal roots in a lane's 16 sites are added at once: to an LDS count at the
+// root's slot when the root lies in the box, else to an LDS table of outside
+// roots that goes to ext[root] in global memory at the end of the box.  Every root's own site lies in its box, so each root's box stores
+// its count in sizes[root]; a cluster's size is sizes[root] + ext[root].
+// A workgroup walks boxes blockIdx.x, blockIdx.x + gridDim.x, ...
+// LDS table of the roots outside a box: 8 sub-tables of 128 slots, a lane
+// using sub-table lane & 7, so a root met by many lanes (the percolating
+// cluster's) spreads its atomics over 8 slots
+constexpr int SZ_SUB = 128, SZ_SLOTS = 8 * SZ_SUB;
+
+template <int D>
+__global__ __launch_bounds__(1024, 8) void cc_sizes_box(const int32_t *__restrict__ parent, CCGeom cg,
+                                                     int32_t *__restrict__ sizes, int32_t *__restrict__ ext) {
+    extern __shared__ int32_t P[];
+    // counts of the roots outside the box (a percolating cluster's root is
+    // outside almost every box): one global atomic per root and box
+    __shared__ int32_t okey[SZ_SLOTS], ocnt[SZ_SLOTS];
+    const uint32_t c = threadIdx.x;
+    const uint32_t l0 = c << 4;
+    for (int i = threadIdx.x; i < SZ_SLOTS; i += blockDim.x) {
+        okey[i] = -1;
+        ocnt[i] = 0;
+    }
+    for (int64_t box = blockIdx.x; box < cg.nboxes; box += gridDim.x) {
+        const X16Box<D> bx(cg, (uint32_t)box);
+        const bool act = c < (bx.nloc >> 4);
+        const uint32_t gsite0 = bx.locate(cg, l0);
+        // box-local slot of global site g, or ~0u outside the box
+        auto slot = [&](int32_t g) -> uint32_t {
+            uint32_t local = 0;
+            bool inside = true;
+#pragma unroll
+            for (int mu = 0; mu < D; ++mu) {
+                const uint32_t y = (((uint32_t)g >> bx.Lsh[mu]) & ((uint32_t)cg.L[mu] - 1u)) - bx.o[mu];
+                inside = inside && y < bx.e[mu];
+                local |= y << bx.lsh[mu];
+            }
+            return inside ? local : ~0u;
+        };
+        __syncthreads();  // the previous box's counts are out
+        if (act) box_load_parents<D>(P, parent, l0, gsite0);
+        __syncthreads();
+        int32_t root[16];
+        if (act) {
+            int32_t prev = -1, r = -1;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int32_t p = P[l0 + (uint32_t)k];
+                if (p != prev) {
+                    prev = p;
+                    const uint32_t sl = slot(p);
+                    r = sl != ~0u ? P[sl] : p;
+                }
+                root[k] = r;
+            }
+        }
+        __syncthreads();  // every read of the parents is done: P becomes the counts
+        if (act) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) P[l0 + (uint32_t)k] = 0;
+        }
+        __syncthreads();
+        if (act) {
+            int len = 1;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                if (k == 15 || root[k + 1] != root[k]) {
+                    const int32_t r = root[k];
+                    const uint32_t sl = slot(r);
+                    if (sl != ~0u) {
+                        atomicAdd(&P[sl], len);
+                    } else {
+                        const uint32_t sub = (threadIdx.x & 7u) * SZ_SUB;
+                        uint32_t h = ((uint32_t)r * 0x9e3779b1u) >> (32 - 7);
+                        bool done = false;
+                        for (int probe = 0; probe < 8 && !done; ++probe, h = (h + 1u) & (SZ_SUB - 1)) {
+                            int32_t key = okey[sub + h];
+                            if (key == -1) {
+                                const int32_t old = atomicCAS(&okey[sub + h], -1, r);
+                                key = old == -1 ? r : old;
+                            }
+                            if (key == r) {
+                                atomicAdd(&ocnt[sub + h], len);
+                                done = true;
+                            }
+                        }
+                        if (!done) atomicAdd(&ext[r], len);  // table crowded: straight to the global count
+                    }
+                    len = 1;
+                } else {
+                    ++len;
+                }
+            }
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int32_t n = P[l0 + (uint32_t)k];
+                if (n) sizes[gsite0 + (uint32_t)k] = n;
+            }
+        }
+        for (int i = threadIdx.x; i < SZ_SLOTS; i += blockDim.x) {
+            if (okey[i] != -1) {
+                atomicAdd(&ext[okey[i]], ocnt[i]);
+                okey[i] = -1;
+                ocnt[i] = 0;
+            }
+        }
+    }
+}
+
+// The size histogram from cc_sizes_box: every root (parent[s] == s after a
+// finished sweep) has size sizes[s] + ext[s]; ext[s] is cleared for the next
+// call (only roots ever receive ext counts).  hist[0..39] by floor(log2
+// size), hist[40] clusters, hist[41] largest.  The eight smallest bins are
+// counted per lane and summed per wave, away from the LDS atomics.
+__global__ __launch_bounds__(256) void cc_hist_roots(const int32_t *__restrict__ parent,
+                                                     const int32_t *__restrict__ sizes, int32_t *__restrict__ ext,
+                                                     int64_t N, unsigned long long *__restrict__ hist) {
+    constexpr int NB = 8;  // bins counted in registers (sizes below 256, most clusters)
+    __shared__ unsigned int h[40];
+    __shared__ unsigned int nclu, largest;
+    if (threadIdx.x < 40) h[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        nclu = 0;
+        largest = 0;
+    }
+    __syncthreads();
+    int small[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) small[k] = 0;
+    unsigned int mine = 0, big = 0;
+    auto count = [&](uint32_t sz) {
+        const int bin = 31 - __clz(sz);
+        if (bin < NB) {
+#pragma unroll
+            for (int k = 0; k < NB; ++k) small[k] += bin == k ? 1 : 0;
+        } else {
+            atomicAdd(&h[bin], 1u);
+        }
+        ++mine;
+        big = sz > big ? sz : big;
+    };
+    // four sites per lane and step, the three arrays loaded side by side
+    const int64_t n4 = N >> 2;
+    const int4 *p4 = reinterpret_cast<const int4 *>(parent);
+    const int4 *s4 = reinterpret_cast<const int4 *>(sizes);
+    int4 *e4 = reinterpret_cast<int4 *>(ext);
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n4; v += (int64_t)gridDim.x * blockDim.x) {
+        const int4 p = p4[v], z = s4[v], x = e4[v];
+        const int32_t s0 = (int32_t)(v << 2);
+        if (x.x | x.y | x.z | x.w) e4[v] = make_int4(0, 0, 0, 0);
+        if (p.x == s0) count((uint32_t)(z.x + x.x));
+        if (p.y == s0 + 1) count((uint32_t)(z.y + x.y));
+        if (p.z == s0 + 2) count((uint32_t)(z.z + x.z));
+        if (p.w == s0 + 3) count((uint32_t)(z.w + x.w));
+    }
+    for (int64_t s = (n4 << 2) + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < N;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        if (parent[s] == (int32_t)s) {
+            const int32_t x = ext[s];
+            if (x) ext[s] = 0;
+            count((uint32_t)(sizes[s] + x));
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const int t = wave_sum_i32(small[k]);
+        if ((threadIdx.x & 63u) == 0u && t) atomicAdd(&h[k], (unsigned int)t);
+    }
+    const int tot = wave_sum_i32((int)mine);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const unsigned int other = (unsigned int)__shfl_xor((int)big, o, 64);
+        big = other > big ? other : big;
+    }
+    if ((threadIdx.x & 63u) == 0u) {
+        atomicAdd(&nclu, (unsigned int)tot);
+        atomicMax(&largest, big);
+    }
+    __syncthreads();
+    if (threadIdx.x < 40 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+    if (threadIdx.x == 0) {
+        atomicAdd(&hist[40], (unsigned long long)nclu);
+        atomicMax(&hist[41], (unsigned long long)largest);
+    }
+}
+
 // ------------------------------------------------------------ launchers ---
 
 // Boxes: start from the whole lattice and halve the longest box edge until a
@@ -1094,8 +1286,7 @@ int64_t launch_cluster_sweep(const ClusterSweepArgs &a) {
         pow2 = pow2 && (cg.L[mu] & (cg.L[mu] - 1)) == 0 && (cg.B[mu] & (cg.B[mu] - 1)) == 0;
     dim3 gb((uint32_t)((faces + 255) / 256)), bb(256);
     // sixteen sites per lane: whole 16-site segments of axis-0 rows, one per lane
-    const bool x16 = pow2 && cg.B[0] % 16 == 0 && cg.box_sites <= 16 * 1024 &&
-                     (reinterpret_cast<uintptr_t>(a.labels) & 15u) == 0;
+    const bool x16 = cluster_boxes_x16(cg, a.labels);
 #define PLQ_CC(DD)                                                                                          \
     if (x16)                                                                                                \
         cc_local_x16<DD><<<gl, bl, lds, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t);     \
@@ -1177,6 +1368,27 @@ void launch_potts_measure(const uint8_t *labels, const CCGeom &cg, int q, long l
         default: potts_measure_kernel<4><<<g, b, 0, s>>>(labels, cg, q, partials, span); break;
     }
     launch_finalize_record(partials, G, nullptr, 0, rec, s);
+}
+
+bool cluster_boxes_x16(const CCGeom &cg, const void *labels) {
+    bool pow2 = true;
+    for (int mu = 0; mu < cg.D; ++mu)
+        pow2 = pow2 && (cg.L[mu] & (cg.L[mu] - 1)) == 0 && (cg.B[mu] & (cg.B[mu] - 1)) == 0;
+    return pow2 && cg.B[0] % 16 == 0 && cg.box_sites <= 16 * 1024 && (reinterpret_cast<uintptr_t>(labels) & 15u) == 0;
+}
+
+void launch_cluster_sizes_box(const int32_t *parent, const CCGeom &cg, int32_t *sizes, int32_t *ext,
+                              unsigned long long *hist, int64_t max_blocks, hipStream_t s) {
+    const int64_t Gb = cg.nboxes < max_blocks ? cg.nboxes : max_blocks;
+    const size_t lds = (size_t)cg.box_sites * sizeof(int32_t);
+    switch (cg.D) {
+        case 1: cc_sizes_box<1><<<dim3((uint32_t)Gb), dim3(1024), lds, s>>>(parent, cg, sizes, ext); break;
+        case 2: cc_sizes_box<2><<<dim3((uint32_t)Gb), dim3(1024), lds, s>>>(parent, cg, sizes, ext); break;
+        case 3: cc_sizes_box<3><<<dim3((uint32_t)Gb), dim3(1024), lds, s>>>(parent, cg, sizes, ext); break;
+        default: cc_sizes_box<4><<<dim3((uint32_t)Gb), dim3(1024), lds, s>>>(parent, cg, sizes, ext); break;
+    }
+    const int64_t hb = (cg.N + 255) / 256 < 2048 ? (cg.N + 255) / 256 : 2048;
+    cc_hist_roots<<<dim3((uint32_t)hb), dim3(256), 0, s>>>(parent, sizes, ext, cg.N, hist);
 }
 
 void launch_cluster_resolve(int32_t *parent, int32_t *roots, const CCGeom &cg, uint32_t cap, int *flag,

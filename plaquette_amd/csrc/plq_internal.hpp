@@ -224,6 +224,13 @@ void launch_potts_measure(const uint8_t *labels, const CCGeom &cg, int q, long l
 // Canonical roots of the current forest, and (when sizes/hist are non-null)
 // cluster sizes and the size histogram (hist[0..39] by floor(log2 size),
 // hist[40] cluster count, hist[41] largest size).
+// Whether the sweep uses the 16-sites-per-lane box kernels (power-of-two
+// boxes, box extent along axis 0 a multiple of 16, 16-byte aligned labels).
+bool cluster_boxes_x16(const CCGeom &cg, const void *labels);
+// Size histogram of the last sweep's clusters on those boxes (hist zeroed by
+// the caller; ext all zero on entry and on return).
+void launch_cluster_sizes_box(const int32_t *parent, const CCGeom &cg, int32_t *sizes, int32_t *ext,
+                              unsigned long long *hist, int64_t max_blocks, hipStream_t s);
 void launch_cluster_resolve(int32_t *parent, int32_t *roots, const CCGeom &cg, uint32_t cap, int *flag,
                             int32_t *sizes, unsigned long long *hist, hipStream_t s);
 

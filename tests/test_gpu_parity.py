@@ -369,6 +369,33 @@ def test_cluster_advance_of_many_sweeps_is_bit_exact(gpu, case):
     c.close()
 
 
+@pytest.mark.parametrize("shape,q,seeded", [([256, 256], 3, False), ([32, 32, 32], 2, True), ([16, 2048], 2, False),
+                                            ([90, 91], 3, False)])
+def test_cluster_stats_after_every_sweep(gpu, shape, q, seeded):
+    # The size histogram taken after every sweep (box-counted where the box
+    # kernels apply, with the cross-box counts cleared between calls) equals
+    # the oracle's partition of that sweep.
+    n = int(np.prod(shape))
+    beta = 1.0050525 if q == 3 else 0.4406868
+    start = start_config(n, q, True, 21)
+    extent = gpu.Extent.SEEDED if seeded else gpu.Extent.ALL
+    relabel = gpu.Relabel.FORCED_CHANGE if seeded else gpu.Relabel.REDRAW
+    c = gpu.GpuClusterChain.new(shape, q, 1.0, extent, relabel, beta, 41, start)
+    d = O.cluster_desc(shape, q=q, seeded=seeded, forced=seeded, beta=beta, gap=1.0, seed=41)
+    st = start
+    for t in range(4):
+        c.advance(1)
+        st, roots = O.cluster_sweeps(d, st, t, 1)
+        for _ in range(2):  # twice: the second call sees the cleared counts
+            stats = c.stats()
+            sizes = np.bincount(roots)
+            sizes = sizes[sizes > 0]
+            assert stats.n_clusters == sizes.size and stats.largest == sizes.max(), t
+            hist = np.bincount(np.floor(np.log2(sizes)).astype(int), minlength=40)
+            assert list(stats.size_hist) == list(hist), t
+    c.close()
+
+
 def test_cluster_budget_exhaustion_is_reported(gpu):
     # gpu_cluster.rs:512-518: labelling past its cap is an error, forced from
     # the host with a zero hop budget on an ordered (percolating) lattice
