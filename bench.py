@@ -226,8 +226,8 @@ def timed(P, ev, stream_of, run):
 def bench_potts_sw(P, ev, steps, warmup):
     """2D q=3 Potts 4096^2 Swendsen-Wang at beta_c = ln(1 + sqrt 3); order
     parameter (label counts) recorded every sweep; cluster-size histogram of
-    the last sweep, and separately the rate with the histogram taken after
-    every sweep.  10 B algorithmic per site and sweep (labels read and
+    the last sweep, and separately the rate with the histogram of every
+    sweep kept on the device.  10 B algorithmic per site and sweep (labels read and
     written, 4-byte cluster id written and read)."""
     import math
 
@@ -246,14 +246,17 @@ def bench_potts_sw(P, ev, steps, warmup):
     t0 = time.perf_counter()
     stats = c.stats()
     stats_s = time.perf_counter() - t0
-    # the configuration's full per-sweep work: a sweep, then the cluster-size
-    # histogram of its partition (plq_chain_cluster_stats), every sweep
-    hist_steps = 50
-    t0 = time.perf_counter()
-    for _ in range(hist_steps):
-        c.advance(1)
-        c.stats()
-    hist_wall = time.perf_counter() - t0
+    # the configuration's full per-sweep work: a sweep, the order parameter
+    # and the cluster-size histogram of its partition, every sweep, kept on
+    # the device (plq_chain_record_clusters)
+    hist_steps = 100
+    c.record(1, hist_steps)
+    c.record_clusters(True)
+    c.advance(2)
+    c.sync()
+    c.record(1, hist_steps)
+    hist_wall, _ = timed(P, ev, c.stream, lambda: (c.advance(hist_steps), c.sync()))
+    hist_series = c.cluster_series()
     c.close()
     last = series[-1]
     counts = list(last.counts[:3])
@@ -271,7 +274,9 @@ def bench_potts_sw(P, ev, steps, warmup):
         "last_energy_per_site": last.energy / n, "last_order_parameter": order,
         "clusters": stats.n_clusters, "largest_cluster": stats.largest, "stats_ms": round(stats_s * 1e3, 2),
         "histogram_every_sweep": {"sweeps": hist_steps, "us_per_sweep": round(hist_wall * 1e6 / hist_steps, 1),
-                                  "site_updates_per_ns": round(hist_steps * n / (hist_wall * 1e9), 2)},
+                                  "site_updates_per_ns": round(hist_steps * n / (hist_wall * 1e9), 2),
+                                  "histograms_kept": len(hist_series),
+                                  "last_clusters": hist_series[-1].n_clusters if hist_series else None},
     }
 
 

@@ -160,6 +160,15 @@ int plq_chain_stream(plq_chain *chain, void **stream);
  * cluster root, the cluster's minimum site index) and its statistics. */
 int plq_chain_cluster_roots(plq_chain *chain, int64_t *roots, size_t n);
 int plq_chain_cluster_stats(plq_chain *chain, plq_cluster_stats *out);
+/* Cluster chains only: with recording on (plq_chain_record), also keep the
+ * statistics of each recorded sweep's partition (as plq_chain_cluster_stats
+ * gives them) in a device ring beside the records, read back by
+ * plq_chain_cluster_series without a synchronisation per sweep.  No
+ * counterpart in the reference, which computes cluster statistics on the host
+ * from read-back configurations.  While on, the chain stays on the
+ * four-kernel sweep.  Calling it discards the kept statistics. */
+int plq_chain_record_clusters(plq_chain *chain, int on);
+int plq_chain_cluster_series(plq_chain *chain, plq_cluster_stats *out, uint64_t capacity, uint64_t *n_out);
 /* Cluster chains only: lower the labelling pass cap (gpu_cluster.rs:620-625);
  * used to force PLQ_ENOCONV from the host. */
 int plq_chain_set_pass_cap(plq_chain *chain, uint64_t cap);

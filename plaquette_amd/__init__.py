@@ -180,6 +180,8 @@ SIGNATURES = {
     "plq_chain_cluster_roots": (C.c_int, [_HANDLE, _P(C.c_int64), C.c_size_t]),
     "plq_chain_cluster_stats": (C.c_int, [_HANDLE, _P(ClusterStats)]),
     "plq_chain_set_pass_cap": (C.c_int, [_HANDLE, C.c_uint64]),
+    "plq_chain_record_clusters": (C.c_int, [_HANDLE, C.c_int]),
+    "plq_chain_cluster_series": (C.c_int, [_HANDLE, C.c_void_p, C.c_uint64, _P(C.c_uint64)]),
     "plq_chain_destroy": (C.c_int, [_HANDLE]),
     "plq_chain_pcg_state": (C.c_int, [_HANDLE, _P(C.c_uint64)]),
     "plq_chain_set_pcg_state": (C.c_int, [_HANDLE, _P(C.c_uint64)]),
@@ -572,6 +574,20 @@ class GpuClusterChain(_Chain):
 
     def set_pass_cap(self, cap: int) -> None:
         _check(lib().plq_chain_set_pass_cap(self._h, int(cap)))
+
+    def record_clusters(self, on: bool = True) -> None:
+        """Keep each recorded sweep's cluster statistics on the device (with
+        record(); read them with cluster_series())."""
+        _check(lib().plq_chain_record_clusters(self._h, 1 if on else 0))
+
+    def cluster_series(self) -> list:
+        n = C.c_uint64()
+        _check(lib().plq_chain_cluster_series(self._h, None, 0, C.byref(n)))
+        if n.value == 0:
+            return []
+        buf = (ClusterStats * n.value)()
+        _check(lib().plq_chain_cluster_series(self._h, C.cast(buf, C.c_void_p), n.value, C.byref(n)))
+        return list(buf)
 
 
 # -- model constructors (the reference's per-model gpu_chain functions) -------

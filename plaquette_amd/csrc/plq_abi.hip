@@ -217,6 +217,41 @@ void enqueue_measure(plq_chain *c, long long *rec) {
     }
 }
 
+// Statistics of the last sweep's partition into dst (CLU_SLOT entries):
+// dst[0..39] clusters by floor(log2 size), dst[40] clusters, dst[41] the
+// largest size, dst[42 ..] the last relabel's partials summed ([0, moved,
+// 0 ...]).  Enqueue only; allocates the work buffers on first use.
+int enqueue_cluster_stats(plq_chain *c, unsigned long long *dst) {
+    if (!c->sizes) PLQ_TRY(alloc((void **)&c->sizes, sizeof(int32_t) * (size_t)c->n_sites));
+    PLQ_HIP(hipMemsetAsync(dst, 0, sizeof(unsigned long long) * 42, c->stream));
+    if (plq::cluster_boxes_x16(c->cg, c->labels)) {
+        // sizes counted per box from the finished forest; ext stays zero between calls
+        if (!c->ext) {
+            PLQ_TRY(alloc((void **)&c->ext, sizeof(int32_t) * (size_t)c->n_sites));
+            PLQ_HIP(hipMemsetAsync(c->ext, 0, sizeof(int32_t) * (size_t)c->n_sites, c->stream));
+        }
+        plq::launch_cluster_sizes_box(c->parent, c->cg, c->sizes, c->ext, dst, plq::MAX_PASS_BLOCKS, c->stream);
+    } else {
+        if (!c->roots) PLQ_TRY(alloc((void **)&c->roots, sizeof(int32_t) * (size_t)c->n_sites));
+        PLQ_HIP(hipMemsetAsync(c->sizes, 0, sizeof(int32_t) * (size_t)c->n_sites, c->stream));
+        uint32_t cap = (uint32_t)(c->pass_cap > 0xffffffffull ? 0xffffffffull : c->pass_cap);
+        plq::launch_cluster_resolve(c->parent, c->roots, c->cg, cap, c->d_flag, c->sizes, dst, c->stream);
+    }
+    plq::launch_finalize_record(c->d_partials + plq::MAX_PASS_BLOCKS * plq::REC_K, c->last_relabel_blocks, nullptr,
+                                0, reinterpret_cast<long long *>(dst + 42), c->stream);
+    return PLQ_OK;
+}
+
+// A cluster-statistics slot as plq_cluster_stats.
+void fill_cluster_stats(const unsigned long long *h, uint64_t sweep, plq_cluster_stats *out) {
+    std::memset(out, 0, sizeof *out);
+    out->sweep = sweep;
+    for (int k = 0; k < 40; ++k) out->size_hist[k] = (int64_t)h[k];
+    out->n_clusters = (int64_t)h[40];
+    out->largest = (int64_t)h[41];
+    out->moved = (int64_t)h[42 + 1];
+}
+
 long long *rec_slot(plq_chain *c) {
     uint64_t slot = c->rec_count % c->rec_cap;
     c->rec_sweeps[slot] = c->sweeps_done;
@@ -345,7 +380,13 @@ int enqueue_cluster_sweep(plq_chain *c) {
     c->have_partition = true;
     c->sweeps_done += 1;
     bool record = c->rec_every && (c->sweeps_done % c->rec_every == 0);
-    if (record) enqueue_measure(c, rec_slot(c));
+    if (record) {
+        enqueue_measure(c, rec_slot(c));
+        if (c->rec_clusters) {
+            const uint64_t slot = (c->rec_count - 1) % c->rec_cap;
+            PLQ_TRY(enqueue_cluster_stats(c, c->d_clu + slot * CLU_SLOT));
+        }
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(PLQ_EDEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
     return PLQ_OK;
@@ -708,7 +749,8 @@ int enqueue_sweeps(plq_chain *c, uint64_t sweeps) {
     // The one-workgroup cluster sweep cannot overrun the default hop budget
     // (every walk is shorter than the site count); a lowered budget takes the
     // four-kernel path, which counts hops.
-    if (c->kind == KIND_CLUSTER && c->small && c->pass_cap >= (uint64_t)c->n_sites + 64 && sweeps > 0)
+    if (c->kind == KIND_CLUSTER && c->small && !c->rec_clusters && c->pass_cap >= (uint64_t)c->n_sites + 64 &&
+        sweeps > 0)
         return enqueue_small_cluster_sweeps(c, sweeps);
     for (uint64_t t = 0; t < sweeps; ++t) {
         int rc = c->kind == KIND_CLUSTER ? enqueue_cluster_sweep(c) : enqueue_local_sweep(c);
@@ -808,7 +850,9 @@ int plq_chain_record(plq_chain *c, uint64_t every, uint64_t capacity) {
     PLQ_HIP(hipSetDevice(c->device));
     PLQ_HIP(hipStreamSynchronize(c->stream));
     if (c->d_rec) (void)hipFree(c->d_rec);
+    if (c->d_clu) (void)hipFree(c->d_clu);
     c->d_rec = nullptr;
+    c->d_clu = nullptr;
     c->rec_every = 0;
     c->rec_cap = 0;
     c->rec_count = 0;
@@ -816,6 +860,8 @@ int plq_chain_record(plq_chain *c, uint64_t every, uint64_t capacity) {
     if (every == 0) return PLQ_OK;
     if (capacity == 0) return fail(PLQ_EINVAL, "recording needs a positive capacity");
     PLQ_TRY(alloc((void **)&c->d_rec, sizeof(long long) * plq::REC_K * capacity));
+    if (c->rec_clusters) PLQ_TRY(alloc((void **)&c->d_clu, sizeof(unsigned long long) * CLU_SLOT * capacity));
+    c->clu_from = 0;
     c->rec_every = every;
     c->rec_cap = capacity;
     c->rec_sweeps.assign(capacity, 0);
@@ -838,6 +884,42 @@ int plq_chain_series(plq_chain *c, plq_obs *out, uint64_t capacity, uint64_t *n_
     for (uint64_t k = 0; k < have; ++k) {
         uint64_t slot = (first + k) % c->rec_cap;
         fill_obs(c, &host[slot * plq::REC_K], c->rec_sweeps[slot], &out[k]);
+    }
+    return check_flag(c);
+}
+
+int plq_chain_record_clusters(plq_chain *c, int on) {
+    if (!c) return fail(PLQ_EINVAL, "chain is null");
+    if (c->kind != KIND_CLUSTER) return fail(PLQ_EINVAL, "not a cluster chain");
+    std::lock_guard<std::mutex> lk(c->mu);
+    PLQ_HIP(hipSetDevice(c->device));
+    PLQ_HIP(hipStreamSynchronize(c->stream));
+    c->rec_clusters = on != 0;
+    c->clu_from = c->rec_count;
+    if (c->d_clu) (void)hipFree(c->d_clu);
+    c->d_clu = nullptr;
+    if (c->rec_clusters && c->rec_cap)
+        PLQ_TRY(alloc((void **)&c->d_clu, sizeof(unsigned long long) * CLU_SLOT * c->rec_cap));
+    return PLQ_OK;
+}
+
+int plq_chain_cluster_series(plq_chain *c, plq_cluster_stats *out, uint64_t capacity, uint64_t *n_out) {
+    if (!c || !n_out) return fail(PLQ_EINVAL, "chain or count is null");
+    std::lock_guard<std::mutex> lk(c->mu);
+    PLQ_HIP(hipSetDevice(c->device));
+    const uint64_t kept = c->rec_count - c->clu_from;
+    uint64_t have = c->rec_clusters && c->d_clu ? (kept < c->rec_cap ? kept : c->rec_cap) : 0;
+    *n_out = have;
+    if (!out || have == 0) return PLQ_OK;
+    if (capacity < have) return fail(PLQ_EINVAL, "series buffer too small");
+    std::vector<unsigned long long> host(c->rec_cap * CLU_SLOT);
+    PLQ_HIP(hipMemcpyAsync(host.data(), c->d_clu, sizeof(unsigned long long) * host.size(), hipMemcpyDeviceToHost,
+                           c->stream));
+    PLQ_HIP(hipStreamSynchronize(c->stream));
+    uint64_t first = c->rec_count - have;
+    for (uint64_t k = 0; k < have; ++k) {
+        uint64_t slot = (first + k) % c->rec_cap;
+        fill_cluster_stats(&host[slot * CLU_SLOT], c->rec_sweeps[slot], &out[k]);
     }
     return check_flag(c);
 }
@@ -901,34 +983,12 @@ int plq_chain_cluster_stats(plq_chain *c, plq_cluster_stats *out) {
     std::lock_guard<std::mutex> lk(c->mu);
     PLQ_HIP(hipSetDevice(c->device));
     if (!c->sizes) PLQ_TRY(alloc((void **)&c->sizes, sizeof(int32_t) * (size_t)c->n_sites));
-    // hist[0..41]: the histogram; hist[42 ..]: the last relabel's partials summed on the device
-    constexpr size_t HN = 42 + plq::REC_K;
-    if (!c->hist) PLQ_TRY(alloc((void **)&c->hist, sizeof(unsigned long long) * HN));
-    PLQ_HIP(hipMemsetAsync(c->hist, 0, sizeof(unsigned long long) * 42, c->stream));
-    if (plq::cluster_boxes_x16(c->cg, c->labels)) {
-        // sizes counted per box from the finished forest; ext stays zero between calls
-        if (!c->ext) {
-            PLQ_TRY(alloc((void **)&c->ext, sizeof(int32_t) * (size_t)c->n_sites));
-            PLQ_HIP(hipMemsetAsync(c->ext, 0, sizeof(int32_t) * (size_t)c->n_sites, c->stream));
-        }
-        plq::launch_cluster_sizes_box(c->parent, c->cg, c->sizes, c->ext, c->hist, plq::MAX_PASS_BLOCKS, c->stream);
-    } else {
-        if (!c->roots) PLQ_TRY(alloc((void **)&c->roots, sizeof(int32_t) * (size_t)c->n_sites));
-        PLQ_HIP(hipMemsetAsync(c->sizes, 0, sizeof(int32_t) * (size_t)c->n_sites, c->stream));
-        uint32_t cap = (uint32_t)(c->pass_cap > 0xffffffffull ? 0xffffffffull : c->pass_cap);
-        plq::launch_cluster_resolve(c->parent, c->roots, c->cg, cap, c->d_flag, c->sizes, c->hist, c->stream);
-    }
-    plq::launch_finalize_record(c->d_partials + plq::MAX_PASS_BLOCKS * plq::REC_K, c->last_relabel_blocks, nullptr,
-                                0, reinterpret_cast<long long *>(c->hist + 42), c->stream);
-    unsigned long long h[HN];
+    if (!c->hist) PLQ_TRY(alloc((void **)&c->hist, sizeof(unsigned long long) * CLU_SLOT));
+    PLQ_TRY(enqueue_cluster_stats(c, c->hist));
+    unsigned long long h[CLU_SLOT];
     PLQ_HIP(hipMemcpyAsync(h, c->hist, sizeof h, hipMemcpyDeviceToHost, c->stream));
     PLQ_HIP(hipStreamSynchronize(c->stream));
-    std::memset(out, 0, sizeof *out);
-    out->sweep = c->sweeps_done;
-    for (int k = 0; k < 40; ++k) out->size_hist[k] = (int64_t)h[k];
-    out->n_clusters = (int64_t)h[40];
-    out->largest = (int64_t)h[41];
-    out->moved = (int64_t)h[42 + 1];
+    fill_cluster_stats(h, c->sweeps_done, out);
     return check_flag(c);
 }
 

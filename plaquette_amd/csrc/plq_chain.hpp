@@ -13,6 +13,10 @@
 
 enum plq_chain_kind { KIND_SITE = 0, KIND_LINK = 1, KIND_CLUSTER = 2 };
 
+// Entries of one cluster-statistics slot: 40 size bins, the cluster count,
+// the largest size, then the last relabel's REC_K partial sums.
+constexpr size_t CLU_SLOT = 42 + plq::REC_K;
+
 struct plq_chain {
     std::mutex mu;
     plq_chain_kind kind;
@@ -56,6 +60,9 @@ struct plq_chain {
     long long *d_partials = nullptr;       // 2 * MAX_PASS_BLOCKS * REC_K
     long long *d_meas = nullptr;           // REC_K (+ cluster stats)
     long long *d_rec = nullptr;            // rec_cap * REC_K
+    bool rec_clusters = false;             // recorded cluster sweeps also keep their partition's statistics
+    unsigned long long *d_clu = nullptr;   // rec_cap * CLU_SLOT: histogram, count, largest, relabel sums
+    uint64_t clu_from = 0;                 // rec_count when the cluster statistics began
     uint64_t rec_every = 0, rec_cap = 0, rec_count = 0;
     std::vector<uint64_t> rec_sweeps;
     std::vector<hipEvent_t> pending;
@@ -84,7 +91,7 @@ struct plq_chain {
         for (auto p : {(void *)col[0], (void *)col[1], (void *)d_links, (void *)d_table, (void *)labels,
                        (void *)parent, (void *)roots, (void *)sizes, (void *)ext, (void *)hist, (void *)d_flag,
                        (void *)d_lat,
-                       (void *)d_partials, (void *)d_meas, (void *)d_rec, (void *)d_block_counters,
+                       (void *)d_partials, (void *)d_meas, (void *)d_rec, (void *)d_block_counters, (void *)d_clu,
                        (void *)d_block_logits, (void *)d_pm_cls, (void *)d_pm_state, (void *)d_pm_jt, (void *)d_pm_blk,
                        (void *)d_pm_off, (void *)d_meas_bits, (void *)d_meas_counts, (void *)d_batch})
             if (p) (void)hipFree(p);

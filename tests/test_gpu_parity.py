@@ -396,6 +396,40 @@ def test_cluster_stats_after_every_sweep(gpu, shape, q, seeded):
     c.close()
 
 
+@pytest.mark.parametrize("shape,q,seeded", [([256, 256], 3, False), ([32, 32, 32], 2, True), ([90, 91], 3, False),
+                                            ([64, 64], 3, False)])
+def test_recorded_cluster_statistics_match_every_sweep(gpu, shape, q, seeded):
+    # Cluster statistics kept on the device for every recorded sweep of one
+    # multi-sweep advance (the small-lattice chain included: it then runs the
+    # four-kernel sweep) equal the oracle's partition of each sweep.
+    n = int(np.prod(shape))
+    beta = 1.0050525 if q == 3 else 0.4406868
+    start = start_config(n, q, True, 23)
+    extent = gpu.Extent.SEEDED if seeded else gpu.Extent.ALL
+    relabel = gpu.Relabel.FORCED_CHANGE if seeded else gpu.Relabel.REDRAW
+    c = gpu.GpuClusterChain.new(shape, q, 1.0, extent, relabel, beta, 43, start)
+    d = O.cluster_desc(shape, q=q, seeded=seeded, forced=seeded, beta=beta, gap=1.0, seed=43)
+    k = 6
+    c.advance(2)
+    st, _ = O.cluster_sweeps(d, start, 0, 2)
+    c.record(1, k)
+    c.record_clusters(True)
+    c.advance(k)
+    got = c.cluster_series()
+    assert len(got) == k
+    for t in range(k):
+        st, roots = O.cluster_sweeps(d, st, 2 + t, 1)
+        sizes = np.bincount(roots)
+        sizes = sizes[sizes > 0]
+        s = got[t]
+        assert s.sweep == 3 + t
+        assert s.n_clusters == sizes.size and s.largest == sizes.max(), t
+        assert list(s.size_hist) == list(np.bincount(np.floor(np.log2(sizes)).astype(int), minlength=40)), t
+        if not seeded:
+            assert s.moved == n
+    c.close()
+
+
 def test_cluster_budget_exhaustion_is_reported(gpu):
     # gpu_cluster.rs:512-518: labelling past its cap is an error, forced from
     # the host with a zero hop budget on an ordered (percolating) lattice
