@@ -230,7 +230,10 @@ int enqueue_cluster_stats(plq_chain *c, unsigned long long *dst) {
             PLQ_TRY(alloc((void **)&c->ext, sizeof(int32_t) * (size_t)c->n_sites));
             PLQ_HIP(hipMemsetAsync(c->ext, 0, sizeof(int32_t) * (size_t)c->n_sites, c->stream));
         }
-        plq::launch_cluster_sizes_box(c->parent, c->cg, c->sizes, c->ext, dst, plq::MAX_PASS_BLOCKS, c->stream);
+        if (!c->d_hist_part)
+            PLQ_TRY(alloc((void **)&c->d_hist_part, sizeof(unsigned long long) * 42 * 2 * plq::MAX_PASS_BLOCKS));
+        plq::launch_cluster_sizes_box(c->parent, c->cg, c->sizes, c->ext, c->d_cross, c->cross_sweep,
+                                      c->d_hist_part, dst, plq::MAX_PASS_BLOCKS, c->stream);
     } else {
         if (!c->roots) PLQ_TRY(alloc((void **)&c->roots, sizeof(int32_t) * (size_t)c->n_sites));
         PLQ_HIP(hipMemsetAsync(c->sizes, 0, sizeof(int32_t) * (size_t)c->n_sites, c->stream));
@@ -374,6 +377,8 @@ int enqueue_cluster_sweep(plq_chain *c) {
     a.cap = (uint32_t)(c->pass_cap > 0xffffffffull ? 0xffffffffull : c->pass_cap);
     a.flag = c->d_flag;
     a.partials = c->d_partials + plq::MAX_PASS_BLOCKS * plq::REC_K;  // the measure kernels use the first half
+    a.cross = c->d_cross;
+    c->cross_sweep = a.sweep;
     a.max_blocks = plq::MAX_PASS_BLOCKS;
     a.stream = c->stream;
     c->last_relabel_blocks = plq::launch_cluster_sweep(a);
@@ -663,6 +668,11 @@ int plq_chain_create_cluster(const plq_cluster_desc *d, const uint8_t *start, si
     if ((rc = alloc((void **)&c->labels, (size_t)ns)) != PLQ_OK) return bail(rc);
     if ((rc = alloc((void **)&c->parent, (size_t)ns * sizeof(int32_t))) != PLQ_OK) return bail(rc);
     if ((rc = alloc((void **)&c->d_flag, sizeof(int))) != PLQ_OK) return bail(rc);
+    if (plq::cluster_boxes_x16(c->cg, c->labels)) {
+        // crossing marks of cc_compress, zero (no sweep's tag) to start
+        if ((rc = alloc((void **)&c->d_cross, (size_t)ns)) != PLQ_OK) return bail(rc);
+        if (hipMemset(c->d_cross, 0, (size_t)ns) != hipSuccess) return bail(fail(PLQ_EDEVICE, "hipMemset failed"));
+    }
     if (hipMemset(c->d_flag, 0, sizeof(int)) != hipSuccess)
         return bail(fail(PLQ_EDEVICE, "hipMemset failed"));
     if ((rc = write_lattice(c, start, n)) != PLQ_OK) return bail(rc);

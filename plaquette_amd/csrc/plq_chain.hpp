@@ -48,6 +48,9 @@ struct plq_chain {
     int32_t *roots = nullptr;
     int32_t *sizes = nullptr;
     int32_t *ext = nullptr;                // cluster counts from other boxes (box-counted sizes), zero between calls
+    uint8_t *d_cross = nullptr;            // roots of face-crossing clusters, marked by cc_compress
+    uint32_t cross_sweep = 0;              // the sweep whose marks are the latest
+    unsigned long long *d_hist_part = nullptr;  // per-workgroup histogram rows of the statistics kernels
     unsigned long long *hist = nullptr;
     int *d_flag = nullptr;
     uint32_t bond_t = 0;
@@ -91,7 +94,8 @@ struct plq_chain {
         for (auto p : {(void *)col[0], (void *)col[1], (void *)d_links, (void *)d_table, (void *)labels,
                        (void *)parent, (void *)roots, (void *)sizes, (void *)ext, (void *)hist, (void *)d_flag,
                        (void *)d_lat,
-                       (void *)d_partials, (void *)d_meas, (void *)d_rec, (void *)d_block_counters, (void *)d_clu,
+                       (void *)d_partials, (void *)d_meas, (void *)d_rec, (void *)d_block_counters, (void *)d_clu, (void *)d_cross,
+                       (void *)d_hist_part,
                        (void *)d_block_logits, (void *)d_pm_cls, (void *)d_pm_state, (void *)d_pm_jt, (void *)d_pm_blk,
                        (void *)d_pm_off, (void *)d_meas_bits, (void *)d_meas_counts, (void *)d_batch})
             if (p) (void)hipFree(p);

@@ -524,11 +524,13 @@ __global__ __launch_bounds__(256) void cc_boundary(const uint8_t *__restrict__ l
 // lane's store) and stores the root into both ends and both parents.  Every
 // store writes the final root of that cluster.  Of the lanes storing into an
 // end s, the first to read reads s's original parent, so x is reached: for
-// s = x directly, otherwise through s's parent.
+// s = x directly, otherwise through s's parent.  With cross set, cross[root]
+// = tag marks the cluster as one that crosses a box face (cc_sizes_box).
 template <int D>
 __global__ __launch_bounds__(256) void cc_compress(const uint8_t *__restrict__ labels, int32_t *__restrict__ parent,
                                                    CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t,
-                                                   uint32_t cap, int *__restrict__ flag) {
+                                                   uint32_t cap, int *__restrict__ flag, uint8_t *__restrict__ cross,
+                                                   uint8_t tag) {
     int64_t gsite = 0, gp = 0;
     if (!face_bond<D>(labels, cg, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, bond_seed, skey, bond_t, gsite,
                       gp))
@@ -544,6 +546,7 @@ __global__ __launch_bounds__(256) void cc_compress(const uint8_t *__restrict__ l
         }
         root = up;
     }
+    if (cross) cross[root] = tag;  // the cluster crosses a face (cc_sizes_box)
     if (pa != root) {
         store_relaxed(&parent[gsite], root);
         store_relaxed(&parent[pa], root);
@@ -1036,6 +1039,88 @@ __global__ __launch_bounds__(256) void cc_hist(const int32_t *__restrict__ roots
     }
 }
 
+// A workgroup's histogram as one row of HIST_K partials (bins, count,
+// largest), summed by cc_hist_reduce: with thousands of workgroups, atomics
+// on the few hot bins would serialise at their cache line.
+constexpr int HIST_K = 42;
+__device__ __forceinline__ void hist_row(const unsigned int *h, unsigned int nclu, unsigned int largest,
+                                         unsigned long long *row) {
+    if (threadIdx.x < 40) row[threadIdx.x] = h[threadIdx.x];
+    if (threadIdx.x == 0) {
+        row[40] = nclu;
+        row[41] = largest;
+    }
+}
+
+// Column c of `rows` partial rows into out[c]: the sum, the maximum for the
+// largest size (column 41).  One workgroup per column.
+__global__ __launch_bounds__(256) void cc_hist_reduce(const unsigned long long *__restrict__ part, int64_t rows,
+                                                      unsigned long long *__restrict__ out) {
+    __shared__ unsigned long long acc[4];
+    const int c = blockIdx.x;
+    unsigned long long v = 0;
+    for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) {
+        const unsigned long long x = part[r * HIST_K + c];
+        v = c == 41 ? (x > v ? x : v) : v + x;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const unsigned long long y = __shfl_xor(v, o, 64);
+        v = c == 41 ? (y > v ? y : v) : v + y;
+    }
+    if ((threadIdx.x & 63u) == 0u) acc[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = acc[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) t = c == 41 ? (acc[w] > t ? acc[w] : t) : t + acc[w];
+        out[c] = t;
+    }
+}
+
+// Per-lane share of a size histogram: the eight smallest bins, the cluster
+// count and the largest size in registers, larger bins in the workgroup's
+// LDS bins h; flush() adds the lane shares to h, nclu and largest by wave
+// sums (all lanes of the workgroup call it), so hot counters see one atomic
+// per wave instead of one per cluster.
+struct LaneHist {
+    static constexpr int NB = 8;
+    int small[NB];
+    unsigned int count, big;
+    __device__ __forceinline__ LaneHist() : count(0), big(0) {
+#pragma unroll
+        for (int k = 0; k < NB; ++k) small[k] = 0;
+    }
+    __device__ __forceinline__ void add(uint32_t sz, unsigned int *h) {
+        const int bin = 31 - __clz(sz);
+        if (bin < NB) {
+#pragma unroll
+            for (int k = 0; k < NB; ++k) small[k] += bin == k ? 1 : 0;
+        } else {
+            atomicAdd(&h[bin], 1u);
+        }
+        ++count;
+        big = sz > big ? sz : big;
+    }
+    __device__ __forceinline__ void flush(unsigned int *h, unsigned int *nclu, unsigned int *largest) {
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const int t = wave_sum_i32(small[k]);
+            if ((threadIdx.x & 63u) == 0u && t) atomicAdd(&h[k], (unsigned int)t);
+        }
+        const int tot = wave_sum_i32((int)count);
+        unsigned int m = big;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            const unsigned int other = (unsigned int)__shfl_xor((int)m, o, 64);
+            m = other > m ? other : m;
+        }
+        if ((threadIdx.x & 63u) == 0u) {
+            atomicAdd(nclu, (unsigned int)tot);
+            atomicMax(largest, m);
+        }
+    }
+};
+
 // Cluster sizes for the boxes cc_local_x16 labels, from the forest of a
 // finished sweep (every site two loads from its cluster root: the box's
 // parents in LDS as in cc_relabel_box).  Per box, the members of each run of
@@ -1051,8 +1136,18 @@ constexpr int SZ_SUB = 128, SZ_SLOTS = 8 * SZ_SUB;
 
 template <int D>
 __global__ __launch_bounds__(1024, 8) void cc_sizes_box(const int32_t *__restrict__ parent, CCGeom cg,
-                                                     int32_t *__restrict__ sizes, int32_t *__restrict__ ext) {
+                                                     int32_t *__restrict__ sizes, int32_t *__restrict__ ext,
+                                                     const uint8_t *__restrict__ cross, uint8_t tag,
+                                                     unsigned long long *__restrict__ hist) {  // hist: this block's row
     extern __shared__ int32_t P[];
+    // sizes of the clusters that do not cross a face, binned here
+    __shared__ unsigned int hb[40];
+    __shared__ unsigned int nclu, largest;
+    if (threadIdx.x < 40) hb[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        nclu = 0;
+        largest = 0;
+    }
     // counts of the roots outside the box (a percolating cluster's root is
     // outside almost every box): one global atomic per root and box
     __shared__ int32_t okey[SZ_SLOTS], ocnt[SZ_SLOTS];
@@ -1062,6 +1157,7 @@ __global__ __launch_bounds__(1024, 8) void cc_sizes_box(const int32_t *__restric
         okey[i] = -1;
         ocnt[i] = 0;
     }
+    LaneHist lh;
     for (int64_t box = blockIdx.x; box < cg.nboxes; box += gridDim.x) {
         const X16Box<D> bx(cg, (uint32_t)box);
         const bool act = c < (bx.nloc >> 4);
@@ -1135,13 +1231,47 @@ __global__ __launch_bounds__(1024, 8) void cc_sizes_box(const int32_t *__restric
         }
         __syncthreads();
         if (act) {
+            // the segment's 16 crossing marks in one load
+            const uint4 cm = *reinterpret_cast<const uint4 *>(cross + gsite0);
+            const uint32_t t4 = 0x01010101u * tag;
+            const uint32_t crossing = equal_bytes4(cm.x, t4) | (equal_bytes4(cm.y, t4) << 4) |
+                                      (equal_bytes4(cm.z, t4) << 8) | (equal_bytes4(cm.w, t4) << 12);
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
                 const int32_t n = P[l0 + (uint32_t)k];
-                if (n) sizes[gsite0 + (uint32_t)k] = n;
+                if (!n) continue;
+                if ((crossing >> k) & 1u)
+                    sizes[gsite0 + (uint32_t)k] = n;  // cc_hist_cross adds the other boxes' share and bins it
+                else
+                    lh.add((uint32_t)n, hb);
             }
         }
-        for (int i = threadIdx.x; i < SZ_SLOTS; i += blockDim.x) {
+        // sub-tables 1..7 fold into sub-table 0, so a root goes to global
+        // memory once per box (the percolating root's ext counter takes one
+        // atomic per box, not eight)
+        __syncthreads();
+        for (int i = SZ_SUB + threadIdx.x; i < SZ_SLOTS; i += blockDim.x) {
+            const int32_t r = okey[i];
+            if (r == -1) continue;
+            uint32_t h = ((uint32_t)r * 0x9e3779b1u) >> (32 - 7);
+            bool done = false;
+            for (int probe = 0; probe < 8 && !done; ++probe, h = (h + 1u) & (SZ_SUB - 1)) {
+                int32_t key = okey[h];
+                if (key == -1) {
+                    const int32_t old = atomicCAS(&okey[h], -1, r);
+                    key = old == -1 ? r : old;
+                }
+                if (key == r) {
+                    atomicAdd(&ocnt[h], ocnt[i]);
+                    done = true;
+                }
+            }
+            if (!done) atomicAdd(&ext[r], ocnt[i]);
+            okey[i] = -1;
+            ocnt[i] = 0;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < SZ_SUB; i += blockDim.x) {
             if (okey[i] != -1) {
                 atomicAdd(&ext[okey[i]], ocnt[i]);
                 okey[i] = -1;
@@ -1149,17 +1279,21 @@ __global__ __launch_bounds__(1024, 8) void cc_sizes_box(const int32_t *__restric
             }
         }
     }
+    lh.flush(hb, &nclu, &largest);
+    __syncthreads();
+    hist_row(hb, nclu, largest, hist + (int64_t)blockIdx.x * HIST_K);
 }
 
-// The size histogram from cc_sizes_box: every root (parent[s] == s after a
-// finished sweep) has size sizes[s] + ext[s]; ext[s] is cleared for the next
-// call (only roots ever receive ext counts).  hist[0..39] by floor(log2
-// size), hist[40] clusters, hist[41] largest.  The eight smallest bins are
-// counted per lane and summed per wave, away from the LDS atomics.
-__global__ __launch_bounds__(256) void cc_hist_roots(const int32_t *__restrict__ parent,
+// The crossing clusters' share of the histogram: every site marked with
+// this sweep's tag that is a root (parent[s] == s) has size sizes[s] +
+// ext[s]; ext[s] is cleared for the next call (only crossing roots receive
+// ext counts).  A mark left by an older sweep with the same tag only sends
+// a cluster through this path, where its size is still right.  Sixteen
+// marks per lane and step, visited in a loop over the matching ones.
+__global__ __launch_bounds__(256) void cc_hist_cross(const int32_t *__restrict__ parent,
                                                      const int32_t *__restrict__ sizes, int32_t *__restrict__ ext,
-                                                     int64_t N, unsigned long long *__restrict__ hist) {
-    constexpr int NB = 8;  // bins counted in registers (sizes below 256, most clusters)
+                                                     const uint8_t *__restrict__ cross, uint8_t tag, int64_t N,
+                                                     unsigned long long *__restrict__ hist) {
     __shared__ unsigned int h[40];
     __shared__ unsigned int nclu, largest;
     if (threadIdx.x < 40) h[threadIdx.x] = 0;
@@ -1168,64 +1302,35 @@ __global__ __launch_bounds__(256) void cc_hist_roots(const int32_t *__restrict__
         largest = 0;
     }
     __syncthreads();
-    int small[NB];
-#pragma unroll
-    for (int k = 0; k < NB; ++k) small[k] = 0;
-    unsigned int mine = 0, big = 0;
-    auto count = [&](uint32_t sz) {
-        const int bin = 31 - __clz(sz);
-        if (bin < NB) {
-#pragma unroll
-            for (int k = 0; k < NB; ++k) small[k] += bin == k ? 1 : 0;
-        } else {
-            atomicAdd(&h[bin], 1u);
-        }
-        ++mine;
-        big = sz > big ? sz : big;
+    LaneHist lh;
+    // the three loads of a visit are independent, so they overlap
+    auto visit = [&](int64_t s) {
+        const int32_t p = parent[s], x = ext[s], z = sizes[s];
+        if (p != (int32_t)s) return;
+        if (x) ext[s] = 0;
+        lh.add((uint32_t)(z + x), h);
     };
-    // four sites per lane and step, the three arrays loaded side by side
-    const int64_t n4 = N >> 2;
-    const int4 *p4 = reinterpret_cast<const int4 *>(parent);
-    const int4 *s4 = reinterpret_cast<const int4 *>(sizes);
-    int4 *e4 = reinterpret_cast<int4 *>(ext);
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n4; v += (int64_t)gridDim.x * blockDim.x) {
-        const int4 p = p4[v], z = s4[v], x = e4[v];
-        const int32_t s0 = (int32_t)(v << 2);
-        if (x.x | x.y | x.z | x.w) e4[v] = make_int4(0, 0, 0, 0);
-        if (p.x == s0) count((uint32_t)(z.x + x.x));
-        if (p.y == s0 + 1) count((uint32_t)(z.y + x.y));
-        if (p.z == s0 + 2) count((uint32_t)(z.z + x.z));
-        if (p.w == s0 + 3) count((uint32_t)(z.w + x.w));
-    }
-    for (int64_t s = (n4 << 2) + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < N;
-         s += (int64_t)gridDim.x * blockDim.x) {
-        if (parent[s] == (int32_t)s) {
-            const int32_t x = ext[s];
-            if (x) ext[s] = 0;
-            count((uint32_t)(sizes[s] + x));
+    const uint32_t t4 = 0x01010101u * tag;
+    const int64_t n16 = N >> 4;
+    const uint4 *c16 = reinterpret_cast<const uint4 *>(cross);
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n16; v += (int64_t)gridDim.x * blockDim.x) {
+        const uint4 m = c16[v];
+        // the marked sites of the 16 as a mask, then one visit per mark: a
+        // wave loops as often as its busiest lane has marks
+        uint32_t hit = equal_bytes4(m.x, t4) | (equal_bytes4(m.y, t4) << 4) | (equal_bytes4(m.z, t4) << 8) |
+                       (equal_bytes4(m.w, t4) << 12);
+        while (hit) {
+            const uint32_t k = (uint32_t)__builtin_ctz(hit);
+            hit &= hit - 1u;
+            visit((v << 4) + k);
         }
     }
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-        const int t = wave_sum_i32(small[k]);
-        if ((threadIdx.x & 63u) == 0u && t) atomicAdd(&h[k], (unsigned int)t);
-    }
-    const int tot = wave_sum_i32((int)mine);
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        const unsigned int other = (unsigned int)__shfl_xor((int)big, o, 64);
-        big = other > big ? other : big;
-    }
-    if ((threadIdx.x & 63u) == 0u) {
-        atomicAdd(&nclu, (unsigned int)tot);
-        atomicMax(&largest, big);
-    }
+    for (int64_t s = (n16 << 4) + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < N;
+         s += (int64_t)gridDim.x * blockDim.x)
+        if (cross[s] == tag) visit(s);
+    lh.flush(h, &nclu, &largest);
     __syncthreads();
-    if (threadIdx.x < 40 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
-    if (threadIdx.x == 0) {
-        atomicAdd(&hist[40], (unsigned long long)nclu);
-        atomicMax(&hist[41], (unsigned long long)largest);
-    }
+    hist_row(h, nclu, largest, hist + (int64_t)blockIdx.x * HIST_K);
 }
 
 // ------------------------------------------------------------ launchers ---
@@ -1298,7 +1403,7 @@ int64_t launch_cluster_sweep(const ClusterSweepArgs &a) {
         cc_boundary<DD><<<gb, bb, 0, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t, a.cap, \
                                                  a.flag);                                                   \
         cc_compress<DD><<<gb, bb, 0, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t, a.cap, \
-                                                 a.flag);                                                   \
+                                                 a.flag, a.cross, cross_tag(a.sweep));                      \
     }
     switch (cg.D) {
         case 1: PLQ_CC(1); break;
@@ -1377,18 +1482,26 @@ bool cluster_boxes_x16(const CCGeom &cg, const void *labels) {
     return pow2 && cg.B[0] % 16 == 0 && cg.box_sites <= 16 * 1024 && (reinterpret_cast<uintptr_t>(labels) & 15u) == 0;
 }
 
+uint8_t cross_tag(uint32_t sweep) { return (uint8_t)(1u + sweep % 255u); }
+
 void launch_cluster_sizes_box(const int32_t *parent, const CCGeom &cg, int32_t *sizes, int32_t *ext,
+                              const uint8_t *cross, uint32_t sweep, unsigned long long *part,
                               unsigned long long *hist, int64_t max_blocks, hipStream_t s) {
     const int64_t Gb = cg.nboxes < max_blocks ? cg.nboxes : max_blocks;
     const size_t lds = (size_t)cg.box_sites * sizeof(int32_t);
+    const uint8_t tag = cross_tag(sweep);
+    const dim3 g((uint32_t)Gb), b(1024);
     switch (cg.D) {
-        case 1: cc_sizes_box<1><<<dim3((uint32_t)Gb), dim3(1024), lds, s>>>(parent, cg, sizes, ext); break;
-        case 2: cc_sizes_box<2><<<dim3((uint32_t)Gb), dim3(1024), lds, s>>>(parent, cg, sizes, ext); break;
-        case 3: cc_sizes_box<3><<<dim3((uint32_t)Gb), dim3(1024), lds, s>>>(parent, cg, sizes, ext); break;
-        default: cc_sizes_box<4><<<dim3((uint32_t)Gb), dim3(1024), lds, s>>>(parent, cg, sizes, ext); break;
+        case 1: cc_sizes_box<1><<<g, b, lds, s>>>(parent, cg, sizes, ext, cross, tag, part); break;
+        case 2: cc_sizes_box<2><<<g, b, lds, s>>>(parent, cg, sizes, ext, cross, tag, part); break;
+        case 3: cc_sizes_box<3><<<g, b, lds, s>>>(parent, cg, sizes, ext, cross, tag, part); break;
+        default: cc_sizes_box<4><<<g, b, lds, s>>>(parent, cg, sizes, ext, cross, tag, part); break;
     }
-    const int64_t hb = (cg.N + 255) / 256 < 2048 ? (cg.N + 255) / 256 : 2048;
-    cc_hist_roots<<<dim3((uint32_t)hb), dim3(256), 0, s>>>(parent, sizes, ext, cg.N, hist);
+    int64_t hb = (cg.N + 4095) / 4096 < max_blocks ? (cg.N + 4095) / 4096 : max_blocks;
+    if (hb < 1) hb = 1;
+    cc_hist_cross<<<dim3((uint32_t)hb), dim3(256), 0, s>>>(parent, sizes, ext, cross, tag, cg.N,
+                                                           part + Gb * HIST_K);
+    cc_hist_reduce<<<dim3(HIST_K), dim3(256), 0, s>>>(part, Gb + hb, hist);
 }
 
 void launch_cluster_resolve(int32_t *parent, int32_t *roots, const CCGeom &cg, uint32_t cap, int *flag,

@@ -203,6 +203,7 @@ CCGeom make_ccgeom(int D, const int64_t *shape, int32_t max_box_sites);
 struct ClusterSweepArgs {
     uint8_t *labels;      // lattice order
     int32_t *parent;      // union-find forest, lattice order
+    uint8_t *cross;       // nullable: N bytes, cc_compress marks the roots of face-crossing clusters
     CCGeom cg;
     uint32_t seed32;
     uint32_t sweep;
@@ -229,8 +230,13 @@ void launch_potts_measure(const uint8_t *labels, const CCGeom &cg, int q, long l
 bool cluster_boxes_x16(const CCGeom &cg, const void *labels);
 // Size histogram of the last sweep's clusters on those boxes (hist zeroed by
 // the caller; ext all zero on entry and on return).
+// cross: the marks cc_compress leaves on the roots of face-crossing clusters
+// (N bytes, tag cross_tag(sweep) of the sweep whose partition is counted).
+// part: scratch for 2 * max_blocks rows of 42 partials.
 void launch_cluster_sizes_box(const int32_t *parent, const CCGeom &cg, int32_t *sizes, int32_t *ext,
+                              const uint8_t *cross, uint32_t sweep, unsigned long long *part,
                               unsigned long long *hist, int64_t max_blocks, hipStream_t s);
+uint8_t cross_tag(uint32_t sweep);
 void launch_cluster_resolve(int32_t *parent, int32_t *roots, const CCGeom &cg, uint32_t cap, int *flag,
                             int32_t *sizes, unsigned long long *hist, hipStream_t s);
 
