@@ -220,7 +220,8 @@ void enqueue_measure(plq_chain *c, long long *rec) {
 // Statistics of the last sweep's partition into dst (CLU_SLOT entries):
 // dst[0..39] clusters by floor(log2 size), dst[40] clusters, dst[41] the
 // largest size, dst[42 ..] the last relabel's partials summed ([0, moved,
-// 0 ...]).  Enqueue only; allocates the work buffers on first use.
+// 0 ...]; on the box path only the moved column).  Enqueue only; allocates
+// the work buffers on first use.
 int enqueue_cluster_stats(plq_chain *c, unsigned long long *dst) {
     if (!c->sizes) PLQ_TRY(alloc((void **)&c->sizes, sizeof(int32_t) * (size_t)c->n_sites));
     PLQ_HIP(hipMemsetAsync(dst, 0, sizeof(unsigned long long) * 42, c->stream));
@@ -233,7 +234,9 @@ int enqueue_cluster_stats(plq_chain *c, unsigned long long *dst) {
         if (!c->d_hist_part)
             PLQ_TRY(alloc((void **)&c->d_hist_part, sizeof(unsigned long long) * 42 * 2 * plq::MAX_PASS_BLOCKS));
         plq::launch_cluster_sizes_box(c->parent, c->cg, c->sizes, c->ext, c->d_cross, c->cross_sweep,
-                                      c->d_hist_part, dst, plq::MAX_PASS_BLOCKS, c->stream);
+                                      c->d_hist_part, c->d_partials + plq::MAX_PASS_BLOCKS * plq::REC_K,
+                                      c->last_relabel_blocks, dst, plq::MAX_PASS_BLOCKS, c->stream);
+        return PLQ_OK;
     } else {
         if (!c->roots) PLQ_TRY(alloc((void **)&c->roots, sizeof(int32_t) * (size_t)c->n_sites));
         PLQ_HIP(hipMemsetAsync(c->sizes, 0, sizeof(int32_t) * (size_t)c->n_sites, c->stream));

@@ -1053,15 +1053,23 @@ __device__ __forceinline__ void hist_row(const unsigned int *h, unsigned int ncl
 }
 
 // Column c of `rows` partial rows into out[c]: the sum, the maximum for the
-// largest size (column 41).  One workgroup per column.
+// largest size (column 41).  One workgroup per column; workgroup HIST_K sums
+// the relabelled-site column of the relabel's partials (REC_K per row, `rrows`
+// rows) into out[HIST_K + 1] and clears out[HIST_K .. HIST_K + REC_K).
 __global__ __launch_bounds__(256) void cc_hist_reduce(const unsigned long long *__restrict__ part, int64_t rows,
+                                                      const long long *__restrict__ rpart, int64_t rrows,
                                                       unsigned long long *__restrict__ out) {
     __shared__ unsigned long long acc[4];
     const int c = blockIdx.x;
     unsigned long long v = 0;
-    for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) {
-        const unsigned long long x = part[r * HIST_K + c];
-        v = c == 41 ? (x > v ? x : v) : v + x;
+    if (c == HIST_K) {
+        if (threadIdx.x < REC_K && threadIdx.x != 1) out[HIST_K + threadIdx.x] = 0;
+        for (int64_t r = threadIdx.x; r < rrows; r += blockDim.x) v += (unsigned long long)rpart[r * REC_K + 1];
+    } else {
+        for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) {
+            const unsigned long long x = part[r * HIST_K + c];
+            v = c == 41 ? (x > v ? x : v) : v + x;
+        }
     }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
@@ -1073,7 +1081,7 @@ __global__ __launch_bounds__(256) void cc_hist_reduce(const unsigned long long *
     if (threadIdx.x == 0) {
         unsigned long long t = acc[0];
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w) t = c == 41 ? (acc[w] > t ? acc[w] : t) : t + acc[w];
-        out[c] = t;
+        out[c == HIST_K ? HIST_K + 1 : c] = t;
     }
 }
 
@@ -1486,7 +1494,8 @@ uint8_t cross_tag(uint32_t sweep) { return (uint8_t)(1u + sweep % 255u); }
 
 void launch_cluster_sizes_box(const int32_t *parent, const CCGeom &cg, int32_t *sizes, int32_t *ext,
                               const uint8_t *cross, uint32_t sweep, unsigned long long *part,
-                              unsigned long long *hist, int64_t max_blocks, hipStream_t s) {
+                              const long long *relabel_part, int64_t relabel_rows, unsigned long long *hist,
+                              int64_t max_blocks, hipStream_t s) {
     const int64_t Gb = cg.nboxes < max_blocks ? cg.nboxes : max_blocks;
     const size_t lds = (size_t)cg.box_sites * sizeof(int32_t);
     const uint8_t tag = cross_tag(sweep);
@@ -1501,7 +1510,7 @@ void launch_cluster_sizes_box(const int32_t *parent, const CCGeom &cg, int32_t *
     if (hb < 1) hb = 1;
     cc_hist_cross<<<dim3((uint32_t)hb), dim3(256), 0, s>>>(parent, sizes, ext, cross, tag, cg.N,
                                                            part + Gb * HIST_K);
-    cc_hist_reduce<<<dim3(HIST_K), dim3(256), 0, s>>>(part, Gb + hb, hist);
+    cc_hist_reduce<<<dim3(HIST_K + 1), dim3(256), 0, s>>>(part, Gb + hb, relabel_part, relabel_rows, hist);
 }
 
 void launch_cluster_resolve(int32_t *parent, int32_t *roots, const CCGeom &cg, uint32_t cap, int *flag,

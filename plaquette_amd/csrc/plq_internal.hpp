@@ -232,10 +232,13 @@ bool cluster_boxes_x16(const CCGeom &cg, const void *labels);
 // the caller; ext all zero on entry and on return).
 // cross: the marks cc_compress leaves on the roots of face-crossing clusters
 // (N bytes, tag cross_tag(sweep) of the sweep whose partition is counted).
-// part: scratch for 2 * max_blocks rows of 42 partials.
+// part: scratch for 2 * max_blocks rows of 42 partials; hist[42 ..] also
+// takes the relabel's partials summed (relabel_rows rows of REC_K; only the
+// relabelled-site column is kept).
 void launch_cluster_sizes_box(const int32_t *parent, const CCGeom &cg, int32_t *sizes, int32_t *ext,
                               const uint8_t *cross, uint32_t sweep, unsigned long long *part,
-                              unsigned long long *hist, int64_t max_blocks, hipStream_t s);
+                              const long long *relabel_part, int64_t relabel_rows, unsigned long long *hist,
+                              int64_t max_blocks, hipStream_t s);
 uint8_t cross_tag(uint32_t sweep);
 void launch_cluster_resolve(int32_t *parent, int32_t *roots, const CCGeom &cg, uint32_t cap, int *flag,
                             int32_t *sizes, unsigned long long *hist, hipStream_t s);
