@@ -222,20 +222,30 @@ void enqueue_measure(plq_chain *c, long long *rec) {
 // largest size, dst[42 ..] the last relabel's partials summed ([0, moved,
 // 0 ...]; on the box path only the moved column).  Enqueue only; allocates
 // the work buffers on first use.
-int enqueue_cluster_stats(plq_chain *c, unsigned long long *dst) {
+int ensure_stats_buffers(plq_chain *c) {
     if (!c->sizes) PLQ_TRY(alloc((void **)&c->sizes, sizeof(int32_t) * (size_t)c->n_sites));
-    PLQ_HIP(hipMemsetAsync(dst, 0, sizeof(unsigned long long) * 42, c->stream));
     if (plq::cluster_boxes_x16(c->cg, c->labels)) {
-        // sizes counted per box from the finished forest; ext stays zero between calls
+        // ext stays zero between calls
         if (!c->ext) {
             PLQ_TRY(alloc((void **)&c->ext, sizeof(int32_t) * (size_t)c->n_sites));
             PLQ_HIP(hipMemsetAsync(c->ext, 0, sizeof(int32_t) * (size_t)c->n_sites, c->stream));
         }
         if (!c->d_hist_part)
             PLQ_TRY(alloc((void **)&c->d_hist_part, sizeof(unsigned long long) * 42 * 2 * plq::MAX_PASS_BLOCKS));
+    }
+    return PLQ_OK;
+}
+
+int enqueue_cluster_stats(plq_chain *c, unsigned long long *dst) {
+    PLQ_TRY(ensure_stats_buffers(c));
+    PLQ_HIP(hipMemsetAsync(dst, 0, sizeof(unsigned long long) * 42, c->stream));
+    if (plq::cluster_boxes_x16(c->cg, c->labels)) {
+        // sizes counted per box from the finished forest (by the sweep's relabel when it was recorded)
         plq::launch_cluster_sizes_box(c->parent, c->cg, c->sizes, c->ext, c->d_cross, c->cross_sweep,
                                       c->d_hist_part, c->d_partials + plq::MAX_PASS_BLOCKS * plq::REC_K,
-                                      c->last_relabel_blocks, dst, plq::MAX_PASS_BLOCKS, c->stream);
+                                      c->last_relabel_blocks, dst, plq::MAX_PASS_BLOCKS, c->sizes_counted,
+                                      c->stream);
+        c->sizes_counted = false;  // the counts are consumed (ext cleared)
         return PLQ_OK;
     } else {
         if (!c->roots) PLQ_TRY(alloc((void **)&c->roots, sizeof(int32_t) * (size_t)c->n_sites));
@@ -382,6 +392,18 @@ int enqueue_cluster_sweep(plq_chain *c) {
     a.partials = c->d_partials + plq::MAX_PASS_BLOCKS * plq::REC_K;  // the measure kernels use the first half
     a.cross = c->d_cross;
     c->cross_sweep = a.sweep;
+    // a recorded sweep whose cluster statistics are kept counts them in its relabel
+    const bool record_next = c->rec_every && ((c->sweeps_done + 1) % c->rec_every == 0);
+    a.count_sizes = false;
+    c->sizes_counted = false;
+    if (record_next && c->rec_clusters && plq::cluster_boxes_x16(c->cg, c->labels)) {
+        PLQ_TRY(ensure_stats_buffers(c));
+        a.count_sizes = true;
+        a.sizes = c->sizes;
+        a.ext = c->ext;
+        a.hist_part = c->d_hist_part;
+        c->sizes_counted = true;
+    }
     a.max_blocks = plq::MAX_PASS_BLOCKS;
     a.stream = c->stream;
     c->last_relabel_blocks = plq::launch_cluster_sweep(a);

@@ -51,6 +51,7 @@ struct plq_chain {
     uint8_t *d_cross = nullptr;            // roots of face-crossing clusters, marked by cc_compress
     uint32_t cross_sweep = 0;              // the sweep whose marks are the latest
     unsigned long long *d_hist_part = nullptr;  // per-workgroup histogram rows of the statistics kernels
+    bool sizes_counted = false;            // the last sweep's relabel counted its sizes into sizes / ext / rows
     unsigned long long *hist = nullptr;
     int *d_flag = nullptr;
     uint32_t bond_t = 0;

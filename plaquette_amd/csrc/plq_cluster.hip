@@ -725,6 +725,8 @@ __device__ __forceinline__ void box_load_parents(int32_t *P, const int32_t *__re
     }
 }
 
+// The new labels of a lane's 16 sites (w, in place), the roots found as
+// the sites are visited; returns the relabelled sites.
 template <int D>
 __device__ __forceinline__ int box_relabel16(const int32_t *P, const CCGeom &cg, const X16Box<D> &bx, uint32_t l0,
                                              uint32_t (&w)[4], uint32_t root_seed, uint32_t skey, int q, int seeded,
@@ -751,6 +753,59 @@ __device__ __forceinline__ int box_relabel16(const int32_t *P, const CCGeom &cg,
             const uint32_t lab = (w[k >> 2] >> (8 * (k & 3))) & 0xffu;
             nl = (seeded && root != seed_root) ? NO_LABEL
                                                : relabel_draw(keyed_bits_k(root_seed, (uint32_t)root, skey), lab, q, forced);
+        }
+        if (nl != NO_LABEL) {
+            const int sh = 8 * (k & 3);
+            w[k >> 2] = (w[k >> 2] & ~(0xffu << sh)) | (nl << sh);
+            ++cnt;
+        }
+    }
+    return cnt;
+}
+
+// The cluster roots of a lane's 16 sites, kept for the counting that
+// follows (the relabel-and-count and sizes kernels).
+template <int D>
+__device__ __forceinline__ void box_roots16(const int32_t *P, const CCGeom &cg, const X16Box<D> &bx, uint32_t l0,
+                                            int32_t (&root)[16], uint32_t cap, bool &over) {
+    int32_t prev = -1, r = -1;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int32_t p = P[l0 + (uint32_t)k];
+        if (p != prev) {
+            prev = p;
+            // p inside the box: its parent, from LDS; else p is a root
+            uint32_t local = 0;
+            bool inside = true;
+#pragma unroll
+            for (int mu = 0; mu < D; ++mu) {
+                const uint32_t y = (((uint32_t)p >> bx.Lsh[mu]) & ((uint32_t)cg.L[mu] - 1u)) - bx.o[mu];
+                inside = inside && y < bx.e[mu];
+                local |= y << bx.lsh[mu];
+            }
+            r = inside ? P[local] : p;
+            over = over || (r != p ? 2u : 1u) > cap;
+        }
+        root[k] = r;
+    }
+}
+
+// The new labels of a lane's 16 sites (w, in place) from their roots
+// (box_roots16); consecutive sites with the same root share the draw.
+// Returns the relabelled sites.
+__device__ __forceinline__ int box_relabel_roots16(const int32_t (&root)[16], uint32_t (&w)[4], uint32_t root_seed,
+                                                   uint32_t skey, int q, int seeded, int forced, int32_t seed_root) {
+    int32_t prev = -1;
+    uint32_t nl = NO_LABEL;
+    int cnt = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        if (root[k] != prev) {
+            prev = root[k];
+            const uint32_t lab = (w[k >> 2] >> (8 * (k & 3))) & 0xffu;
+            nl = (seeded && prev != seed_root)
+                     ? NO_LABEL
+                     : relabel_draw(keyed_bits_k(root_seed, (uint32_t)prev, skey), lab, q, forced);
         }
         if (nl != NO_LABEL) {
             const int sh = 8 * (k & 3);
@@ -1142,13 +1197,126 @@ struct LaneHist {
 // cluster's) spreads its atomics over 8 slots
 constexpr int SZ_SUB = 128, SZ_SLOTS = 8 * SZ_SUB;
 
+// The counting of one box (all threads of the workgroup call it, after the
+// roots are known): P becomes the box's per-slot counts, outside roots go
+// through the LDS table okey / ocnt to ext, and each counted root either
+// stores its count in sizes (when marked as crossing) or is binned in lh.
+template <int D>
+__device__ __forceinline__ void box_count16(int32_t *P, int32_t *okey, int32_t *ocnt, const CCGeom &cg,
+                                            const X16Box<D> &bx, bool act, uint32_t l0, uint32_t gsite0,
+                                            const int32_t (&root)[16], int32_t *__restrict__ sizes,
+                                            int32_t *__restrict__ ext, const uint8_t *__restrict__ cross,
+                                            uint8_t tag, LaneHist &lh, unsigned int *hb) {
+    const uint32_t c = threadIdx.x;
+    (void)c;
+    // box-local slot of global site g, or ~0u outside the box
+    auto slot = [&](int32_t g) -> uint32_t {
+        uint32_t local = 0;
+        bool inside = true;
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) {
+            const uint32_t y = (((uint32_t)g >> bx.Lsh[mu]) & ((uint32_t)cg.L[mu] - 1u)) - bx.o[mu];
+            inside = inside && y < bx.e[mu];
+            local |= y << bx.lsh[mu];
+        }
+        return inside ? local : ~0u;
+    };
+    __syncthreads();  // every read of the parents is done: P becomes the counts
+    if (act) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) P[l0 + (uint32_t)k] = 0;
+    }
+    __syncthreads();
+    if (act) {
+        int len = 1;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (k == 15 || root[k + 1] != root[k]) {
+                const int32_t r = root[k];
+                const uint32_t sl = slot(r);
+                if (sl != ~0u) {
+                    atomicAdd(&P[sl], len);
+                } else {
+                    const uint32_t sub = (threadIdx.x & 7u) * SZ_SUB;
+                    uint32_t h = ((uint32_t)r * 0x9e3779b1u) >> (32 - 7);
+                    bool done = false;
+                    for (int probe = 0; probe < 8 && !done; ++probe, h = (h + 1u) & (SZ_SUB - 1)) {
+                        int32_t key = okey[sub + h];
+                        if (key == -1) {
+                            const int32_t old = atomicCAS(&okey[sub + h], -1, r);
+                            key = old == -1 ? r : old;
+                        }
+                        if (key == r) {
+                            atomicAdd(&ocnt[sub + h], len);
+                            done = true;
+                        }
+                    }
+                    if (!done) atomicAdd(&ext[r], len);  // table crowded: straight to the global count
+                }
+                len = 1;
+            } else {
+                ++len;
+            }
+        }
+    }
+    __syncthreads();
+    if (act) {
+        // the segment's 16 crossing marks in one load
+        const uint4 cm = *reinterpret_cast<const uint4 *>(cross + gsite0);
+        const uint32_t t4 = 0x01010101u * tag;
+        const uint32_t crossing = equal_bytes4(cm.x, t4) | (equal_bytes4(cm.y, t4) << 4) |
+                                  (equal_bytes4(cm.z, t4) << 8) | (equal_bytes4(cm.w, t4) << 12);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int32_t n = P[l0 + (uint32_t)k];
+            if (!n) continue;
+            if ((crossing >> k) & 1u)
+                sizes[gsite0 + (uint32_t)k] = n;  // cc_hist_cross adds the other boxes' share and bins it
+            else
+                lh.add((uint32_t)n, hb);
+        }
+    }
+    // sub-tables 1..7 fold into sub-table 0, so a root goes to global
+    // memory once per box (the percolating root's ext counter takes one
+    // atomic per box, not eight)
+    __syncthreads();
+    for (int i = SZ_SUB + threadIdx.x; i < SZ_SLOTS; i += blockDim.x) {
+        const int32_t r = okey[i];
+        if (r == -1) continue;
+        uint32_t h = ((uint32_t)r * 0x9e3779b1u) >> (32 - 7);
+        bool done = false;
+        for (int probe = 0; probe < 8 && !done; ++probe, h = (h + 1u) & (SZ_SUB - 1)) {
+            int32_t key = okey[h];
+            if (key == -1) {
+                const int32_t old = atomicCAS(&okey[h], -1, r);
+                key = old == -1 ? r : old;
+            }
+            if (key == r) {
+                atomicAdd(&ocnt[h], ocnt[i]);
+                done = true;
+            }
+        }
+        if (!done) atomicAdd(&ext[r], ocnt[i]);
+        okey[i] = -1;
+        ocnt[i] = 0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < SZ_SUB; i += blockDim.x) {
+        if (okey[i] != -1) {
+            atomicAdd(&ext[okey[i]], ocnt[i]);
+            okey[i] = -1;
+            ocnt[i] = 0;
+        }
+    }
+}
+
 template <int D>
 __global__ __launch_bounds__(1024, 8) void cc_sizes_box(const int32_t *__restrict__ parent, CCGeom cg,
                                                      int32_t *__restrict__ sizes, int32_t *__restrict__ ext,
                                                      const uint8_t *__restrict__ cross, uint8_t tag,
                                                      unsigned long long *__restrict__ hist) {  // hist: this block's row
     extern __shared__ int32_t P[];
-    // sizes of the clusters that do not cross a face, binned here
+    __shared__ int32_t okey[SZ_SLOTS], ocnt[SZ_SLOTS];
     __shared__ unsigned int hb[40];
     __shared__ unsigned int nclu, largest;
     if (threadIdx.x < 40) hb[threadIdx.x] = 0;
@@ -1156,140 +1324,88 @@ __global__ __launch_bounds__(1024, 8) void cc_sizes_box(const int32_t *__restric
         nclu = 0;
         largest = 0;
     }
-    // counts of the roots outside the box (a percolating cluster's root is
-    // outside almost every box): one global atomic per root and box
-    __shared__ int32_t okey[SZ_SLOTS], ocnt[SZ_SLOTS];
-    const uint32_t c = threadIdx.x;
-    const uint32_t l0 = c << 4;
     for (int i = threadIdx.x; i < SZ_SLOTS; i += blockDim.x) {
         okey[i] = -1;
         ocnt[i] = 0;
     }
+    const uint32_t c = threadIdx.x;
+    const uint32_t l0 = c << 4;
     LaneHist lh;
+    bool over = false;
     for (int64_t box = blockIdx.x; box < cg.nboxes; box += gridDim.x) {
         const X16Box<D> bx(cg, (uint32_t)box);
         const bool act = c < (bx.nloc >> 4);
         const uint32_t gsite0 = bx.locate(cg, l0);
-        // box-local slot of global site g, or ~0u outside the box
-        auto slot = [&](int32_t g) -> uint32_t {
-            uint32_t local = 0;
-            bool inside = true;
-#pragma unroll
-            for (int mu = 0; mu < D; ++mu) {
-                const uint32_t y = (((uint32_t)g >> bx.Lsh[mu]) & ((uint32_t)cg.L[mu] - 1u)) - bx.o[mu];
-                inside = inside && y < bx.e[mu];
-                local |= y << bx.lsh[mu];
-            }
-            return inside ? local : ~0u;
-        };
+        __syncthreads();  // the previous box's counts are out
+        if (act) box_load_parents<D>(P, parent, l0, gsite0);
+        __syncthreads();
+        int32_t root[16];
+        if (act) box_roots16<D>(P, cg, bx, l0, root, 0xffffffffu, over);
+        box_count16<D>(P, okey, ocnt, cg, bx, act, l0, gsite0, root, sizes, ext, cross, tag, lh, hb);
+    }
+    lh.flush(hb, &nclu, &largest);
+    __syncthreads();
+    hist_row(hb, nclu, largest, hist + (int64_t)blockIdx.x * HIST_K);
+}
+
+// One sweep's relabel (as cc_relabel_box) and the counting of its clusters
+// (as cc_sizes_box) in one pass over the box's parents, for a recorded sweep
+// whose cluster statistics are kept.
+template <int D>
+__global__ __launch_bounds__(1024, 8) void cc_relabel_count(uint8_t *__restrict__ labels,
+                                                         const int32_t *__restrict__ parent, CCGeom cg,
+                                                         uint32_t root_seed, uint32_t seed_seed, uint32_t skey,
+                                                         uint32_t sweep, int q, int seeded, int forced,
+                                                         uint32_t cap, int *__restrict__ flag,
+                                                         long long *__restrict__ partials,
+                                                         int32_t *__restrict__ sizes, int32_t *__restrict__ ext,
+                                                         const uint8_t *__restrict__ cross, uint8_t tag,
+                                                         unsigned long long *__restrict__ hist) {
+    extern __shared__ int32_t P[];
+    __shared__ int32_t okey[SZ_SLOTS], ocnt[SZ_SLOTS];
+    __shared__ unsigned int hb[40];
+    __shared__ unsigned int nclu, largest;
+    __shared__ int32_t seed_root;
+    __shared__ int wsum[16];
+    if (threadIdx.x < 40) hb[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        nclu = 0;
+        largest = 0;
+        if (seeded) seed_root = seed_cluster_root(parent, cg.N, seed_seed, sweep, cap, flag);
+    }
+    for (int i = threadIdx.x; i < SZ_SLOTS; i += blockDim.x) {
+        okey[i] = -1;
+        ocnt[i] = 0;
+    }
+    const uint32_t c = threadIdx.x;
+    const uint32_t l0 = c << 4;
+    LaneHist lh;
+    long long moved = 0;
+    bool over = false;
+    for (int64_t box = blockIdx.x; box < cg.nboxes; box += gridDim.x) {
+        const X16Box<D> bx(cg, (uint32_t)box);
+        const bool act = c < (bx.nloc >> 4);
+        const uint32_t gsite0 = bx.locate(cg, l0);
         __syncthreads();  // the previous box's counts are out
         if (act) box_load_parents<D>(P, parent, l0, gsite0);
         __syncthreads();
         int32_t root[16];
         if (act) {
-            int32_t prev = -1, r = -1;
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const int32_t p = P[l0 + (uint32_t)k];
-                if (p != prev) {
-                    prev = p;
-                    const uint32_t sl = slot(p);
-                    r = sl != ~0u ? P[sl] : p;
-                }
-                root[k] = r;
-            }
+            box_roots16<D>(P, cg, bx, l0, root, cap, over);
+            uint4 *dst = reinterpret_cast<uint4 *>(labels + gsite0);
+            const uint4 l4 = *dst;
+            uint32_t w[4] = {l4.x, l4.y, l4.z, l4.w};
+            const int cnt = box_relabel_roots16(root, w, root_seed, skey, q, seeded, forced, seed_root);
+            if (cnt) *dst = make_uint4(w[0], w[1], w[2], w[3]);
+            moved += cnt;
         }
-        __syncthreads();  // every read of the parents is done: P becomes the counts
-        if (act) {
-#pragma unroll
-            for (int k = 0; k < 16; ++k) P[l0 + (uint32_t)k] = 0;
-        }
-        __syncthreads();
-        if (act) {
-            int len = 1;
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                if (k == 15 || root[k + 1] != root[k]) {
-                    const int32_t r = root[k];
-                    const uint32_t sl = slot(r);
-                    if (sl != ~0u) {
-                        atomicAdd(&P[sl], len);
-                    } else {
-                        const uint32_t sub = (threadIdx.x & 7u) * SZ_SUB;
-                        uint32_t h = ((uint32_t)r * 0x9e3779b1u) >> (32 - 7);
-                        bool done = false;
-                        for (int probe = 0; probe < 8 && !done; ++probe, h = (h + 1u) & (SZ_SUB - 1)) {
-                            int32_t key = okey[sub + h];
-                            if (key == -1) {
-                                const int32_t old = atomicCAS(&okey[sub + h], -1, r);
-                                key = old == -1 ? r : old;
-                            }
-                            if (key == r) {
-                                atomicAdd(&ocnt[sub + h], len);
-                                done = true;
-                            }
-                        }
-                        if (!done) atomicAdd(&ext[r], len);  // table crowded: straight to the global count
-                    }
-                    len = 1;
-                } else {
-                    ++len;
-                }
-            }
-        }
-        __syncthreads();
-        if (act) {
-            // the segment's 16 crossing marks in one load
-            const uint4 cm = *reinterpret_cast<const uint4 *>(cross + gsite0);
-            const uint32_t t4 = 0x01010101u * tag;
-            const uint32_t crossing = equal_bytes4(cm.x, t4) | (equal_bytes4(cm.y, t4) << 4) |
-                                      (equal_bytes4(cm.z, t4) << 8) | (equal_bytes4(cm.w, t4) << 12);
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const int32_t n = P[l0 + (uint32_t)k];
-                if (!n) continue;
-                if ((crossing >> k) & 1u)
-                    sizes[gsite0 + (uint32_t)k] = n;  // cc_hist_cross adds the other boxes' share and bins it
-                else
-                    lh.add((uint32_t)n, hb);
-            }
-        }
-        // sub-tables 1..7 fold into sub-table 0, so a root goes to global
-        // memory once per box (the percolating root's ext counter takes one
-        // atomic per box, not eight)
-        __syncthreads();
-        for (int i = SZ_SUB + threadIdx.x; i < SZ_SLOTS; i += blockDim.x) {
-            const int32_t r = okey[i];
-            if (r == -1) continue;
-            uint32_t h = ((uint32_t)r * 0x9e3779b1u) >> (32 - 7);
-            bool done = false;
-            for (int probe = 0; probe < 8 && !done; ++probe, h = (h + 1u) & (SZ_SUB - 1)) {
-                int32_t key = okey[h];
-                if (key == -1) {
-                    const int32_t old = atomicCAS(&okey[h], -1, r);
-                    key = old == -1 ? r : old;
-                }
-                if (key == r) {
-                    atomicAdd(&ocnt[h], ocnt[i]);
-                    done = true;
-                }
-            }
-            if (!done) atomicAdd(&ext[r], ocnt[i]);
-            okey[i] = -1;
-            ocnt[i] = 0;
-        }
-        __syncthreads();
-        for (int i = threadIdx.x; i < SZ_SUB; i += blockDim.x) {
-            if (okey[i] != -1) {
-                atomicAdd(&ext[okey[i]], ocnt[i]);
-                okey[i] = -1;
-                ocnt[i] = 0;
-            }
-        }
+        box_count16<D>(P, okey, ocnt, cg, bx, act, l0, gsite0, root, sizes, ext, cross, tag, lh, hb);
     }
+    if (over) atomicOr(flag, 1);
     lh.flush(hb, &nclu, &largest);
     __syncthreads();
     hist_row(hb, nclu, largest, hist + (int64_t)blockIdx.x * HIST_K);
+    box_partials(moved, wsum, partials);
 }
 
 // The crossing clusters' share of the histogram: every site marked with
@@ -1422,6 +1538,22 @@ int64_t launch_cluster_sweep(const ClusterSweepArgs &a) {
 #undef PLQ_CC
     int64_t need = (cg.N + 255) / 256;
     int64_t G = need < a.max_blocks ? need : a.max_blocks;
+    if (x16 && a.count_sizes) {
+        const int64_t Gb = cg.nboxes < a.max_blocks ? cg.nboxes : a.max_blocks;
+        const uint8_t tag = cross_tag(a.sweep);
+        switch (cg.D) {
+#define PLQ_RC(DD)                                                                                           \
+    cc_relabel_count<DD><<<dim3((uint32_t)Gb), bl, lds, a.stream>>>(                                       \
+        a.labels, a.parent, cg, a.seed32 ^ ROOT_SALT, a.seed32 ^ SEED_SALT, skey, a.sweep, a.q, a.seeded,   \
+        a.forced, a.cap, a.flag, a.partials, a.sizes, a.ext, a.cross, tag, a.hist_part)
+            case 1: PLQ_RC(1); break;
+            case 2: PLQ_RC(2); break;
+            case 3: PLQ_RC(3); break;
+            default: PLQ_RC(4); break;
+#undef PLQ_RC
+        }
+        return Gb;
+    }
     if (x16) {
         const int64_t Gb = cg.nboxes < a.max_blocks ? cg.nboxes : a.max_blocks;
         switch (cg.D) {
@@ -1495,12 +1627,12 @@ uint8_t cross_tag(uint32_t sweep) { return (uint8_t)(1u + sweep % 255u); }
 void launch_cluster_sizes_box(const int32_t *parent, const CCGeom &cg, int32_t *sizes, int32_t *ext,
                               const uint8_t *cross, uint32_t sweep, unsigned long long *part,
                               const long long *relabel_part, int64_t relabel_rows, unsigned long long *hist,
-                              int64_t max_blocks, hipStream_t s) {
+                              int64_t max_blocks, bool counted, hipStream_t s) {
     const int64_t Gb = cg.nboxes < max_blocks ? cg.nboxes : max_blocks;
     const size_t lds = (size_t)cg.box_sites * sizeof(int32_t);
     const uint8_t tag = cross_tag(sweep);
     const dim3 g((uint32_t)Gb), b(1024);
-    switch (cg.D) {
+    if (!counted) switch (cg.D) {
         case 1: cc_sizes_box<1><<<g, b, lds, s>>>(parent, cg, sizes, ext, cross, tag, part); break;
         case 2: cc_sizes_box<2><<<g, b, lds, s>>>(parent, cg, sizes, ext, cross, tag, part); break;
         case 3: cc_sizes_box<3><<<g, b, lds, s>>>(parent, cg, sizes, ext, cross, tag, part); break;

@@ -204,6 +204,9 @@ struct ClusterSweepArgs {
     uint8_t *labels;      // lattice order
     int32_t *parent;      // union-find forest, lattice order
     uint8_t *cross;       // nullable: N bytes, cc_compress marks the roots of face-crossing clusters
+    bool count_sizes;     // the relabel also counts cluster sizes (box kernels only) into:
+    int32_t *sizes, *ext;
+    unsigned long long *hist_part;  // per-workgroup histogram rows
     CCGeom cg;
     uint32_t seed32;
     uint32_t sweep;
@@ -235,10 +238,11 @@ bool cluster_boxes_x16(const CCGeom &cg, const void *labels);
 // part: scratch for 2 * max_blocks rows of 42 partials; hist[42 ..] also
 // takes the relabel's partials summed (relabel_rows rows of REC_K; only the
 // relabelled-site column is kept).
+// counted: the sweep's relabel already counted the sizes (ClusterSweepArgs::count_sizes).
 void launch_cluster_sizes_box(const int32_t *parent, const CCGeom &cg, int32_t *sizes, int32_t *ext,
                               const uint8_t *cross, uint32_t sweep, unsigned long long *part,
                               const long long *relabel_part, int64_t relabel_rows, unsigned long long *hist,
-                              int64_t max_blocks, hipStream_t s);
+                              int64_t max_blocks, bool counted, hipStream_t s);
 uint8_t cross_tag(uint32_t sweep);
 void launch_cluster_resolve(int32_t *parent, int32_t *roots, const CCGeom &cg, uint32_t cap, int *flag,
                             int32_t *sizes, unsigned long long *hist, hipStream_t s);
