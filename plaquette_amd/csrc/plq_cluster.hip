@@ -1207,8 +1207,6 @@ __device__ __forceinline__ void box_count16(int32_t *P, int32_t *okey, int32_t *
                                             const int32_t (&root)[16], int32_t *__restrict__ sizes,
                                             int32_t *__restrict__ ext, const uint8_t *__restrict__ cross,
                                             uint8_t tag, LaneHist &lh, unsigned int *hb) {
-    const uint32_t c = threadIdx.x;
-    (void)c;
     // box-local slot of global site g, or ~0u outside the box
     auto slot = [&](int32_t g) -> uint32_t {
         uint32_t local = 0;

@@ -430,6 +430,41 @@ def test_recorded_cluster_statistics_match_every_sweep(gpu, shape, q, seeded):
     c.close()
 
 
+@pytest.mark.parametrize("shape,q,seeded", [([256, 256], 3, False), ([32, 32, 32], 2, True)])
+def test_recorded_cluster_statistics_every_third_sweep_with_calls_between(gpu, shape, q, seeded):
+    # A record every third sweep: the recorded sweeps count their clusters
+    # in the relabel, the others do not.  A statistics call after every
+    # sweep, recorded or not, and the kept series both equal the oracle.
+    n = int(np.prod(shape))
+    beta = 1.0050525 if q == 3 else 0.4406868
+    start = start_config(n, q, True, 29)
+    extent = gpu.Extent.SEEDED if seeded else gpu.Extent.ALL
+    relabel = gpu.Relabel.FORCED_CHANGE if seeded else gpu.Relabel.REDRAW
+    c = gpu.GpuClusterChain.new(shape, q, 1.0, extent, relabel, beta, 47, start)
+    d = O.cluster_desc(shape, q=q, seeded=seeded, forced=seeded, beta=beta, gap=1.0, seed=47)
+    c.record(3, 8)
+    c.record_clusters(True)
+    st, want = start, []
+
+    def summary(roots):
+        sizes = np.bincount(roots)
+        sizes = sizes[sizes > 0]
+        return sizes.size, int(sizes.max()), np.bincount(np.floor(np.log2(sizes)).astype(int), minlength=40).tolist()
+
+    for t in range(9):
+        c.advance(1)
+        st, roots = O.cluster_sweeps(d, st, t, 1)
+        s = c.stats()
+        assert (s.n_clusters, s.largest, list(s.size_hist)) == summary(roots), t
+        if (t + 1) % 3 == 0:
+            want.append((t + 1, summary(roots)))
+    got = c.cluster_series()
+    assert [g.sweep for g in got] == [w[0] for w in want]
+    for g, (_, w) in zip(got, want):
+        assert (g.n_clusters, g.largest, list(g.size_hist)) == w
+    c.close()
+
+
 def test_cluster_budget_exhaustion_is_reported(gpu):
     # gpu_cluster.rs:512-518: labelling past its cap is an error, forced from
     # the host with a zero hop budget on an ordered (percolating) lattice
