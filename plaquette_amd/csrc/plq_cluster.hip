@@ -711,18 +711,24 @@ __global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, 
 // 16 sites of one axis-0 row segment (its labels in w, updated in place);
 // consecutive sites with the same parent share the draw.  Returns the
 // relabelled sites; the caller synchronises before P is reused.
+//
+// A lane's own 16 words of P move as four 128-bit LDS accesses (P is
+// declared 16-byte aligned and l0 is a multiple of 16): 32-bit accesses at
+// the 16-word lane stride would meet on a few LDS banks.
 template <int D>
 __device__ __forceinline__ void box_load_parents(int32_t *P, const int32_t *__restrict__ parent, uint32_t l0,
                                                  uint32_t gsite0) {
     const int4 *src = reinterpret_cast<const int4 *>(parent + gsite0);
+    int4 *dst = reinterpret_cast<int4 *>(P + l0);
 #pragma unroll
-    for (int d = 0; d < 4; ++d) {
-        const int4 v = src[d];
-        P[l0 + 4 * d] = v.x;
-        P[l0 + 4 * d + 1] = v.y;
-        P[l0 + 4 * d + 2] = v.z;
-        P[l0 + 4 * d + 3] = v.w;
-    }
+    for (int d = 0; d < 4; ++d) dst[d] = src[d];
+}
+
+// Word k of a lane's 16 LDS words, through the 128-bit load that holds it
+// (k is a constant once the caller's loop is unrolled).
+__device__ __forceinline__ int32_t lane_word(const int32_t *P, uint32_t l0, int k, int4 &q) {
+    if ((k & 3) == 0) q = reinterpret_cast<const int4 *>(P + l0)[k >> 2];
+    return (k & 3) == 0 ? q.x : (k & 3) == 1 ? q.y : (k & 3) == 2 ? q.z : q.w;
 }
 
 // The new labels of a lane's 16 sites (w, in place), the roots found as
@@ -734,9 +740,10 @@ __device__ __forceinline__ int box_relabel16(const int32_t *P, const CCGeom &cg,
     int32_t prev = -1;
     uint32_t nl = NO_LABEL;
     int cnt = 0;
+    int4 q4;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-        const int32_t p = P[l0 + (uint32_t)k];
+        const int32_t p = lane_word(P, l0, k, q4);
         if (p != prev) {
             prev = p;
             // p inside the box: its parent, from LDS; else p is a root
@@ -769,9 +776,10 @@ template <int D>
 __device__ __forceinline__ void box_roots16(const int32_t *P, const CCGeom &cg, const X16Box<D> &bx, uint32_t l0,
                                             int32_t (&root)[16], uint32_t cap, bool &over) {
     int32_t prev = -1, r = -1;
+    int4 q4;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-        const int32_t p = P[l0 + (uint32_t)k];
+        const int32_t p = lane_word(P, l0, k, q4);
         if (p != prev) {
             prev = p;
             // p inside the box: its parent, from LDS; else p is a root
@@ -841,7 +849,8 @@ __global__ __launch_bounds__(1024, 8) void cc_relabel_box(uint8_t *__restrict__ 
                                                        uint32_t root_seed, uint32_t seed_seed, uint32_t skey,
                                                        uint32_t sweep, int q, int seeded, int forced, uint32_t cap,
                                                        int *__restrict__ flag, long long *__restrict__ partials) {
-    extern __shared__ int32_t P[];
+    extern __shared__ int4 box_lds[];  // its own name: extern LDS arrays of one name share one alignment
+    int32_t *P = reinterpret_cast<int32_t *>(box_lds);
     __shared__ int32_t seed_root;
     __shared__ int wsum[16];
     if (seeded && threadIdx.x == 0) seed_root = seed_cluster_root(parent, cg.N, seed_seed, sweep, cap, flag);
@@ -1222,7 +1231,7 @@ __device__ __forceinline__ void box_count16(int32_t *P, int32_t *okey, int32_t *
     __syncthreads();  // every read of the parents is done: P becomes the counts
     if (act) {
 #pragma unroll
-        for (int k = 0; k < 16; ++k) P[l0 + (uint32_t)k] = 0;
+        for (int d = 0; d < 4; ++d) reinterpret_cast<int4 *>(P + l0)[d] = make_int4(0, 0, 0, 0);
     }
     __syncthreads();
     if (act) {
@@ -1264,9 +1273,10 @@ __device__ __forceinline__ void box_count16(int32_t *P, int32_t *okey, int32_t *
         const uint32_t t4 = 0x01010101u * tag;
         const uint32_t crossing = equal_bytes4(cm.x, t4) | (equal_bytes4(cm.y, t4) << 4) |
                                   (equal_bytes4(cm.z, t4) << 8) | (equal_bytes4(cm.w, t4) << 12);
+        int4 q4;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            const int32_t n = P[l0 + (uint32_t)k];
+            const int32_t n = lane_word(P, l0, k, q4);
             if (!n) continue;
             if ((crossing >> k) & 1u)
                 sizes[gsite0 + (uint32_t)k] = n;  // cc_hist_cross adds the other boxes' share and bins it
@@ -1313,7 +1323,8 @@ __global__ __launch_bounds__(1024, 8) void cc_sizes_box(const int32_t *__restric
                                                      int32_t *__restrict__ sizes, int32_t *__restrict__ ext,
                                                      const uint8_t *__restrict__ cross, uint8_t tag,
                                                      unsigned long long *__restrict__ hist) {  // hist: this block's row
-    extern __shared__ int32_t P[];
+    extern __shared__ int4 box_lds[];  // its own name: extern LDS arrays of one name share one alignment
+    int32_t *P = reinterpret_cast<int32_t *>(box_lds);
     __shared__ int32_t okey[SZ_SLOTS], ocnt[SZ_SLOTS];
     __shared__ unsigned int hb[40];
     __shared__ unsigned int nclu, largest;
@@ -1359,7 +1370,8 @@ __global__ __launch_bounds__(1024, 8) void cc_relabel_count(uint8_t *__restrict_
                                                          int32_t *__restrict__ sizes, int32_t *__restrict__ ext,
                                                          const uint8_t *__restrict__ cross, uint8_t tag,
                                                          unsigned long long *__restrict__ hist) {
-    extern __shared__ int32_t P[];
+    extern __shared__ int4 box_lds[];  // its own name: extern LDS arrays of one name share one alignment
+    int32_t *P = reinterpret_cast<int32_t *>(box_lds);
     __shared__ int32_t okey[SZ_SLOTS], ocnt[SZ_SLOTS];
     __shared__ unsigned int hb[40];
     __shared__ unsigned int nclu, largest;
