@@ -58,8 +58,15 @@ __device__ __forceinline__ void lds_store(int32_t *p, int32_t v) {
 // Root of x with path splitting: every node on the walk is re-pointed at its
 // grandparent (an ancestor, so parent <= child still holds under concurrent
 // unions); the grandparent just read is the next node's parent.
-__device__ __forceinline__ int32_t lds_find(int32_t *P, int32_t x) {
-    int32_t p = lds_load(&P[x]);
+// Word k of a lane's 16 LDS words (16-byte aligned), through the 128-bit load that holds it
+// (k is a constant once the caller's loop is unrolled).
+__device__ __forceinline__ int32_t lane_word(const int32_t *P, uint32_t l0, int k, int4 &q) {
+    if ((k & 3) == 0) q = reinterpret_cast<const int4 *>(P + l0)[k >> 2];
+    return (k & 3) == 0 ? q.x : (k & 3) == 1 ? q.y : (k & 3) == 2 ? q.z : q.w;
+}
+
+// p: x's parent as read earlier (any ancestor of x will do).
+__device__ __forceinline__ int32_t lds_find_from(int32_t *P, int32_t x, int32_t p) {
     while (p != x) {
         const int32_t gp = lds_load(&P[p]);
         if (gp != p) lds_store(&P[x], gp);
@@ -68,6 +75,8 @@ __device__ __forceinline__ int32_t lds_find(int32_t *P, int32_t x) {
     }
     return x;
 }
+
+__device__ __forceinline__ int32_t lds_find(int32_t *P, int32_t x) { return lds_find_from(P, x, lds_load(&P[x])); }
 
 __device__ __forceinline__ void lds_union(int32_t *P, int32_t a, int32_t b) {
     for (;;) {
@@ -328,12 +337,18 @@ __device__ __forceinline__ void x16_label(int32_t *P, uint16_t *H, int32_t *__re
     }
     if (act) {
         H[c] = (uint16_t)(h & 0x7fffu);
-        int32_t run = (int32_t)l0;
+        // the lane's 16 words as four 128-bit stores (P is 16-byte aligned);
+        // 32-bit stores at the 16-word lane stride meet on a few LDS banks
+        int32_t run[16];
+        int32_t r = (int32_t)l0;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            if ((starts >> k) & 1u) run = (int32_t)(l0 + (uint32_t)k);
-            P[l0 + (uint32_t)k] = run;
+            if ((starts >> k) & 1u) r = (int32_t)(l0 + (uint32_t)k);
+            run[k] = r;
         }
+        int4 *own = reinterpret_cast<int4 *>(P + l0);
+#pragma unroll
+        for (int d = 0; d < 4; ++d) own[d] = make_int4(run[4 * d], run[4 * d + 1], run[4 * d + 2], run[4 * d + 3]);
     }
     __syncthreads();
     if (act) {
@@ -363,12 +378,16 @@ __device__ __forceinline__ void x16_label(int32_t *P, uint16_t *H, int32_t *__re
     }
     __syncthreads();
     if (act) {
+        // the run starts' parents in four 128-bit loads; no union runs in
+        // this phase, so a parent read here is an ancestor with the same root
+        int4 q4;
         int32_t out[16];
         int32_t g = 0;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
+            const int32_t p0 = lane_word(P, l0, k, q4);
             if ((starts >> k) & 1u) {
-                const uint32_t r = (uint32_t)lds_find(P, (int32_t)(l0 + (uint32_t)k));
+                const uint32_t r = (uint32_t)lds_find_from(P, (int32_t)(l0 + (uint32_t)k), p0);
                 g = (int32_t)(r == l0 + (uint32_t)k ? gsite0 + (uint32_t)k : bx.locate(cg, r));
             }
             out[k] = g;
@@ -386,7 +405,8 @@ __device__ __forceinline__ void x16_label(int32_t *P, uint16_t *H, int32_t *__re
 template <int D>
 __global__ __launch_bounds__(1024) void cc_local_x16(const uint8_t *__restrict__ labels, int32_t *__restrict__ parent,
                                                      CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t) {
-    extern __shared__ int32_t P[];
+    extern __shared__ int4 box_lds[];  // its own name: extern LDS arrays of one name share one alignment
+    int32_t *P = reinterpret_cast<int32_t *>(box_lds);
     __shared__ uint16_t H[1024];
     const X16Box<D> bx(cg, blockIdx.x);
     uint4 me = make_uint4(0u, 0u, 0u, 0u);
@@ -722,13 +742,6 @@ __device__ __forceinline__ void box_load_parents(int32_t *P, const int32_t *__re
     int4 *dst = reinterpret_cast<int4 *>(P + l0);
 #pragma unroll
     for (int d = 0; d < 4; ++d) dst[d] = src[d];
-}
-
-// Word k of a lane's 16 LDS words, through the 128-bit load that holds it
-// (k is a constant once the caller's loop is unrolled).
-__device__ __forceinline__ int32_t lane_word(const int32_t *P, uint32_t l0, int k, int4 &q) {
-    if ((k & 3) == 0) q = reinterpret_cast<const int4 *>(P + l0)[k >> 2];
-    return (k & 3) == 0 ? q.x : (k & 3) == 1 ? q.y : (k & 3) == 2 ? q.z : q.w;
 }
 
 // The new labels of a lane's 16 sites (w, in place), the roots found as
