@@ -1120,12 +1120,18 @@ __global__ __launch_bounds__(256) void cc_hist(const int32_t *__restrict__ roots
 // largest), summed by cc_hist_reduce: with thousands of workgroups, atomics
 // on the few hot bins would serialise at their cache line.
 constexpr int HIST_K = 42;
+// The workgroups' histogram rows are stored column-major, entry c of row r
+// at part[c * HIST_ROWS + r], so that cc_hist_reduce reads each column
+// contiguously; the statistics kernels of one call write at most
+// 2 * MAX_PASS_BLOCKS rows.
+constexpr int64_t HIST_ROWS = 2 * (int64_t)MAX_PASS_BLOCKS;
+
 __device__ __forceinline__ void hist_row(const unsigned int *h, unsigned int nclu, unsigned int largest,
-                                         unsigned long long *row) {
-    if (threadIdx.x < 40) row[threadIdx.x] = h[threadIdx.x];
+                                         unsigned long long *part, int64_t row) {
+    if (threadIdx.x < 40) part[threadIdx.x * HIST_ROWS + row] = h[threadIdx.x];
     if (threadIdx.x == 0) {
-        row[40] = nclu;
-        row[41] = largest;
+        part[40 * HIST_ROWS + row] = nclu;
+        part[41 * HIST_ROWS + row] = largest;
     }
 }
 
@@ -1144,7 +1150,7 @@ __global__ __launch_bounds__(256) void cc_hist_reduce(const unsigned long long *
         for (int64_t r = threadIdx.x; r < rrows; r += blockDim.x) v += (unsigned long long)rpart[r * REC_K + 1];
     } else {
         for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) {
-            const unsigned long long x = part[r * HIST_K + c];
+            const unsigned long long x = part[c * HIST_ROWS + r];
             v = c == 41 ? (x > v ? x : v) : v + x;
         }
     }
@@ -1367,7 +1373,7 @@ __global__ __launch_bounds__(1024, 8) void cc_sizes_box(const int32_t *__restric
     }
     lh.flush(hb, &nclu, &largest);
     __syncthreads();
-    hist_row(hb, nclu, largest, hist + (int64_t)blockIdx.x * HIST_K);
+    hist_row(hb, nclu, largest, hist, blockIdx.x);
 }
 
 // One sweep's relabel (as cc_relabel_box) and the counting of its clusters
@@ -1427,7 +1433,7 @@ __global__ __launch_bounds__(1024, 8) void cc_relabel_count(uint8_t *__restrict_
     if (over) atomicOr(flag, 1);
     lh.flush(hb, &nclu, &largest);
     __syncthreads();
-    hist_row(hb, nclu, largest, hist + (int64_t)blockIdx.x * HIST_K);
+    hist_row(hb, nclu, largest, hist, blockIdx.x);
     box_partials(moved, wsum, partials);
 }
 
@@ -1440,7 +1446,7 @@ __global__ __launch_bounds__(1024, 8) void cc_relabel_count(uint8_t *__restrict_
 __global__ __launch_bounds__(256) void cc_hist_cross(const int32_t *__restrict__ parent,
                                                      const int32_t *__restrict__ sizes, int32_t *__restrict__ ext,
                                                      const uint8_t *__restrict__ cross, uint8_t tag, int64_t N,
-                                                     unsigned long long *__restrict__ hist) {
+                                                     unsigned long long *__restrict__ hist, int64_t row0) {
     __shared__ unsigned int h[40];
     __shared__ unsigned int nclu, largest;
     if (threadIdx.x < 40) h[threadIdx.x] = 0;
@@ -1477,7 +1483,7 @@ __global__ __launch_bounds__(256) void cc_hist_cross(const int32_t *__restrict__
         if (cross[s] == tag) visit(s);
     lh.flush(h, &nclu, &largest);
     __syncthreads();
-    hist_row(h, nclu, largest, hist + (int64_t)blockIdx.x * HIST_K);
+    hist_row(h, nclu, largest, hist, row0 + blockIdx.x);
 }
 
 // ------------------------------------------------------------ launchers ---
@@ -1651,6 +1657,7 @@ void launch_cluster_sizes_box(const int32_t *parent, const CCGeom &cg, int32_t *
                               const uint8_t *cross, uint32_t sweep, unsigned long long *part,
                               const long long *relabel_part, int64_t relabel_rows, unsigned long long *hist,
                               int64_t max_blocks, bool counted, hipStream_t s) {
+    if (max_blocks > MAX_PASS_BLOCKS) max_blocks = MAX_PASS_BLOCKS;  // the rows fit HIST_ROWS
     const int64_t Gb = cg.nboxes < max_blocks ? cg.nboxes : max_blocks;
     const size_t lds = (size_t)cg.box_sites * sizeof(int32_t);
     const uint8_t tag = cross_tag(sweep);
@@ -1663,8 +1670,7 @@ void launch_cluster_sizes_box(const int32_t *parent, const CCGeom &cg, int32_t *
     }
     int64_t hb = (cg.N + 4095) / 4096 < max_blocks ? (cg.N + 4095) / 4096 : max_blocks;
     if (hb < 1) hb = 1;
-    cc_hist_cross<<<dim3((uint32_t)hb), dim3(256), 0, s>>>(parent, sizes, ext, cross, tag, cg.N,
-                                                           part + Gb * HIST_K);
+    cc_hist_cross<<<dim3((uint32_t)hb), dim3(256), 0, s>>>(parent, sizes, ext, cross, tag, cg.N, part, Gb);
     cc_hist_reduce<<<dim3(HIST_K + 1), dim3(256), 0, s>>>(part, Gb + hb, relabel_part, relabel_rows, hist);
 }
 

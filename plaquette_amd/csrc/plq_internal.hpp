@@ -206,7 +206,7 @@ struct ClusterSweepArgs {
     uint8_t *cross;       // nullable: N bytes, cc_compress marks the roots of face-crossing clusters
     bool count_sizes;     // the relabel also counts cluster sizes (box kernels only) into:
     int32_t *sizes, *ext;
-    unsigned long long *hist_part;  // per-workgroup histogram rows
+    unsigned long long *hist_part;  // per-workgroup histogram rows (as launch_cluster_sizes_box's part)
     CCGeom cg;
     uint32_t seed32;
     uint32_t sweep;
@@ -235,7 +235,8 @@ bool cluster_boxes_x16(const CCGeom &cg, const void *labels);
 // the caller; ext all zero on entry and on return).
 // cross: the marks cc_compress leaves on the roots of face-crossing clusters
 // (N bytes, tag cross_tag(sweep) of the sweep whose partition is counted).
-// part: scratch for 2 * max_blocks rows of 42 partials; hist[42 ..] also
+// part: scratch for 2 * MAX_PASS_BLOCKS rows of 42 partials (stored
+// column-major, 42 * 2 * MAX_PASS_BLOCKS entries); hist[42 ..] also
 // takes the relabel's partials summed (relabel_rows rows of REC_K; only the
 // relabelled-site column is kept).
 // counted: the sweep's relabel already counted the sizes (ClusterSweepArgs::count_sizes).
