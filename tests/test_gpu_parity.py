@@ -430,7 +430,8 @@ def test_recorded_cluster_statistics_match_every_sweep(gpu, shape, q, seeded):
     c.close()
 
 
-@pytest.mark.parametrize("shape,q,seeded", [([256, 256], 3, False), ([32, 32, 32], 2, True)])
+@pytest.mark.parametrize("shape,q,seeded", [([256, 256], 3, False), ([32, 32, 32], 2, True), ([65536], 2, False),
+                                            ([16, 8, 8, 16], 3, False)])
 def test_recorded_cluster_statistics_every_third_sweep_with_calls_between(gpu, shape, q, seeded):
     # A record every third sweep: the recorded sweeps count their clusters
     # in the relabel, the others do not.  A statistics call after every
