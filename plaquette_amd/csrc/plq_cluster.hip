@@ -1253,6 +1253,28 @@ __device__ __forceinline__ void box_count16(int32_t *P, int32_t *okey, int32_t *
         for (int d = 0; d < 4; ++d) reinterpret_cast<int4 *>(P + l0)[d] = make_int4(0, 0, 0, 0);
     }
     __syncthreads();
+    // n sites of the outside root r into the lane's sub-table
+    auto outside_add = [&](int32_t r, int n) {
+        const uint32_t sub = (threadIdx.x & 7u) * SZ_SUB;
+        uint32_t h = ((uint32_t)r * 0x9e3779b1u) >> (32 - 7);
+        bool done = false;
+        for (int probe = 0; probe < 8 && !done; ++probe, h = (h + 1u) & (SZ_SUB - 1)) {
+            int32_t key = okey[sub + h];
+            if (key == -1) {
+                const int32_t old = atomicCAS(&okey[sub + h], -1, r);
+                key = old == -1 ? r : old;
+            }
+            if (key == r) {
+                atomicAdd(&ocnt[sub + h], n);
+                done = true;
+            }
+        }
+        if (!done) atomicAdd(&ext[r], n);  // table crowded: straight to the global count
+    };
+    // Runs of one outside root (a percolating cluster's, in most boxes) are
+    // summed in registers while they follow each other in the segment
+    int32_t orr = -1;
+    int olen = 0;
     if (act) {
         int len = 1;
 #pragma unroll
@@ -1262,28 +1284,33 @@ __device__ __forceinline__ void box_count16(int32_t *P, int32_t *okey, int32_t *
                 const uint32_t sl = slot(r);
                 if (sl != ~0u) {
                     atomicAdd(&P[sl], len);
+                } else if (r == orr) {
+                    olen += len;
                 } else {
-                    const uint32_t sub = (threadIdx.x & 7u) * SZ_SUB;
-                    uint32_t h = ((uint32_t)r * 0x9e3779b1u) >> (32 - 7);
-                    bool done = false;
-                    for (int probe = 0; probe < 8 && !done; ++probe, h = (h + 1u) & (SZ_SUB - 1)) {
-                        int32_t key = okey[sub + h];
-                        if (key == -1) {
-                            const int32_t old = atomicCAS(&okey[sub + h], -1, r);
-                            key = old == -1 ? r : old;
-                        }
-                        if (key == r) {
-                            atomicAdd(&ocnt[sub + h], len);
-                            done = true;
-                        }
-                    }
-                    if (!done) atomicAdd(&ext[r], len);  // table crowded: straight to the global count
+                    if (olen) outside_add(orr, olen);
+                    orr = r;
+                    olen = len;
                 }
                 len = 1;
             } else {
                 ++len;
             }
         }
+    }
+    {
+        // then the wave's pending runs of its first pending root go in as one
+        const uint64_t pend = __ballot(olen > 0);
+        if (pend) {
+            const int leader = __ffsll((unsigned long long)pend) - 1;
+            const int32_t r0 = __shfl(orr, leader, 64);
+            const bool same = olen > 0 && orr == r0;
+            int v = same ? olen : 0;
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+            if (same) olen = 0;
+            if ((int)(threadIdx.x & 63u) == leader) olen = v;
+        }
+        if (olen) outside_add(orr, olen);
     }
     __syncthreads();
     if (act) {
