@@ -47,13 +47,27 @@ def test_lowbias32_is_the_published_bijection():
 
 
 def test_keyed_draw_composes_two_hashes_and_is_uniform():
-    # keyed(seed, var, sweep) = lowbias32(lowbias32(seed ^ var) ^ sweep*0x9e3779b9) >> 8:
-    # undo the outer hash and the inner one to recover seed ^ var
+    # keyed(seed, var, sweep) = lowbias32(lowbias32(seed ^ var) ^ sweep*0x9e3779b9) >> 8
+    # (rng.wgsl:47-52).  The draw keeps the top 24 bits of the outer hash, so
+    # try all 256 values of the dropped low byte: undo the outer hash with the
+    # independent inverse, remove the sweep key, undo the inner hash, and
+    # exactly the true seed ^ var must come back for some completion.  A
+    # missing or swapped round, a wrong sweep constant or a wrong shift fails.
     seed = 0xDEADBEEF
-    for var, sweep in [(0, 0), (5, 0), (12345, 7), (1 << 20, 999)]:
+    cases = [(0, 0), (5, 0), (12345, 7), (1 << 20, 999), (0xFFFFFFFF, 1), (77, 0xFFFFFFFF)]
+    rng = np.random.default_rng(3)
+    cases += [(int(v), int(s)) for v, s in zip(rng.integers(0, 1 << 32, 40), rng.integers(0, 1 << 32, 40))]
+    for var, sweep in cases:
         m = O.keyed_bits(seed, var, sweep)
         assert 0 <= m < (1 << 24)
         assert O.keyed_uniform(seed, var, sweep) == m / 16777216.0
+        key = (sweep * 0x9E3779B9) & M32
+        inner = [_inv_lowbias32(_inv_lowbias32((m << 8) | low) ^ key) for low in range(256)]
+        assert (seed ^ var) & M32 in inner, (var, sweep)
+        # every completion is an input that hashes forward to the same draw
+        for x in inner:
+            assert O.keyed_bits(0, x, sweep) == m
+    assert O.keyed_bits(seed, 9, 3) != O.keyed_bits(seed, 9, 4)
     # low-order chi-square over 64 bins of many draws
     ms = np.array([O.keyed_bits(7, v, 3) for v in range(20000)])
     counts = np.bincount(ms >> 18, minlength=64)
@@ -346,6 +360,62 @@ def test_z2_two_dimensional_strong_coupling_is_tanh_beta():
             ps.append(O.plaquette_sum(shape, st) / 64.0)
     est = O.reduce(np.array(ps))
     assert abs(est.mean - math.tanh(beta)) < 5 * est.stderr + 2e-3
+
+
+# ------------------------------------------------------- replica exchange ---
+
+def test_exact_z2_helper_matches_the_oracle_geometry():
+    # the enumerated density of states uses its own plaquette list; pin it
+    # against the oracle's plaquette_links (lattice.rs:420-430) and sums
+    import exact_z2 as X
+
+    dos = X.plaquette_dos(2, 4)
+    assert dos[1].sum() == 1 << 16
+    assert abs(X.mean_plaquette_sum(dos, 0.6) - _exact_z2_plaquette([2, 4], 0.6)) < 1e-9
+    rng = np.random.default_rng(9)
+    plaqs = X.plaquette_links_2d(2, 4)
+    for _ in range(50):
+        st = rng.integers(0, 2, 16).astype(np.uint8)
+        u = 1 - 2 * st.astype(np.int64)
+        assert sum(u[a] * u[b] * u[c] * u[d] for a, b, c, d in plaqs) == O.plaquette_sum([2, 4], st)
+
+
+@pytest.mark.parametrize("betas", [[0.25, 0.55, 0.85], [0.3, 0.9]])
+def test_replica_exchange_samples_the_exact_laws(betas):
+    # Heat-bath replicas of a 2x4 Z2 plane (2^16 states, enumerated) with an
+    # exchange round after every sweep: every beta slot must sample its exact
+    # plaquette law, the swaps must follow min(1, e^delta) of the recorded
+    # sums on the pairs of the round's parity, and each pair's mean
+    # acceptance must equal the exact <min(1, e^delta)>.
+    import exact_z2 as X
+
+    R = len(betas)
+    ens = O.Ensemble(2, [[2, 4]] * R, betas, [31 + r for r in range(R)], kernel=O.KERNEL_HEAT_BATH,
+                     measure_every=1, exchange=True, exchange_seed=17)
+    ens.advance(40000)
+    sums, slots = np.array(ens.records), np.array(ens.slots)
+    rates = X.check_exchange(sums, slots, betas, X.plaquette_dos(2, 4), O.reduce)
+    # the oracle's own counters agree with the swaps seen in the slots
+    seen = np.zeros(R - 1)
+    for t in range(len(slots) - 1):
+        rep_at = np.empty(R, np.int64)
+        rep_at[slots[t]] = np.arange(R)
+        for k in range(t % 2, R - 1, 2):
+            seen[k] += slots[t + 1, rep_at[k]] == k + 1
+    assert np.array_equal(seen, ens.accepted[:R - 1].astype(np.float64) - _last_round_swaps(ens, slots, R))
+    assert all(0.05 < r[1] < 0.95 for r in rates)
+
+
+def _last_round_swaps(ens, slots, R):
+    # the final round's swaps happened after the last record was taken
+    t = len(slots) - 1
+    cur = ens.slot_of
+    out = np.zeros(R - 1)
+    rep_at = np.empty(R, np.int64)
+    rep_at[slots[t]] = np.arange(R)
+    for k in range(t % 2, R - 1, 2):
+        out[k] = cur[rep_at[k]] == k + 1
+    return out
 
 
 # ------------------------------------------------------------- cluster ------
