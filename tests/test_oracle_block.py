@@ -149,3 +149,47 @@ def test_exact_conditional_proposal_is_always_accepted():
     logits = np.full(4 * 64, np.float32(2 * beta * h), np.float32)
     _, acc, _ = _run_mh((16, 16), logits, beta, 0.0, h, 300)
     assert acc > 0.999
+
+
+# Stationarity at J != 0.  pi must be invariant under one block sweep: from
+# (nearly) independent equilibrium samples of the plain heat-bath chain, one
+# block sweep (both colours) must leave <E> unchanged.  The paired shift is a
+# sharp test: a double-counted ring bond, a dropped q-ratio or a dropped
+# in-block bond move it by 10 to 130 standard errors.  Logits: the ring-only
+# mean-field logits of tests/block_logits.py.  (A chain of block sweeps alone
+# is not used: with nearly deterministic proposals it has a slow mode of small
+# amplitude that the tau_int window misses, so its error bars are not
+# trustworthy.)  The moved samples are also compared with Kaufman's exact
+# E/N of the torus where J = 1, h = 0.
+@pytest.mark.parametrize("shape,beta,j,h,n", [((16, 16), 0.6, 1.0, 0.0, 3000),
+                                              ((32, 32), 0.4406868, 1.0, 0.0, 1500),
+                                              ((16, 32), 0.5, 0.8, 0.15, 2000)])
+def test_block_sweep_leaves_the_boltzmann_law_invariant(shape, beta, j, h, n):
+    import block_logits as B
+    import exact_ising2d as X
+
+    N = shape[0] * shape[1]
+    d = O.local_desc(list(shape), kernel=O.KERNEL_HEAT_BATH, beta=beta, j=j, h=h, seed=n)
+    st = O.local_sweeps(d, np.random.default_rng(n).integers(0, 2, N).astype(np.uint8), 0, 300)
+    shifts, after, acc, t = [], [], 0, 300
+    for k in range(n):
+        st = O.local_sweeps(d, st, t, 10)
+        t += 10
+        b0, m0 = O.ising_sums(list(shape), st)
+        x = st
+        for colour in (0, 1):
+            x, a = O.block_mh_pass(shape, x, B.mean_field_logits(shape, x, beta, j, h), colour, 0xB10C + n, k,
+                                   beta, j, h)
+            acc += a
+        b1, m1 = O.ising_sums(list(shape), x)
+        shifts.append((-j * (b1 - b0) - h * (m1 - m0)) / N)
+        after.append((-j * b1 - h * m1) / N)
+    shifts = np.array(shifts)
+    rate = acc / (n * (shape[0] // 8) * (shape[1] // 8))
+    assert rate > 0.1, rate
+    err = shifts.std() / math.sqrt(n)
+    assert abs(shifts.mean()) < 4.5 * err, (shifts.mean(), err, rate)
+    if j == 1.0 and h == 0.0:
+        est = O.reduce(np.array(after))
+        want = X.energy_per_site(shape[0], shape[1], beta)
+        assert abs(est.mean - want) < 5 * est.stderr, (est.mean, want, est.stderr)
