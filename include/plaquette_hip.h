@@ -129,8 +129,13 @@ int plq_chain_create_local(const plq_local_desc *desc, const uint8_t *start, siz
 int plq_chain_create_cluster(const plq_cluster_desc *desc, const uint8_t *start, size_t n,
                              plq_chain **out);
 /* GpuChain::advance / GpuClusterChain::advance (device.rs:387-399,
- * gpu_cluster.rs:477-526): enqueue `sweeps` sweeps, return without waiting.
- * Cluster chains wait only if a labelling needed more passes than its cap. */
+ * gpu_cluster.rs:477-526): enqueue `sweeps` sweeps, return without waiting,
+ * for cluster chains too.  A cluster labelling that exceeds its pass cap sets
+ * a device flag; the next call that waits for the chain (next, next_batch,
+ * read, measure, series, cluster_roots, cluster_stats, cluster_series, sync)
+ * returns PLQ_ENOCONV.  The flag is sticky: like the reference, which panics
+ * (gpu_cluster.rs:512-518), such a chain holds no defined configuration and
+ * every later waiting call returns PLQ_ENOCONV; destroy it. */
 int plq_chain_advance(plq_chain *chain, uint64_t sweeps);
 /* Iterator::next (device.rs:692-701, gpu_cluster.rs:630-637): advance
  * sweeps_between sweeps and copy the configuration out. */

@@ -800,9 +800,10 @@ int plq_chain_advance(plq_chain *c, uint64_t sweeps) {
     if (!c) return fail(PLQ_EINVAL, "chain is null");
     std::lock_guard<std::mutex> lk(c->mu);
     PLQ_HIP(hipSetDevice(c->device));
-    PLQ_TRY(enqueue_sweeps(c, sweeps));
-    if (c->kind == KIND_CLUSTER && sweeps > 0) return check_flag(c);
-    return PLQ_OK;
+    // Enqueue only, for cluster chains too: a labelling that overran its hop
+    // budget sets a sticky device flag, which the next call that waits for
+    // the chain's stream (next, read, measure, series, sync, ...) reports.
+    return enqueue_sweeps(c, sweeps);
 }
 
 int plq_chain_next_batch(plq_chain *c, uint64_t count, uint8_t *out, size_t n) {

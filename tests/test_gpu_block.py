@@ -174,3 +174,140 @@ def test_block_update_samples_the_product_target_at_zero_coupling(gpu):
     assert abs(m - math.tanh(beta * h)) < 0.01
     c.close()
     net.close()
+
+
+# -- the mixed chain (config 5's sampler): block sweeps between plain sweeps -----
+
+MIXED_CASES = [
+    # shape, beta, J, h, plain sweeps per step, kernel; 32^2 takes the
+    # one-workgroup small chain for its plain sweeps, 64x48 the colour passes
+    ((32, 32), BETA_C, 1.0, 0.0, 1, O.KERNEL_METROPOLIS),
+    ((64, 48), 0.35, 0.8, 0.1, 2, O.KERNEL_HEAT_BATH),
+    ((32, 64), 0.6, 1.0, -0.15, 3, O.KERNEL_METROPOLIS),
+]
+
+
+@pytest.mark.parametrize("shape,beta,j,h,plain,kernel", MIXED_CASES)
+def test_mixed_chain_is_bit_exact_given_the_logits(gpu, shape, beta, j, h, plain, kernel):
+    # One advance_with_blocks step at a time: the oracle runs the block pass
+    # of both colours with the logits the device used, then the plain
+    # sweeps with the chain's own sweep counter.
+    net = gpu.BlockNet(51)
+    net.params = _params(52, 0.3)
+    n = shape[0] * shape[1]
+    start = _hot(n, 53)
+    c = gpu.ising_gpu_chain(list(shape), j, h, beta, 77, start, kernel=kernel)
+    d = O.local_desc(list(shape), kernel=kernel, beta=beta, j=j, h=h, seed=77)
+    seed32 = O.fold_seed(77)
+    st, sweep, total = start.copy(), 0, 0
+    for t in range(12):
+        c.advance_with_blocks(net, 1, plain)
+        lg = c.block_logits()
+        st, a0 = O.block_mh_pass(shape, st, lg, 0, seed32, t, beta, j, h)
+        st, a1 = O.block_mh_pass(shape, st, lg, 1, seed32, t, beta, j, h)
+        st = O.local_sweeps(d, st, sweep, plain)
+        sweep += plain
+        total += a0 + a1
+        got = c.read()
+        assert np.array_equal(got, st), f"step {t}: {(got != st).sum()} sites differ"
+    acc, prop, done = c.block_stats()
+    assert done == 12 and acc == total and c.sweeps_done == sweep
+    c.close()
+    net.close()
+
+
+@pytest.mark.parametrize("shape,beta,j,h,plain,kernel", MIXED_CASES)
+def test_mixed_chain_of_many_steps_in_one_call_is_bit_exact(gpu, shape, beta, j, h, plain, kernel):
+    # Many steps per call, with recording on.  A bias-only network makes
+    # every logit exactly b3 on both sides, so the oracle needs no device
+    # logits; the block sweep keys (block sweep counter) and the plain sweep
+    # keys (chain sweep counter) must interleave as on the device.
+    b3 = np.float32(0.375 if h >= 0 else -0.25)
+    params = np.zeros(O.BN_PARAMS, np.float32)
+    params[O.BN_PARAMS - 1] = b3
+    net = gpu.BlockNet(0)
+    net.params = params
+    n = shape[0] * shape[1]
+    start = _hot(n, 61)
+    small_beta = 0.12
+    c = gpu.ising_gpu_chain(list(shape), j, h, small_beta, 78, start, kernel=kernel)
+    c.record(plain, 1024)
+    d = O.local_desc(list(shape), kernel=kernel, beta=small_beta, j=j, h=h, seed=78)
+    seed32 = O.fold_seed(78)
+    lg = np.full((n // 64, 64), b3, np.float32)
+    st, sweep, total, recs = start.copy(), 0, 0, []
+    for steps in (5, 7):
+        c.advance_with_blocks(net, steps, plain)
+        for _ in range(steps):
+            t = len(recs)
+            st, a0 = O.block_mh_pass(shape, st, lg, 0, seed32, t, small_beta, j, h)
+            st, a1 = O.block_mh_pass(shape, st, lg, 1, seed32, t, small_beta, j, h)
+            st = O.local_sweeps(d, st, sweep, plain)
+            sweep += plain
+            total += a0 + a1
+            recs.append(O.ising_sums(list(shape), st))
+        got = c.read()
+        assert np.array_equal(got, st), f"after {len(recs)} steps: {(got != st).sum()} sites differ"
+    acc, prop, done = c.block_stats()
+    assert done == 12 and acc == total and total > 0
+    series = c.series()
+    assert [(o.bond_sum, o.spin_sum) for o in series] == recs
+    c.close()
+    net.close()
+
+
+def test_block_sweep_leaves_the_boltzmann_law_invariant_on_the_device(gpu):
+    # As the oracle's check at J != 0: a block sweep applied to equilibrium
+    # configurations of the plain chain must not move <E>.  The network is
+    # trained briefly on the chain, so that blocks are accepted often enough
+    # for the paired shift to be sharp.  Tolerance: 4.5 standard errors of
+    # the mean paired shift.
+    shape, beta, n = (32, 32), BETA_C, 1024
+    net = gpu.BlockNet(71)
+    c = _chain(gpu, shape, beta, 72, _hot(n, 73))
+    c.advance(500)
+    net.train(c, 300, 2, 3e-3)
+    shifts = []
+    for _ in range(3000):
+        c.advance(4)
+        e0 = c.measure().energy
+        c.block_sweeps(net, 1)
+        shifts.append((c.measure().energy - e0) / n)
+    acc, prop, _ = c.block_stats()
+    shifts = np.array(shifts)
+    err = shifts.std() / math.sqrt(shifts.size)
+    print("acceptance", acc / prop, "shift", shifts.mean(), "+-", err)
+    assert acc / prop > 0.01
+    assert abs(shifts.mean()) < 4.5 * err
+    c.close()
+    net.close()
+
+
+def test_mixed_chain_matches_kaufman(gpu):
+    # The mixed chain (a block sweep, then one plain sweep) at 32^2 and
+    # beta_c against the exact finite-torus E/N and specific heat; 4.5
+    # standard errors on the tau_int-inflated and jackknifed errors.
+    import exact_ising2d as X
+    from plaquette_amd import statistics as S
+
+    L, beta = 32, BETA_C
+    n = L * L
+    net = gpu.BlockNet(81)
+    c = _chain(gpu, (L, L), beta, 82, _hot(n, 83))
+    c.advance(1000)
+    net.train(c, 300, 2, 3e-3)
+    c.advance_with_blocks(net, 2000, 1)
+    steps = 100000
+    c.record(1, steps)
+    c.advance_with_blocks(net, steps, 1)
+    e = np.array([o.energy for o in c.series()])
+    acc, prop, _ = c.block_stats()
+    c.close()
+    net.close()
+    assert e.size == steps and acc / prop > 0.01
+    est = S.reduce(e / n)
+    want = X.energy_per_site(L, L, beta)
+    assert abs(est.mean - want) < 4.5 * est.stderr, (est.mean, want, est.stderr)
+    cv = S.specific_heat(e, beta, n)
+    want_c = X.specific_heat(L, L, beta)
+    assert abs(cv.value - want_c) < 4.5 * cv.stderr, (cv.value, want_c, cv.stderr)

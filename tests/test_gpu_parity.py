@@ -472,8 +472,12 @@ def test_cluster_budget_exhaustion_is_reported(gpu):
     n = 64 * 64
     c = gpu.GpuClusterChain.swendsen_wang([64, 64], 3, 1.0, 8.0, 1, np.zeros(n, np.uint8))
     c.set_pass_cap(0)
+    c.advance(1)  # enqueue only: the overrun is reported where the chain is waited for
     with pytest.raises(gpu.PlaquetteError) as err:
-        c.advance(1)
+        c.sync()
+    assert err.value.code == gpu.PLQ_ENOCONV
+    with pytest.raises(gpu.PlaquetteError) as err:  # and stays: the chain holds no configuration
+        c.read()
     assert err.value.code == gpu.PLQ_ENOCONV
     c.close()
 
@@ -554,3 +558,96 @@ def test_cpu_stream_state_continues_in_a_new_chain(gpu):
     assert a.pcg_state() == b.pcg_state()
     a.close()
     b.close()
+
+
+# -- the configurations at their full sizes ------------------------------------------
+
+@pytest.mark.parametrize("start_kind", ["cold", "hot"])
+def test_potts_sw_at_4096_squared_is_bit_exact(gpu, start_kind):
+    # q = 3 Swendsen-Wang at beta_c on the full 4096^2 lattice (1,024 boxes
+    # of 128x128).  From a cold start the first sweeps have one percolating
+    # cluster whose root every box's boundary unions meet.  Labels, the last
+    # partition, the recorded observables and the recorded cluster
+    # statistics of every sweep against the oracle.
+    shape, q, beta = [4096, 4096], 3, 1.0050525
+    n = 4096 * 4096
+    start = start_config(n, q, start_kind == "hot", 5)
+    k = 3
+    c = gpu.GpuClusterChain.new(shape, q, 1.0, gpu.Extent.ALL, gpu.Relabel.REDRAW, beta, 2026, start)
+    c.record(1, k)
+    c.record_clusters(True)
+    c.advance(k)
+    got_series, got_stats = c.series(), c.cluster_series()
+    got, got_roots = c.read(), c.roots()
+    c.close()
+    d = O.cluster_desc(shape, q=q, beta=beta, gap=1.0, seed=2026)
+    st = start
+    largest = 0
+    for t in range(k):
+        st, roots = O.cluster_sweeps(d, st, t, 1)
+        agree, counts = O.potts_sums(shape, q, st)
+        assert got_series[t].bond_sum == agree and list(got_series[t].counts[:q]) == list(counts), t
+        sizes = np.bincount(roots)
+        sizes = sizes[sizes > 0]
+        largest = max(largest, int(sizes.max()))
+        s = got_stats[t]
+        assert s.n_clusters == sizes.size and s.largest == sizes.max(), t
+        hist = np.bincount(np.floor(np.log2(sizes)).astype(int), minlength=40)
+        assert list(s.size_hist) == list(hist), t
+    assert np.array_equal(got, st)
+    assert np.array_equal(got_roots, roots)
+    if start_kind == "cold":
+        assert largest > n // 2  # a percolating cluster was met
+
+
+@pytest.mark.parametrize("kernel", [M, HB])
+def test_ising_at_256_cubed_is_bit_exact(gpu, kernel):
+    # The headline configuration: 3D Ising 256^3, beta = 0.2216546, cold
+    # start, the default launch shape (sixteen sites per lane) against
+    # the oracle's sequential sweeps, with the fused E and M records.
+    shape = [256, 256, 256]
+    n = 256 ** 3
+    start = np.zeros(n, np.uint8)
+    c = gpu.ising_gpu_chain(shape, 1.0, 0.0, 0.2216546, 20260728, start, kernel=kernel)
+    c.record(1, 4)
+    c.advance(2)
+    series = c.series()
+    got = c.read()
+    c.close()
+    d = O.local_desc(shape, kernel=kernel, beta=0.2216546, seed=20260728)
+    st = start
+    for t in range(2):
+        st = O.local_sweeps(d, st, t, 1)
+        b, m = O.ising_sums(shape, st)
+        assert series[t].bond_sum == b and series[t].spin_sum == m, t
+    assert np.array_equal(got, st), f"{(got != st).sum()} sites differ"
+
+
+def test_cluster_statistics_across_a_reused_crossing_tag(gpu):
+    # The crossing marks carry a tag 1 + sweep mod 255, so sweeps t and
+    # t + 255 share a tag and a stale mark from 255 sweeps earlier is seen
+    # again.  Statistics of every one of 260 recorded sweeps at 256^2 (four
+    # boxes) against the oracle's partitions.
+    shape, q, beta = [256, 256], 3, 1.0050525
+    n = 256 * 256
+    start = start_config(n, q, True, 29)
+    k = 260
+    c = gpu.GpuClusterChain.new(shape, q, 1.0, gpu.Extent.ALL, gpu.Relabel.REDRAW, beta, 47, start)
+    c.record(1, k)
+    c.record_clusters(True)
+    c.advance(k)
+    got = c.cluster_series()
+    final = c.read()
+    c.close()
+    assert len(got) == k
+    d = O.cluster_desc(shape, q=q, beta=beta, gap=1.0, seed=47)
+    st = start
+    for t in range(k):
+        st, roots = O.cluster_sweeps(d, st, t, 1)
+        sizes = np.bincount(roots)
+        sizes = sizes[sizes > 0]
+        s = got[t]
+        assert s.n_clusters == sizes.size and s.largest == sizes.max(), t
+        hist = np.bincount(np.floor(np.log2(sizes)).astype(int), minlength=40)
+        assert list(s.size_hist) == list(hist), t
+    assert np.array_equal(final, st)
