@@ -116,3 +116,26 @@ def test_full_batched_workload_runs_and_agrees_across_launch_shapes(gpu):
     b = outs[0][1]
     for g in range(4):
         assert np.allclose(np.sort(b[12 * g:12 * g + 12]), np.sort(betas[12 * g:12 * g + 12]))
+
+
+@pytest.mark.parametrize("betas", [[0.25, 0.55, 0.85], [0.3, 0.9]])
+def test_device_exchange_samples_the_exact_laws(gpu, betas):
+    # Heat-bath replicas of a 2x4 Z2 plane (2^16 states, enumerated) with an
+    # exchange round after every sweep, on the device: every beta slot
+    # samples its exact plaquette law, swaps follow min(1, e^delta) of the
+    # recorded sums on the pairs of the round's parity, and each pair's mean
+    # acceptance equals the exact <min(1, e^delta)> (tests/exact_z2.py).
+    import exact_z2 as X
+
+    R, T = len(betas), 40000
+    e = gpu.Ensemble(2, [[2, 4]] * R, betas, [31 + r for r in range(R)], kernel=O.KERNEL_HEAT_BATH,
+                     measure_every=1, exchange=True, exchange_seed=17, record_capacity=T)
+    e.advance(T)
+    sums, slots = e.series()
+    acc, att = e.exchange_counts()
+    e.close()
+    assert sums.shape == (T, R)
+    X.check_exchange(sums, slots, betas, X.plaquette_dos(2, 4), O.reduce)
+    # pair k is attempted in the rounds t < T with t = k (mod 2)
+    assert list(att[:R - 1]) == [(T - k % 2 + 1) // 2 for k in range(R - 1)]
+    assert np.all(acc[:R - 1] <= att[:R - 1])
