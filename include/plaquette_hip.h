@@ -206,6 +206,18 @@ int plq_hot_start(uint64_t seed, int32_t q, uint8_t *out, size_t n, uint64_t *st
  * (may be NULL) sum / N (Ising) or sum / N - 1/q (Potts).  A local site chain
  * takes its own model; a cluster chain either (Ising needs q = 2). */
 int plq_chain_correlator(plq_chain *chain, int32_t model, int64_t *sums, double *values, size_t len);
+/* The same correlator without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * between HIP graph captures, tight serving loops, harnesses that forbid
+ * hipMalloc).  The bit planes and counters live in a fixed device array of
+ * the library (2^21 plane words + 8,192 counters, 8.06 MiB per device, held
+ * until the process ends): a configuration needing more than 2^21 words
+ * (ceil(log2 q) * N / L0 * ceil(L0 / 32)) or more than 8,192 entries gets
+ * PLQ_EINVAL; plq_chain_correlator has no such limit.  The counters are read
+ * back into the caller's `sums` (or, if NULL, `values`) storage, so at least
+ * one of them must be given.  Calls on one device share the array; the
+ * library serialises them. */
+int plq_chain_correlator_noalloc(plq_chain *chain, int32_t model, int64_t *sums, double *values, size_t len);
 /* Z2 gauge chains: Z2Gauge::polyakov_loop (gauge/mod.rs:178-201) along dir. */
 int plq_chain_polyakov_loop(plq_chain *chain, int32_t dir, double *value);
 /* Z2 gauge chains: Z2Gauge::wilson_rectangles (gauge/mod.rs:224-268).

@@ -186,6 +186,7 @@ SIGNATURES = {
     "plq_chain_pcg_state": (C.c_int, [_HANDLE, _P(C.c_uint64)]),
     "plq_chain_set_pcg_state": (C.c_int, [_HANDLE, _P(C.c_uint64)]),
     "plq_chain_correlator": (C.c_int, [_HANDLE, C.c_int32, _P(C.c_int64), _P(C.c_double), C.c_size_t]),
+    "plq_chain_correlator_noalloc": (C.c_int, [_HANDLE, C.c_int32, _P(C.c_int64), _P(C.c_double), C.c_size_t]),
     "plq_chain_polyakov_loop": (C.c_int, [_HANDLE, C.c_int32, _P(C.c_double)]),
     "plq_chain_wilson_rectangles": (C.c_int, [_HANDLE, C.c_int32, _P(C.c_double), C.c_size_t, _P(C.c_int32)]),
     "plq_ensemble_create": (C.c_int, [_P(EnsembleDesc), _P(C.c_uint8), C.c_size_t, _P(_HANDLE)]),

@@ -292,6 +292,8 @@ int meas_state_planes(int q);
 // bits_per_var 1..4, or 2..4 with bits_per_var 1.
 void launch_meas_pack(const uint8_t *lat, const MeasGeom &m, int vars_per_site, int bits_per_var, uint32_t *bits,
                       hipStream_t s);
+// p[0..n) <- 0 by a fill kernel.
+void launch_meas_clear(unsigned long long *p, int64_t n, hipStream_t s);
 // agree[k] += sites where s(x) == s(x + r e_mu), pair k numbered axis after
 // axis over r = 0..L_mu/2; agree must start at zero.
 void launch_meas_pairs(const uint32_t *bits, const MeasGeom &m, int planes, int npairs, unsigned long long *agree,

@@ -260,6 +260,13 @@ __global__ __launch_bounds__(MEAS_BLOCK) void meas_wilson_kernel(const uint32_t 
     add_count(cnt, odd + k);
 }
 
+// Zero n counters (a fill kernel rather than hipMemsetAsync, so the clear is
+// an ordinary kernel node should a caller capture the stream).
+__global__ __launch_bounds__(MEAS_BLOCK) void meas_clear_kernel(unsigned long long *__restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * MEAS_BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * MEAS_BLOCK)
+        p[i] = 0ull;
+}
+
 constexpr int MAX_GRID_Y = 65535;
 
 // Workgroups along x for `ny` independent sums over `total` words: enough to
@@ -309,6 +316,12 @@ void launch_meas_pack(const uint8_t *lat, const MeasGeom &m, int vars_per_site, 
         case 25: meas_pack_kernel<3, 1><<<grid, block, 0, s>>>(lat, m, bits); break;
         default: meas_pack_kernel<4, 1><<<grid, block, 0, s>>>(lat, m, bits); break;
     }
+}
+
+void launch_meas_clear(unsigned long long *p, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    const int64_t need = (n + MEAS_BLOCK - 1) / MEAS_BLOCK;
+    meas_clear_kernel<<<dim3((uint32_t)(need < 64 ? need : 64)), dim3(MEAS_BLOCK), 0, s>>>(p, n);
 }
 
 void launch_meas_pairs(const uint32_t *bits, const MeasGeom &m, int planes, int npairs, unsigned long long *agree,

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstring>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -13,6 +14,13 @@
 #include "../../include/plaquette_hip.h"
 #include "plq_chain.hpp"
 #include "plq_internal.hpp"
+
+// Scratch of plq_chain_correlator_noalloc: bit planes, then counters.  A
+// file-scope device array, so that entry point never allocates; every device
+// the library runs on holds its own copy (8.06 MiB) until the process ends.
+constexpr int64_t NOALLOC_PLANE_WORDS = int64_t(1) << 21;  // 8 MiB of bit-plane words
+constexpr int64_t NOALLOC_COUNTS = 8192;                    // 64 KiB of u64 counters
+static __device__ __align__(256) uint32_t scratch[NOALLOC_PLANE_WORDS + 2 * NOALLOC_COUNTS];
 
 namespace {
 
@@ -37,6 +45,33 @@ int launch_status() {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(PLQ_EDEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
     return PLQ_OK;
+}
+
+// The device address of `scratch` on `device` (current): in host code the
+// array's name is a host-side stand-in, so kernels and copies take this
+// address, found once per device with hipGetSymbolAddress.
+constexpr int NOALLOC_MAX_DEVICES = 64;
+
+int scratch_address(int device, uint32_t **out) {
+    static std::mutex mu;
+    static uint32_t *addr[NOALLOC_MAX_DEVICES] = {};
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return fail(PLQ_ENODEV, "device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(PLQ_EDEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        addr[device] = static_cast<uint32_t *>(p);
+    }
+    *out = addr[device];
+    return PLQ_OK;
+}
+
+// Calls on one device share its scratch: they are serialised by this lock.
+std::mutex &scratch_mutex(int device) {
+    static std::mutex mu[NOALLOC_MAX_DEVICES];
+    return mu[device];
 }
 
 // Grow the chain's measurement buffers to `words` plane words and `counts`
@@ -109,21 +144,45 @@ int read_counts(plq_chain *c, std::vector<unsigned long long> &out) {
     return PLQ_OK;
 }
 
-}  // namespace
-
-int plq_chain_correlator(plq_chain *c, int32_t model, int64_t *sums, double *values, size_t len) {
-    if (!c) return fail(PLQ_EINVAL, "chain is null");
-    std::lock_guard<std::mutex> lk(c->mu);
+// Validation shared by both correlator entry points; *npairs receives the
+// number of (axis, displacement) entries.
+int correlator_args(const plq_chain *c, int32_t model, size_t len, int64_t *npairs) {
     if (c->kind == KIND_LINK) return fail(PLQ_EINVAL, "the correlator is measured on site chains (Ising, Potts)");
     if (model != PLQ_MODEL_ISING && model != PLQ_MODEL_POTTS)
         return fail(PLQ_EINVAL, "the correlator convention is PLQ_MODEL_ISING or PLQ_MODEL_POTTS");
     if (c->kind == KIND_SITE && model != c->model)
         return fail(PLQ_EINVAL, "the correlator convention must be the chain's model");
     if (model == PLQ_MODEL_ISING && c->q != 2) return fail(PLQ_EINVAL, "the Ising correlator needs q = 2");
-    const int64_t npairs = pair_count(c);
-    if (len != (size_t)npairs)
+    *npairs = pair_count(c);
+    if (len != (size_t)*npairs)
         return fail(PLQ_EINVAL, "the output holds one entry per axis and displacement r = 0..L/2 (sum of L/2 + 1)");
-    if (npairs > (int64_t(1) << 30)) return fail(PLQ_EINVAL, "too many displacements");
+    if (*npairs > (int64_t(1) << 30)) return fail(PLQ_EINVAL, "too many displacements");
+    return PLQ_OK;
+}
+
+// models/mod.rs:44-60 returns the raw sums; ising/mod.rs:89-93 and
+// potts/mod.rs:230-232 scale them.  agree may alias sums or values: entry k
+// is read before entry k of either output is written.
+void correlator_values(const plq_chain *c, int32_t model, int64_t npairs, const unsigned long long *agree,
+                       int64_t *sums, double *values) {
+    const double n = (double)c->n_sites;
+    const double floor = 1.0 / (double)c->q;
+    for (int64_t k = 0; k < npairs; ++k) {
+        unsigned long long a;
+        std::memcpy(&a, agree + k, sizeof a);  // agree may be the caller's int64 or double storage
+        const int64_t s = model == PLQ_MODEL_ISING ? 2 * (int64_t)a - c->n_sites : (int64_t)a;
+        if (sums) sums[k] = s;
+        if (values) values[k] = model == PLQ_MODEL_ISING ? (double)s / n : (double)s / n - floor;
+    }
+}
+
+}  // namespace
+
+int plq_chain_correlator(plq_chain *c, int32_t model, int64_t *sums, double *values, size_t len) {
+    if (!c) return fail(PLQ_EINVAL, "chain is null");
+    std::lock_guard<std::mutex> lk(c->mu);
+    int64_t npairs = 0;
+    PLQ_TRY(correlator_args(c, model, len, &npairs));
     PLQ_HIP(hipSetDevice(c->device));
     PLQ_TRY(check_cluster(c));
     const plq::MeasGeom m = plq::meas_geom(c->D, c->shape);
@@ -134,15 +193,47 @@ int plq_chain_correlator(plq_chain *c, int32_t model, int64_t *sums, double *val
     PLQ_TRY(launch_status());
     std::vector<unsigned long long> agree((size_t)npairs);
     PLQ_TRY(read_counts(c, agree));
-    // models/mod.rs:44-60 returns the raw sums; ising/mod.rs:89-93 and
-    // potts/mod.rs:230-232 scale them.
-    const double n = (double)c->n_sites;
-    const double floor = 1.0 / (double)c->q;
-    for (int64_t k = 0; k < npairs; ++k) {
-        const int64_t s = model == PLQ_MODEL_ISING ? 2 * (int64_t)agree[k] - c->n_sites : (int64_t)agree[k];
-        if (sums) sums[k] = s;
-        if (values) values[k] = model == PLQ_MODEL_ISING ? (double)s / n : (double)s / n - floor;
+    correlator_values(c, model, npairs, agree.data(), sums, values);
+    return PLQ_OK;
+}
+
+int plq_chain_correlator_noalloc(plq_chain *c, int32_t model, int64_t *sums, double *values, size_t len) {
+    if (!c) return fail(PLQ_EINVAL, "chain is null");
+    if (!sums && !values) return fail(PLQ_EINVAL, "sums and values are both null: nothing to return");
+    std::lock_guard<std::mutex> lk(c->mu);
+    int64_t npairs = 0;
+    PLQ_TRY(correlator_args(c, model, len, &npairs));
+    const plq::MeasGeom m = plq::meas_geom(c->D, c->shape);
+    const int planes = plq::meas_state_planes(c->q);
+    const int64_t words = (int64_t)planes * m.R * m.W;
+    if (words > NOALLOC_PLANE_WORDS || npairs > NOALLOC_COUNTS) {
+        char buf[256];
+        snprintf(buf, sizeof buf,
+                 "the configuration needs %lld bit-plane words and %lld counters; the no-allocation "
+                 "scratch holds %lld and %lld (use plq_chain_correlator)",
+                 (long long)words, (long long)npairs, (long long)NOALLOC_PLANE_WORDS, (long long)NOALLOC_COUNTS);
+        return fail(PLQ_EINVAL, buf);
     }
+    PLQ_HIP(hipSetDevice(c->device));
+    PLQ_TRY(check_cluster(c));
+    uint32_t *base = nullptr;
+    PLQ_TRY(scratch_address(c->device, &base));
+    // One device array serves every chain on the device: hold it from the
+    // first launch to the end of the readback.
+    std::lock_guard<std::mutex> sk(scratch_mutex(c->device));
+    uint32_t *bits = base;
+    unsigned long long *counts = reinterpret_cast<unsigned long long *>(base + NOALLOC_PLANE_WORDS);
+    plq::launch_meas_clear(counts, npairs, c->stream);
+    plq::launch_meas_pack(lattice_bytes(c), m, 1, planes, bits, c->stream);
+    plq::launch_meas_pairs(bits, m, planes, (int)npairs, counts, c->stream);
+    PLQ_TRY(launch_status());
+    // The counters come back into the caller's own 8-byte entries and are
+    // turned into the results in place.
+    void *host = sums ? (void *)sums : (void *)values;
+    PLQ_HIP(hipMemcpyAsync(host, counts, (size_t)npairs * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           c->stream));
+    PLQ_HIP(hipStreamSynchronize(c->stream));
+    correlator_values(c, model, npairs, reinterpret_cast<const unsigned long long *>(host), sums, values);
     return PLQ_OK;
 }
 
