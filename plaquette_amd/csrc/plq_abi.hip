@@ -126,6 +126,17 @@ int upload_table(plq_chain *c) {
     std::vector<uint32_t> table;
     std::string err;
     if (!plq::build_table(tp, table, err)) return fail(PLQ_EINVAL, err);
+    // The uniform-parity 16-site pass (power-of-two Ising shapes, every
+    // threshold >= 1); PLQ_X16U=0 keeps the chain on ising_pass_x16, which
+    // tests use to check that both give identical bits.
+    c->x16u = 0;
+    if (c->kind == KIND_SITE && c->model == PLQ_MODEL_ISING && c->vec == 16 && !c->pmode &&
+        plq::x16u_applies(c->g, table.data(), table.size(), c->pass_blocks)) {
+        c->x16u = 1;
+        c->ug = plq::x16u_geom(c->g);
+        if (const char *e = getenv("PLQ_X16U"))
+            if (atoi(e) == 0) c->x16u = 0;
+    }
     if (c->d_table) (void)hipFree(c->d_table);
     c->d_table = nullptr;
     PLQ_TRY(alloc((void **)&c->d_table, table.size() * sizeof(uint32_t)));
@@ -177,6 +188,8 @@ plq::SitePassArgs site_args(plq_chain *c, int color, uint32_t sweep, int mode, i
     a.own = c->col[color];
     a.g = c->g;
     a.sg = c->sg;
+    a.ug = c->ug;
+    a.uniform = c->x16u;
     a.vec = c->vec;
     a.model = c->model == PLQ_MODEL_POTTS ? 1 : 0;
     a.mode = mode;

@@ -34,6 +34,8 @@ struct plq_chain {
     plq::Geom g{};
     plq::SwarGeom sg{};
     int vec = 1;                           // sites per lane of the site kernel
+    int x16u = 0;                          // the uniform-parity 16-site pass serves this chain
+    plq::X16UGeom ug{};
     bool small = false;                    // whole sweeps in one workgroup (plq_small.hip, plq_cluster_small.hip)
     int64_t pass_blocks = plq::MAX_PASS_BLOCKS;  // grid cap of a colour pass
     uint8_t *col[2] = {nullptr, nullptr};  // site colours

@@ -35,11 +35,26 @@ struct MeasArgs {
     int on;                // this pass measures
 };
 
+// Geometry of the sixteen-sites-per-lane pass whose waves hold rows of one
+// parity (ising_pass_x16u): lanes are mapped so that the 64 lanes of a wave
+// cover 2^(6 - lh16) rows two apart, so every row of a wave has the same
+// colour offset along axis 0.  Byte offsets in a colour array are
+// (row << sh) | (chunk << 4).
+struct X16UGeom {
+    uint32_t lh16;       // log2(16-byte chunks per colour row), 0..6
+    uint32_t sh;         // lh16 + 4: log2(bytes per colour row)
+    uint32_t lk2;        // 7 - lh16: log2 of the rows a pair of waves covers
+    uint32_t ll1, ll2;   // log2 L1, log2 L2
+    uint32_t m1, m2, m3; // L1 - 1, L2 - 1, L3 - 1
+};
+
 struct SitePassArgs {
     const uint8_t *oth;  // the other colour's array (read)
     uint8_t *own;        // this colour's array (updated)
     Geom g;
     SwarGeom sg;
+    X16UGeom ug;
+    int uniform;         // 1: ising_pass_x16u applies (see x16u_applies)
     int vec;             // sites per lane: 1 (generic), 4 or 16 (power-of-two Ising)
     int model;           // 0 Ising, 1 Potts
     int mode;            // 0 Metropolis, 1 heat bath, 2 measure only
@@ -56,6 +71,10 @@ struct SitePassArgs {
 void launch_pack_sites(const uint8_t *lat, uint8_t *c0, uint8_t *c1, const Geom &g, hipStream_t s);
 void launch_unpack_sites(uint8_t *lat, const uint8_t *c0, const uint8_t *c1, const Geom &g, hipStream_t s);
 void launch_site_pass(const SitePassArgs &a);
+// Whether the uniform-parity sixteen-site pass serves this geometry and
+// threshold table (every threshold at least 1; shapes as X16UGeom needs).
+bool x16u_applies(const Geom &g, const uint32_t *host_table, size_t table_len, int64_t max_blocks);
+X16UGeom x16u_geom(const Geom &g);
 int site_pass_blocks(const SitePassArgs &a);
 void launch_finalize_record(const long long *pa, int64_t na, const long long *pb, int64_t nb, long long *rec,
                             hipStream_t s);

@@ -569,6 +569,170 @@ __global__ __launch_bounds__(256) void ising_pass_x16(const uint4 *__restrict__ 
     }
 }
 
+// Byte-offset access: a uniform base plus a 32-bit byte offset.
+template <class T>
+__device__ __forceinline__ T ldb(const T *base, uint32_t off) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+template <class T>
+__device__ __forceinline__ void stb(T *base, uint32_t off, T v) {
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off) = v;
+}
+
+// (x & m) | (y & ~m): one v_bfi_b32.
+__device__ __forceinline__ uint32_t bfi(uint32_t m, uint32_t x, uint32_t y) { return (x & m) | (y & ~m); }
+
+// One 16-site chunk of ising_pass_x16u whose rows have axis-0 colour offset
+// P (wave-uniform).  A is the chunk's byte offset in a colour array.
+template <int D, int MODE, int P>
+__device__ __forceinline__ void x16u_chunk(const uint4 *__restrict__ oth, uint4 *__restrict__ own, const X16UGeom &ug,
+                                           uint32_t A, uint32_t seed, uint32_t kmix, const char *Tb, int c, int on,
+                                           int &bond, int &down, int &total) {
+    constexpr int NN = 2 * D;
+    constexpr int NR = D > 1 ? 2 * (D - 1) : 1;
+    // Neighbour rows: the axis field of the byte offset replaced by its
+    // periodic successor or predecessor (one add and one bit-field insert).
+    const uint32_t S1 = 1u << ug.sh, M1 = ug.m1 << ug.sh;
+    const uint32_t cm = (1u << (ug.lh16 + 4)) - 16u;  // chunk field of the offset
+    uint32_t rowoff[NR];
+    rowoff[0] = bfi(M1, A + S1, A);
+    if (D > 1) rowoff[1] = bfi(M1, A - S1, A);
+    if (D > 2) {
+        const uint32_t S2 = 1u << (ug.sh + ug.ll1), M2 = ug.m2 << (ug.sh + ug.ll1);
+        rowoff[2] = bfi(M2, A + S2, A);
+        rowoff[3] = bfi(M2, A - S2, A);
+    }
+    if (D > 3) {
+        const uint32_t S3 = 1u << (ug.sh + ug.ll1 + ug.ll2), M3 = ug.m3 << (ug.sh + ug.ll1 + ug.ll2);
+        rowoff[4] = bfi(M3, A + S3, A);
+        rowoff[5] = bfi(M3, A - S3, A);
+    }
+    // The byte the axis-0 shift pulls in: the first byte of the next chunk
+    // (P = 1) or the last byte of the previous one (P = 0), along the row.
+    const uint32_t adjoff = P ? bfi(cm, A + 16u, A) : bfi(cm, A - 16u, A) + 12u;
+    const uint4 a = ldb(oth, A);
+    const uint32_t adj = ldb(reinterpret_cast<const uint32_t *>(oth), adjoff);
+    uint4 rows[NR];
+#pragma unroll
+    for (int k = 0; k < (D > 1 ? NR : 0); ++k) rows[k] = ldb(oth, rowoff[k]);
+    const uint4 s = ldb(own, A);
+    // The sixteen draws (site0 = 2A + P has bits 1..4 clear) while the loads
+    // are in flight; y = the final hash before its 8-bit shift, so that
+    // m < T  <=>  y <= T * 256 - 1.
+    uint32_t y[16];
+    __builtin_amdgcn_sched_barrier(0);
+    {
+        const uint32_t bx = seed ^ (2u * A + (uint32_t)P);
+        const uint32_t v0 = bx ^ (bx >> 16);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) y[k] = lowbias32_tail(lowbias32_mid(v0 ^ (2u * (uint32_t)k)) ^ kmix);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    // Down-neighbour counts, four sites per dword.
+    uint32_t ds[4];
+    const uint32_t av[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t sh0 = P ? __builtin_amdgcn_alignbyte(j < 3 ? av[j + 1] : adj, av[j], 1)
+                               : __builtin_amdgcn_alignbyte(av[j], j > 0 ? av[j - 1] : adj, 3);
+        ds[j] = av[j] + sh0;
+    }
+#pragma unroll
+    for (int k = 0; k < (D > 1 ? NR : 0); ++k) {
+        ds[0] += rows[k].x;
+        ds[1] += rows[k].y;
+        ds[2] += rows[k].z;
+        ds[3] += rows[k].w;
+    }
+    const uint32_t sv[4] = {s.x, s.y, s.z, s.w};
+    // Rejections as bits, site 15 first, so that site i lands in bit i: a
+    // compare into VCC and an add-with-carry per site.
+    uint32_t bits = 0;
+#pragma unroll
+    for (int j = 3; j >= 0; --j) {
+        const uint32_t addr4 = (ds[j] << 3) | (sv[j] << 2);
+#pragma unroll
+        for (int k = 3; k >= 0; --k) {
+            const uint32_t t2 = *reinterpret_cast<const uint32_t *>(Tb + ((addr4 >> (8 * k)) & 0xffu));
+            asm("v_cmp_gt_u32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc"
+                : "+v"(bits)
+                : "v"(y[4 * j + k]), "v"(t2)
+                : "vcc");
+        }
+    }
+    // Spread each dword's four bits to its four bytes.
+    uint32_t ov[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t raw = ((bits >> (4 * j)) & 0xfu) * 0x00204081u;
+        ov[j] = MODE == 0 ? (sv[j] ^ (~raw & 0x01010101u)) : (raw & 0x01010101u);
+    }
+    stb(own, A, make_uint4(ov[0], ov[1], ov[2], ov[3]));
+    if (on) {
+        down += __popc(ov[0]) + __popc(ov[1]) + __popc(ov[2]) + __popc(ov[3]);
+        total += 16;
+        if (c == 1) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    int sn = (int)((ov[j] >> (8 * k)) & 1u);
+                    int dn = (int)((ds[j] >> (8 * k)) & 0xffu);
+                    bond += (1 - 2 * sn) * (NN - 2 * dn);
+                }
+        }
+    }
+}
+
+// Sixteen same-colour sites per lane, as ising_pass_x16, with the lanes of a
+// wave on rows of one parity.  Logical chunk u (64 per wave) goes to row
+// (wave >> 1) * 2K + 2 * (lane >> lh16) + (wave & 1), chunk lane & (2^lh16 - 1),
+// K = 64 >> lh16 rows per wave: a pair of waves covers 2K consecutive rows,
+// which lie on one axis-1 line (2K <= L1), so the axis-0 colour offset
+// p = (x1 + x2 + x3 + c) & 1 is the same for the whole wave and the axis-0
+// neighbour bytes are selected by a scalar branch.  Neighbour rows are
+// found by bit-field inserts on the byte offset.  Every threshold T >= 1 (the
+// host checks), so m < T is y <= T * 256 - 1 with y the hash before its final
+// 8-bit shift, decided by one compare and one add-with-carry per site.
+// Requires span to be a multiple of 64 (whole waves per chunk range).
+template <int D, int MODE>
+__global__ __launch_bounds__(256) void ising_pass_x16u(const uint4 *__restrict__ oth, uint4 *__restrict__ own,
+                                                       X16UGeom ug, int c, uint32_t seed, uint32_t skey,
+                                                       const uint32_t *__restrict__ table, MeasArgs m, uint32_t work,
+                                                       uint32_t span) {
+    constexpr int NN = 2 * D;
+    constexpr int K = NN + 1;
+    __shared__ uint32_t T[2 * 16];  // T[d * 2 + s] = threshold * 256 - 1, byte address (d << 3) | (s << 2)
+    __shared__ long long lds[16];
+    if (threadIdx.x < 2 * K) {
+        int s = threadIdx.x / K, d = threadIdx.x % K;
+        T[d * 2 + s] = (table[threadIdx.x] << 8) - 1u;
+    }
+    __syncthreads();
+    const uint32_t begin = (uint32_t)logical_block() * span;
+    uint32_t end = begin + span;
+    if (end > work) end = work;
+    const uint32_t kmix = sweep_mix(skey);
+    const char *Tb = reinterpret_cast<const char *>(T);
+    const uint32_t cmask = (1u << ug.lh16) - 1u;
+    int bond = 0, down = 0, total = 0;
+    for (uint32_t u = begin + threadIdx.x; u < end; u += blockDim.x) {
+        const uint32_t wv = u >> 6, l = u & 63u;
+        const uint32_t row = ((wv >> 1) << ug.lk2) + ((l >> ug.lh16) << 1) + (wv & 1u);
+        const uint32_t A = (row << ug.sh) | ((l & cmask) << 4);
+        const uint32_t x1 = row & ug.m1;
+        const uint32_t x2 = D > 2 ? (row >> ug.ll1) & ug.m2 : 0u;
+        const uint32_t x3 = D > 3 ? row >> (ug.ll1 + ug.ll2) : 0u;
+        const uint32_t p = __builtin_amdgcn_readfirstlane((x1 + x2 + x3 + (uint32_t)c) & 1u);
+        if (p) x16u_chunk<D, MODE, 1>(oth, own, ug, A, seed, kmix, Tb, c, m.on, bond, down, total);
+        else x16u_chunk<D, MODE, 0>(oth, own, ug, A, seed, kmix, Tb, c, m.on, bond, down, total);
+    }
+    if (m.on) {
+        int counts[2] = {total - down, down};
+        emit_partials(m, (long long)bond, counts, 2, lds);
+    }
+}
+
 // --------------------------------------------------------------- launchers ---
 
 static void grid_for_work(int64_t work, int64_t max_blocks, int *blocks, int64_t *span) {
@@ -578,6 +742,43 @@ static void grid_for_work(int64_t work, int64_t max_blocks, int *blocks, int64_t
     if (G >= 8) G = (G / 8) * 8;
     *blocks = (int)G;
     *span = (work + G - 1) / G;
+}
+
+static uint32_t log2u(int64_t v) {
+    uint32_t k = 0;
+    while ((int64_t(1) << k) < v) ++k;
+    return k;
+}
+
+X16UGeom x16u_geom(const Geom &g) {
+    X16UGeom u{};
+    u.lh16 = log2u(g.L[0] / 32);
+    u.sh = u.lh16 + 4;
+    u.lk2 = 7 - u.lh16;
+    u.ll1 = log2u(g.L[1]);
+    u.ll2 = log2u(g.L[2]);
+    u.m1 = (uint32_t)g.L[1] - 1u;
+    u.m2 = (uint32_t)g.L[2] - 1u;
+    u.m3 = (uint32_t)g.L[3] - 1u;
+    return u;
+}
+
+bool x16u_applies(const Geom &g, const uint32_t *host_table, size_t table_len, int64_t max_blocks) {
+    if (g.D < 2 || g.L[0] < 32 || g.L[0] > 2048) return false;
+    for (int mu = 0; mu < g.D; ++mu)
+        if (g.L[mu] & (g.L[mu] - 1)) return false;
+    const int64_t chunks_per_row = g.L[0] / 32;
+    const int64_t rows_per_pair = 2 * (64 / chunks_per_row);  // 2K
+    if (g.L[1] < rows_per_pair) return false;
+    // whole waves per block span: work and the grid are powers of two
+    const int64_t work = (int64_t)g.R * chunks_per_row;
+    if (work < 256 || work > (int64_t(1) << 31)) return false;
+    int64_t G = work / 256 < max_blocks ? work / 256 : max_blocks;
+    if (G & (G - 1)) return false;
+    if (g.N > (int64_t(1) << 32)) return false;
+    for (size_t k = 0; k < table_len; ++k)
+        if (host_table[k] == 0 || host_table[k] > DRAW_ONE) return false;
+    return true;
 }
 
 void launch_pack_sites(const uint8_t *lat, uint8_t *c0, uint8_t *c1, const Geom &g, hipStream_t s) {
@@ -644,6 +845,19 @@ static void dispatch_x4(const SitePassArgs &a, int blocks, int64_t work, int64_t
 }
 
 template <int D>
+static void dispatch_x16u(const SitePassArgs &a, int blocks, int64_t work, int64_t span) {
+    dim3 grid((uint32_t)blocks), block(256);
+    const uint4 *oth = (const uint4 *)a.oth;
+    uint4 *own = (uint4 *)a.own;
+    if (a.mode == 0)
+        ising_pass_x16u<D, 0><<<grid, block, 0, a.stream>>>(oth, own, a.ug, a.color, a.seed, a.skey, a.table, a.m,
+                                                            (uint32_t)work, (uint32_t)span);
+    else
+        ising_pass_x16u<D, 1><<<grid, block, 0, a.stream>>>(oth, own, a.ug, a.color, a.seed, a.skey, a.table, a.m,
+                                                            (uint32_t)work, (uint32_t)span);
+}
+
+template <int D>
 static void dispatch_x16(const SitePassArgs &a, int blocks, int64_t work, int64_t span) {
     dim3 grid((uint32_t)blocks), block(256);
     const uint4 *oth = (const uint4 *)a.oth;
@@ -669,7 +883,13 @@ void launch_site_pass(const SitePassArgs &a) {
         case 3: FN<3>(a, blocks, work, span); break;   \
         default: FN<4>(a, blocks, work, span); break;  \
     }
-    if (a.vec == 16) {
+    if (a.vec == 16 && a.uniform && a.mode != 2 && span % 64 == 0 && a.g.D >= 2) {
+        switch (a.g.D) {
+            case 2: dispatch_x16u<2>(a, blocks, work, span); break;
+            case 3: dispatch_x16u<3>(a, blocks, work, span); break;
+            default: dispatch_x16u<4>(a, blocks, work, span); break;
+        }
+    } else if (a.vec == 16) {
         PLQ_BY_D(dispatch_x16)
     } else if (a.vec == 4) {
         PLQ_BY_D(dispatch_x4)

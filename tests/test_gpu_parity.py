@@ -121,8 +121,11 @@ def test_launch_shape_does_not_change_the_bits_at_256_cubed(gpu, monkeypatch, ke
     n = int(np.prod(shape))
     start = start_config(n, 2, True, 5)
     outs = []
-    for vec in ("16", "4", "1"):
-        monkeypatch.setenv("PLQ_SITES_PER_LANE", vec)
+    for vec in ("16", "16-rowwise", "4", "1"):
+        # "16" is the uniform-parity pass at this shape; "16-rowwise" the
+        # 16-site pass whose waves mix rows of both parities
+        monkeypatch.setenv("PLQ_SITES_PER_LANE", vec[:2].rstrip("-"))
+        monkeypatch.setenv("PLQ_X16U", "0" if vec == "16-rowwise" else "1")
         c = gpu.ising_gpu_chain(shape, 1.0, 0.0, 0.2216546, 99, start, kernel=kernel)
         c.record(4, 8)
         c.advance(12)
@@ -152,6 +155,40 @@ def test_every_launch_shape_is_bit_exact_against_the_oracle(gpu, monkeypatch, ve
             st = O.local_sweeps(d, st, t, 1)
             assert (series[t].bond_sum, series[t].spin_sum) == O.ising_sums(shape, st)
         assert np.array_equal(got, st)
+
+
+# Shapes the uniform-parity 16-site pass serves (L1 >= 2 * 64 / (L0 / 32)),
+# and one whose f32 thresholds underflow to 0 at beta = 5 (that chain falls
+# back to the row-wise 16-site pass)
+X16U_CASES = [
+    # shape, kernel, profile, beta, h
+    ([256, 16, 8], M, 0, 0.2216546, 0.07),
+    ([256, 16, 8], HB, 1, 0.2216546, -0.05),
+    ([64, 64, 4], HB, 0, 0.3, 0.0),
+    ([128, 32, 4, 2], M, 1, 0.15, 0.1),
+    ([512, 32], HB, 1, 0.4406868, 0.0),
+    ([2048, 8], M, 0, 0.4406868, 0.2),
+    ([256, 16, 8], M, 1, 5.0, 0.0),
+]
+
+
+@pytest.mark.parametrize("case", X16U_CASES, ids=[f"{'x'.join(map(str, c[0]))}-{c[1]}-{c[2]}-b{c[3]}" for c in X16U_CASES])
+def test_uniform_parity_pass_is_bit_exact(gpu, monkeypatch, case):
+    shape, kernel, profile, beta, h = case
+    n = int(np.prod(shape))
+    for hot in (True, False):
+        start = start_config(n, 2, hot, 14)
+        c = gpu.ising_gpu_chain(shape, 1.0, h, beta, 8, start, kernel=kernel, profile=profile)
+        c.record(1, 25)
+        c.advance(25)
+        got, series = c.read(), c.series()
+        c.close()
+        d = O.local_desc(shape, kernel=kernel, profile=profile, beta=beta, h=h, seed=8)
+        st = start
+        for t in range(25):
+            st = O.local_sweeps(d, st, t, 1)
+            assert (series[t].bond_sum, series[t].spin_sum) == O.ising_sums(shape, st), t
+        assert np.array_equal(got, st), f"{(got != st).sum()} sites differ"
 
 
 SMALL_CASES = [([64, 64], M), ([64, 64], HB), ([16], M), ([2, 4], HB), ([16, 16, 16], M), ([8, 8, 4, 4], HB),
