@@ -582,30 +582,38 @@ __device__ __forceinline__ void stb(T *base, uint32_t off, T v) {
 // (x & m) | (y & ~m): one v_bfi_b32.
 __device__ __forceinline__ uint32_t bfi(uint32_t m, uint32_t x, uint32_t y) { return (x & m) | (y & ~m); }
 
+// Offsets of one wave's chunk rows that do not depend on the lane: the
+// wave's rows share x2 and x3, so their axis-2 and axis-3 neighbours are a
+// uniform delta from the row's own offset.
+struct X16UWave {
+    uint32_t base;      // byte offset of the wave's first row
+    uint32_t dz[2];     // axis-2 successor / predecessor deltas (mod 2^32)
+    uint32_t dw[2];     // axis-3 successor / predecessor deltas
+};
+
 // One 16-site chunk of ising_pass_x16u whose rows have axis-0 colour offset
 // P (wave-uniform).  A is the chunk's byte offset in a colour array.
 template <int D, int MODE, int P>
 __device__ __forceinline__ void x16u_chunk(const uint4 *__restrict__ oth, uint4 *__restrict__ own, const X16UGeom &ug,
-                                           uint32_t A, uint32_t seed, uint32_t kmix, const char *Tb, int c, int on,
-                                           int &bond, int &down, int &total) {
+                                           const X16UWave &wv, uint32_t A, uint32_t seed, uint32_t kmix,
+                                           const char *Tb, int c, int on, int &bond, int &down, int &total) {
     constexpr int NN = 2 * D;
-    constexpr int NR = D > 1 ? 2 * (D - 1) : 1;
-    // Neighbour rows: the axis field of the byte offset replaced by its
-    // periodic successor or predecessor (one add and one bit-field insert).
+    constexpr int NR = 2 * (D - 1);
+    // Axis-1 neighbours: the x1 field of the byte offset replaced by its
+    // periodic successor or predecessor (one add and one bit-field insert);
+    // axes 2 and 3: the wave's uniform deltas.
     const uint32_t S1 = 1u << ug.sh, M1 = ug.m1 << ug.sh;
     const uint32_t cm = (1u << (ug.lh16 + 4)) - 16u;  // chunk field of the offset
     uint32_t rowoff[NR];
     rowoff[0] = bfi(M1, A + S1, A);
-    if (D > 1) rowoff[1] = bfi(M1, A - S1, A);
+    rowoff[1] = bfi(M1, A - S1, A);
     if (D > 2) {
-        const uint32_t S2 = 1u << (ug.sh + ug.ll1), M2 = ug.m2 << (ug.sh + ug.ll1);
-        rowoff[2] = bfi(M2, A + S2, A);
-        rowoff[3] = bfi(M2, A - S2, A);
+        rowoff[2] = A + wv.dz[0];
+        rowoff[3] = A + wv.dz[1];
     }
     if (D > 3) {
-        const uint32_t S3 = 1u << (ug.sh + ug.ll1 + ug.ll2), M3 = ug.m3 << (ug.sh + ug.ll1 + ug.ll2);
-        rowoff[4] = bfi(M3, A + S3, A);
-        rowoff[5] = bfi(M3, A - S3, A);
+        rowoff[4] = A + wv.dw[0];
+        rowoff[5] = A + wv.dw[1];
     }
     // The byte the axis-0 shift pulls in: the first byte of the next chunk
     // (P = 1) or the last byte of the previous one (P = 0), along the row.
@@ -614,7 +622,7 @@ __device__ __forceinline__ void x16u_chunk(const uint4 *__restrict__ oth, uint4 
     const uint32_t adj = ldb(reinterpret_cast<const uint32_t *>(oth), adjoff);
     uint4 rows[NR];
 #pragma unroll
-    for (int k = 0; k < (D > 1 ? NR : 0); ++k) rows[k] = ldb(oth, rowoff[k]);
+    for (int k = 0; k < NR; ++k) rows[k] = ldb(oth, rowoff[k]);
     const uint4 s = ldb(own, A);
     // The sixteen draws (site0 = 2A + P has bits 1..4 clear) while the loads
     // are in flight; y = the final hash before its 8-bit shift, so that
@@ -638,7 +646,7 @@ __device__ __forceinline__ void x16u_chunk(const uint4 *__restrict__ oth, uint4 
         ds[j] = av[j] + sh0;
     }
 #pragma unroll
-    for (int k = 0; k < (D > 1 ? NR : 0); ++k) {
+    for (int k = 0; k < NR; ++k) {
         ds[0] += rows[k].x;
         ds[1] += rows[k].y;
         ds[2] += rows[k].z;
@@ -646,14 +654,15 @@ __device__ __forceinline__ void x16u_chunk(const uint4 *__restrict__ oth, uint4 
     }
     const uint32_t sv[4] = {s.x, s.y, s.z, s.w};
     // Rejections as bits, site 15 first, so that site i lands in bit i: a
-    // compare into VCC and an add-with-carry per site.
+    // compare into VCC and an add-with-carry per site.  Class byte d + 8 s,
+    // its threshold at LDS byte 4 (d + 8 s).
     uint32_t bits = 0;
 #pragma unroll
     for (int j = 3; j >= 0; --j) {
-        const uint32_t addr4 = (ds[j] << 3) | (sv[j] << 2);
+        const uint32_t cls4 = (sv[j] << 3) + ds[j];
 #pragma unroll
         for (int k = 3; k >= 0; --k) {
-            const uint32_t t2 = *reinterpret_cast<const uint32_t *>(Tb + ((addr4 >> (8 * k)) & 0xffu));
+            const uint32_t t2 = *reinterpret_cast<const uint32_t *>(Tb + (((cls4 >> (8 * k)) & 0xffu) << 2));
             asm("v_cmp_gt_u32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc"
                 : "+v"(bits)
                 : "v"(y[4 * j + k]), "v"(t2)
@@ -690,11 +699,10 @@ __device__ __forceinline__ void x16u_chunk(const uint4 *__restrict__ oth, uint4 
 // K = 64 >> lh16 rows per wave: a pair of waves covers 2K consecutive rows,
 // which lie on one axis-1 line (2K <= L1), so the axis-0 colour offset
 // p = (x1 + x2 + x3 + c) & 1 is the same for the whole wave and the axis-0
-// neighbour bytes are selected by a scalar branch.  Neighbour rows are
-// found by bit-field inserts on the byte offset.  Every threshold T >= 1 (the
-// host checks), so m < T is y <= T * 256 - 1 with y the hash before its final
-// 8-bit shift, decided by one compare and one add-with-carry per site.
-// Requires span to be a multiple of 64 (whole waves per chunk range).
+// neighbour bytes are selected by a scalar branch.  Every threshold T >= 1
+// (the host checks), so m < T is y <= T * 256 - 1 with y the hash before
+// its final 8-bit shift, decided by one compare and one add-with-carry per
+// site.  Requires D >= 2 and span a multiple of 64 (whole waves per range).
 template <int D, int MODE>
 __global__ __launch_bounds__(256) void ising_pass_x16u(const uint4 *__restrict__ oth, uint4 *__restrict__ own,
                                                        X16UGeom ug, int c, uint32_t seed, uint32_t skey,
@@ -702,11 +710,11 @@ __global__ __launch_bounds__(256) void ising_pass_x16u(const uint4 *__restrict__
                                                        uint32_t span) {
     constexpr int NN = 2 * D;
     constexpr int K = NN + 1;
-    __shared__ uint32_t T[2 * 16];  // T[d * 2 + s] = threshold * 256 - 1, byte address (d << 3) | (s << 2)
+    __shared__ uint32_t T[2 * 8];  // T[d + 8 s] = threshold * 256 - 1
     __shared__ long long lds[16];
     if (threadIdx.x < 2 * K) {
         int s = threadIdx.x / K, d = threadIdx.x % K;
-        T[d * 2 + s] = (table[threadIdx.x] << 8) - 1u;
+        T[d + 8 * s] = (table[threadIdx.x] << 8) - 1u;
     }
     __syncthreads();
     const uint32_t begin = (uint32_t)logical_block() * span;
@@ -714,18 +722,28 @@ __global__ __launch_bounds__(256) void ising_pass_x16u(const uint4 *__restrict__
     if (end > work) end = work;
     const uint32_t kmix = sweep_mix(skey);
     const char *Tb = reinterpret_cast<const char *>(T);
-    const uint32_t cmask = (1u << ug.lh16) - 1u;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave in block
+    // the lane's part of the byte offset: row 2 (lane >> lh16), chunk lane & mask
+    const uint32_t lanepart = ((lane >> ug.lh16) << (ug.sh + 1)) | ((lane & ((1u << ug.lh16) - 1u)) << 4);
+    const uint32_t S2 = 1u << (ug.sh + ug.ll1), S3 = 1u << (ug.sh + ug.ll1 + ug.ll2);
     int bond = 0, down = 0, total = 0;
-    for (uint32_t u = begin + threadIdx.x; u < end; u += blockDim.x) {
-        const uint32_t wv = u >> 6, l = u & 63u;
-        const uint32_t row = ((wv >> 1) << ug.lk2) + ((l >> ug.lh16) << 1) + (wv & 1u);
-        const uint32_t A = (row << ug.sh) | ((l & cmask) << 4);
-        const uint32_t x1 = row & ug.m1;
-        const uint32_t x2 = D > 2 ? (row >> ug.ll1) & ug.m2 : 0u;
-        const uint32_t x3 = D > 3 ? row >> (ug.ll1 + ug.ll2) : 0u;
-        const uint32_t p = __builtin_amdgcn_readfirstlane((x1 + x2 + x3 + (uint32_t)c) & 1u);
-        if (p) x16u_chunk<D, MODE, 1>(oth, own, ug, A, seed, kmix, Tb, c, m.on, bond, down, total);
-        else x16u_chunk<D, MODE, 0>(oth, own, ug, A, seed, kmix, Tb, c, m.on, bond, down, total);
+    for (uint32_t u0 = begin; u0 < end; u0 += blockDim.x) {
+        if (u0 + (wib << 6) >= end) break;  // whole waves: the wave's range is empty
+        const uint32_t wave = (u0 >> 6) + wib;
+        const uint32_t row0 = ((wave >> 1) << ug.lk2) + (wave & 1u);
+        X16UWave wv;
+        wv.base = row0 << ug.sh;
+        const uint32_t x2 = (row0 >> ug.ll1) & ug.m2;
+        const uint32_t x3 = row0 >> (ug.ll1 + ug.ll2);
+        wv.dz[0] = x2 == ug.m2 ? 0u - ug.m2 * S2 : S2;
+        wv.dz[1] = x2 == 0u ? ug.m2 * S2 : 0u - S2;
+        wv.dw[0] = x3 == ug.m3 ? 0u - ug.m3 * S3 : S3;
+        wv.dw[1] = x3 == 0u ? ug.m3 * S3 : 0u - S3;
+        const uint32_t p = ((row0 & 1u) + (D > 2 ? x2 : 0u) + (D > 3 ? x3 : 0u) + (uint32_t)c) & 1u;
+        const uint32_t A = wv.base + lanepart;
+        if (p) x16u_chunk<D, MODE, 1>(oth, own, ug, wv, A, seed, kmix, Tb, c, m.on, bond, down, total);
+        else x16u_chunk<D, MODE, 0>(oth, own, ug, wv, A, seed, kmix, Tb, c, m.on, bond, down, total);
     }
     if (m.on) {
         int counts[2] = {total - down, down};
