@@ -618,15 +618,23 @@ __device__ __forceinline__ void x16u_chunk(const uint4 *__restrict__ oth, uint4 
     // The byte the axis-0 shift pulls in: the first byte of the next chunk
     // (P = 1) or the last byte of the previous one (P = 0), along the row.
     const uint32_t adjoff = P ? bfi(cm, A + 16u, A) : bfi(cm, A - 16u, A) + 12u;
-    const uint4 a = ldb(oth, A);
-    const uint32_t adj = ldb(reinterpret_cast<const uint32_t *>(oth), adjoff);
-    uint4 rows[NR];
+    // Even waves of a block issue their loads first and compute the sixteen
+    // draws while they are in flight; odd waves compute the draws first and
+    // then load, which spreads the pass's read burst over time (measured
+    // about 1% faster than every wave loading first).
+    uint4 a, s, rows[NR];
+    uint32_t adj;
+    auto load = [&]() {
+        a = ldb(oth, A);
+        adj = ldb(reinterpret_cast<const uint32_t *>(oth), adjoff);
 #pragma unroll
-    for (int k = 0; k < NR; ++k) rows[k] = ldb(oth, rowoff[k]);
-    const uint4 s = ldb(own, A);
-    // The sixteen draws (site0 = 2A + P has bits 1..4 clear) while the loads
-    // are in flight; y = the final hash before its 8-bit shift, so that
-    // m < T  <=>  y <= T * 256 - 1.
+        for (int k = 0; k < NR; ++k) rows[k] = ldb(oth, rowoff[k]);
+        s = ldb(own, A);
+    };
+    const bool late = (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) & 1u) != 0;
+    if (!late) load();
+    // The draws (site0 = 2A + P has bits 1..4 clear); y = the final hash
+    // before its 8-bit shift, so that m < T  <=>  y <= T * 256 - 1.
     uint32_t y[16];
     __builtin_amdgcn_sched_barrier(0);
     {
@@ -636,6 +644,7 @@ __device__ __forceinline__ void x16u_chunk(const uint4 *__restrict__ oth, uint4 
         for (int k = 0; k < 16; ++k) y[k] = lowbias32_tail(lowbias32_mid(v0 ^ (2u * (uint32_t)k)) ^ kmix);
     }
     __builtin_amdgcn_sched_barrier(0);
+    if (late) load();
     // Down-neighbour counts, four sites per dword.
     uint32_t ds[4];
     const uint32_t av[4] = {a.x, a.y, a.z, a.w};
