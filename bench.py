@@ -27,6 +27,7 @@ from __future__ import annotations
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -474,66 +475,112 @@ def bench_cpu_stream(P, ev, steps, warmup):
     }
 
 
-def learned_small(P, ev, stats):
-    """The same comparison on 64^2 at beta_c, where tau_int of |M| (about 150
-    sweeps) converges within 100,000 steps: train 300 steps on the chain's
-    configurations, then time plain and mixed chains of 100,000 steps each."""
+W3_SLICE = slice(9568, 9857)  # the last layer's weights and bias: the logits are linear in them
+
+
+def calibrate_sharpness(P, chain, net, sweeps=100):
+    """Scale the last layer (logits x alpha) to the alpha of highest block
+    acceptance on `chain`, from {1, 1.25, 1.5, 1.75, 2, 2.5}.  The proposal
+    stays exact for any alpha: the MH ratio uses the logits proposed with."""
+    base = net.params
+    best = (0.0, 1.0)
+    for alpha in (1.0, 1.25, 1.5, 1.75, 2.0, 2.5):
+        p = base.copy()
+        p[W3_SLICE] *= alpha
+        net.params = p
+        a0, p0, _ = chain.block_stats()
+        chain.block_sweeps(net, sweeps)
+        chain.sync()
+        a1, p1, _ = chain.block_stats()
+        best = max(best, ((a1 - a0) / (p1 - p0), alpha))
+    p = base.copy()
+    p[W3_SLICE] *= best[1]
+    net.params = p
+    return {"alpha": best[1], "acceptance": round(best[0], 5)}
+
+
+def compare_chains(P, ev, stats, L, beta, st, net, steps, every, seed):
+    """Plain (one checkerboard sweep per step) and mixed (one block sweep of
+    both colours, then one checkerboard sweep) chains from the same
+    configuration, `steps` steps each with E and |M| recorded every `every`
+    steps.  tau_int (in steps) from Sokal's window (statistics.rs:83-137);
+    samples/s = steps/s / (2 tau_int); a window counts as converged when the
+    series spans at least 50 tau_int (Sokal's rule of thumb)."""
     import numpy as np
 
-    L, beta, steps = 64, 0.4406868, 100000
     n = L * L
-    start = np.random.default_rng(SEED + 2).integers(0, 2, n).astype(np.uint8)
-    c = P.ising_gpu_chain([L, L], 1.0, 0.0, beta, SEED + 2, start)
-    c.advance(5000)
-    net = P.BlockNet(SEED + 2)
-    net.train(c, 300, 2, 3e-3)
-    st = c.read()
-    c.close()
     out = {}
     for mode in ("plain", "mixed"):
-        ch = P.ising_gpu_chain([L, L], 1.0, 0.0, beta, SEED + 3, st)
-        ch.record(1, steps)
+        ch = P.ising_gpu_chain([L, L], 1.0, 0.0, beta, seed, st)
+        ch.record(every, steps // every + 1)
         ch.sync()
         if mode == "plain":
             wall, _ = timed(P, ev, ch.stream, lambda: (ch.advance(steps), ch.sync()))
         else:
             wall, _ = timed(P, ev, ch.stream, lambda: (ch.advance_with_blocks(net, steps, 1), ch.sync()))
-        m = np.array([abs(o.spin_sum) / n for o in ch.series()])
-        est = stats.reduce(m)
+        ser = ch.series()
         acc, prop, _ = ch.block_stats()
-        out[mode] = {"steps_per_s": round(steps / wall, 1), "tau_int_abs_m": round(est.tau_int, 2),
-                     "samples_per_s": round(steps / wall / (2.0 * est.tau_int), 2),
-                     "window_converged": bool(steps >= 50 * est.tau_int),
-                     "block_acceptance": (acc / prop) if prop else None}
         ch.close()
+        res = {"steps": steps, "steps_per_s": round(steps / wall, 1), "record_every": every,
+               "block_acceptance": round(acc / prop, 5) if prop else None}
+        for name, series in (("abs_m", [abs(o.spin_sum) / n for o in ser]), ("energy", [o.energy / n for o in ser])):
+            est = stats.reduce(np.array(series))
+            tau = est.tau_int * every
+            res[name] = {"mean": est.mean, "stderr": est.stderr, "tau_int_steps": round(tau, 2),
+                         "n_eff": round(est.n_eff, 1), "samples_per_s": round(steps / wall / (2.0 * tau), 4),
+                         "window_converged": bool(len(series) >= 50 * est.tau_int)}
+        out[mode] = res
+    for name in ("abs_m", "energy"):
+        out[f"ratio_vs_plain_{name}"] = round(out["mixed"][name]["samples_per_s"] / out["plain"][name]["samples_per_s"], 4)
+        d = out["mixed"][name]["mean"] - out["plain"][name]["mean"]
+        err = math.hypot(out["mixed"][name]["stderr"], out["plain"][name]["stderr"])
+        out[f"agreement_sigma_{name}"] = round(abs(d) / err, 2) if err > 0 else None
+    return out
+
+
+def learned_small(P, ev, stats):
+    """The same comparison on 64^2 at beta_c: train 2,000 steps on the
+    chain's configurations, calibrate the logit scale, then plain and mixed
+    chains of 400,000 steps each (tau_int of |M| is about 180 sweeps)."""
+    import numpy as np
+
+    L, beta = 64, 0.4406868
+    n = L * L
+    start = np.random.default_rng(SEED + 2).integers(0, 2, n).astype(np.uint8)
+    c = P.ising_gpu_chain([L, L], 1.0, 0.0, beta, SEED + 2, start)
+    c.advance(5000)
+    net = P.BlockNet(SEED + 2)
+    net.train(c, 2000, 5, 3e-3)
+    cal = calibrate_sharpness(P, c, net)
+    st = c.read()
+    c.close()
+    out = compare_chains(P, ev, stats, L, beta, st, net, 400000, 1, SEED + 3)
+    out["calibration"] = cal
     net.close()
-    out["ratio_vs_plain"] = round(out["mixed"]["samples_per_s"] / out["plain"]["samples_per_s"], 4)
-    out["tau_ratio_plain_over_mixed"] = round(out["plain"]["tau_int_abs_m"] / out["mixed"]["tau_int_abs_m"], 3)
     return out
 
 
 def bench_learned(P, ev):
     """2D Ising 512^2 at beta_c: the learned block proposal against the plain
-    checkerboard.  A chain from a hot start runs 2,000 sweeps, then trains
-    the network 1,000 Adam steps, one per configuration taken every 5
-    sweeps (the plain sampler's configurations).  From the trained chain's
-    configuration, a plain chain (one checkerboard sweep per step) and a
-    mixed chain (one block sweep of both block colours, then one
-    checkerboard sweep, per step) each run the same number of steps with
-    |M| recorded every step; tau_int of |M| comes from Sokal's window
-    (statistics.rs:83-137) and samples/s = steps/s / (2 tau_int)."""
+    checkerboard.  A chain from a hot start runs 2,000 sweeps, then the
+    network trains 20,000 Adam steps, one per configuration taken every 5
+    sweeps (20,000 configurations of the plain sampler), and the logit scale
+    is calibrated on block acceptance.  From the trained chain's
+    configuration a plain chain and a mixed chain each run 1,000,000 steps
+    with E and |M| recorded every 10 steps (compare_chains)."""
     import numpy as np
 
     from plaquette_amd import statistics as stats
 
-    L, beta, steps = 512, 0.4406868, 20000
+    L, beta, steps = 512, 0.4406868, 1000000
     n = L * L
     start = np.random.default_rng(SEED).integers(0, 2, n).astype(np.uint8)
     c = P.ising_gpu_chain([L, L], 1.0, 0.0, beta, SEED, start)
     c.advance(2000)
     net = P.BlockNet(SEED)
-    wall_train, ms_train = timed(P, ev, c.stream, lambda: net.train(c, 1000, 5, 3e-3))
+    wall_train, ms_train = timed(P, ev, c.stream, lambda: net.train(c, 20000, 5, 3e-3))
     losses = net.train(c, 1, 5, 3e-3)
+    cal = calibrate_sharpness(P, c, net)
     st = c.read()
     # the block sweep alone, by events on its stream
     k = 50
@@ -542,39 +589,24 @@ def bench_learned(P, ev):
     _, ms_blocks = timed(P, ev, c.stream, lambda: (c.block_sweeps(net, k), c.sync()))
     block_us = ms_blocks * 1e3 / k
     c.close()
-    runs = {}
-    for mode in ("plain", "mixed"):
-        ch = P.ising_gpu_chain([L, L], 1.0, 0.0, beta, SEED + 1, st)
-        ch.record(1, steps)
-        ch.sync()
-        if mode == "plain":
-            wall, _ = timed(P, ev, ch.stream, lambda: (ch.advance(steps), ch.sync()))
-        else:
-            wall, _ = timed(P, ev, ch.stream, lambda: (ch.advance_with_blocks(net, steps, 1), ch.sync()))
-        m = np.array([abs(o.spin_sum) / n for o in ch.series()])
-        est = stats.reduce(m)
-        acc, prop, _ = ch.block_stats()
-        runs[mode] = {"steps_per_s": steps / wall, "tau_int_abs_m": est.tau_int, "mean_abs_m": est.mean,
-                      "samples_per_s": steps / wall / (2.0 * est.tau_int),
-                      "window_converged": bool(steps >= 50 * est.tau_int),
-                      "block_acceptance": (acc / prop) if prop else None}
-        ch.close()
+    runs = compare_chains(P, ev, stats, L, beta, st, net, steps, 10, SEED + 1)
     net.close()
-    ratio = runs["mixed"]["samples_per_s"] / runs["plain"]["samples_per_s"]
     small = learned_small(P, ev, stats)
     flop = (n // 64) * BLOCK_FLOP
     tflops = flop / (block_us * 1e-6) / 1e12
+    mixed = runs["mixed"]
     return {
         "metric": "tau_int-corrected samples/s of |M|, learned block proposal vs plain checkerboard",
-        "value": round(runs["mixed"]["samples_per_s"], 4), "unit": "samples/s", "n_gpus": 1, "steps": steps,
-        "warmup": 2000, "ms_per_step": round(1e3 / runs["mixed"]["steps_per_s"], 5), "higher_is_better": True,
+        "value": mixed["abs_m"]["samples_per_s"], "unit": "samples/s", "n_gpus": 1, "steps": steps,
+        "warmup": 2000, "ms_per_step": round(1e3 / mixed["steps_per_s"], 5), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "bf16 network, fp32 accumulate; u8 states",
         "data": "synthetic (chain configurations)",
         "config": {"workload": "2D Ising 512^2 at beta_c = 0.4406868, 4,096 blocks of 8x8, 3-layer 3x3 CNN with "
-                               "32 channels, 1,000 training steps", "lattice": [L, L]},
-        "ratio_vs_plain": round(ratio, 4), "plain": runs["plain"], "mixed": runs["mixed"],
-        "converged_64": small,
-        "training": {"wall_s": round(wall_train, 3), "final_loss": float(losses[-1])},
+                               "32 channels (middle layer dilated by 2), trained on 20,000 configurations",
+                   "lattice": [L, L]},
+        "ratio_vs_plain": runs["ratio_vs_plain_abs_m"], "comparison_512": runs, "comparison_64": small,
+        "training": {"steps": 20000, "wall_s": round(wall_train, 3), "final_loss": float(losses[-1]),
+                     "calibration": cal},
         "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(tflops / 2500.0, 5), "traffic": None,
                      "kernel": "block_pass_kernel (both block colours)", "avg_sweep_us": round(block_us, 2)},

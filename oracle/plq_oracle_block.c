@@ -16,7 +16,8 @@
  *                         adds and multiplies, so any compiler that keeps
  *                         them unfused computes the same bits.
  *   orc_blocknet_logits   forward pass of the 3-layer CNN (boundary ring ->
- *                         64 logits) in FP64, optionally reproducing the
+ *                         64 logits; middle layer dilated by 2) in FP64,
+ *                         optionally reproducing the
  *                         device's BF16 rounding of weights and activations.
  *   orc_blocknet_loss_grad  mean binary cross-entropy of the interior given
  *                         the ring, and its gradient (straight through the
@@ -32,6 +33,8 @@
 #include "plq_oracle.h"
 
 #define BN_B ORC_BN_BLOCK
+/* the middle layer's taps are two sites apart (dilation 2) */
+#define DIL2 2
 #define BN_C ORC_BN_CHANNELS
 #define OFF_W1 0
 #define OFF_B1 (OFF_W1 + 9 * BN_C)
@@ -135,7 +138,7 @@ static void forward(const net_t *n, int round, act_t *a) {
             for (int co = 0; co < BN_C; ++co) {
                 double s = n->b2[co];
                 for (int t = 0; t < 9; ++t) {
-                    int yy = y + t / 3 - 1, xx = x + t % 3 - 1;
+                    int yy = y + DIL2 * (t / 3 - 1), xx = x + DIL2 * (t % 3 - 1);
                     if (!inside(yy, xx)) continue;
                     for (int ci = 0; ci < BN_C; ++ci) s += n->w2[t][co][ci] * a->h1[yy][xx][ci];
                 }
@@ -229,7 +232,7 @@ double orc_blocknet_loss_grad(const float *params, int round, int64_t L0, int64_
                         double s = 0.0;
                         for (int y = 0; y < BN_B; ++y)
                             for (int x = 0; x < BN_B; ++x) {
-                                int yy = y + t / 3 - 1, xx = x + t % 3 - 1;
+                                int yy = y + DIL2 * (t / 3 - 1), xx = x + DIL2 * (t % 3 - 1);
                                 if (inside(yy, xx)) s += g2[y][x][co] * a->h1[yy][xx][ci];
                             }
                         grad[OFF_W2 + (t * BN_C + co) * BN_C + ci] += s;
@@ -239,7 +242,7 @@ double orc_blocknet_loss_grad(const float *params, int round, int64_t L0, int64_
                     for (int ci = 0; ci < BN_C; ++ci) {
                         double s = 0.0;
                         for (int t = 0; t < 9; ++t) {
-                            int yy = y - (t / 3 - 1), xx = x - (t % 3 - 1);
+                            int yy = y - DIL2 * (t / 3 - 1), xx = x - DIL2 * (t % 3 - 1);
                             if (!inside(yy, xx)) continue;
                             for (int co = 0; co < BN_C; ++co) s += n->w2[t][co][ci] * g2[yy][xx][co];
                         }

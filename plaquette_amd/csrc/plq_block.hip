@@ -3,7 +3,9 @@
 // The lattice is cut into 8x8 blocks coloured (bx + by) & 1; the blocks of
 // one colour share no bond, so a pass updates all of them at once.  For each
 // block a 3-layer CNN (3x3 kernels, 32 channels, tanh) maps the 32 spins of
-// the block's boundary ring to 64 logits, one per interior site, and the
+// the block's boundary ring to 64 logits, one per interior site.  The
+// middle layer's taps are two sites apart (dilation 2), so that the three
+// layers reach all 64 sites from the ring (depths 0, 0-2 and 0-3).  The
 // block proposal draws the interior as 64 independent Bernoulli variables
 // (state 0, spin +1, with probability sigmoid(logit)).  Because the logits
 // depend on the ring only, which the move leaves unchanged, the proposal
@@ -42,6 +44,7 @@ constexpr int B = BN_BLOCK;      // 8
 constexpr int CH = BN_CHANNELS;  // 32
 constexpr int HS = 40;           // LDS row stride (bf16) of an activation: 80 B, conflict-free b128 reads
 constexpr int TS = 72;           // LDS row stride (bf16) of the transposed gradient
+constexpr int DIL2 = 2;          // dilation of the middle layer: its taps are 2 sites apart
 
 __device__ __forceinline__ uint16_t bf16_bits(float f) {
     uint32_t u = __float_as_uint(f);
@@ -126,7 +129,7 @@ __device__ __forceinline__ void layer2(const BlockNetView &n, const bf16x8 *wB, 
         f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int t = 0; t < 9; ++t) {
-            const int yy = y + t / 3 - 1, xx = x + t % 3 - 1;
+            const int yy = y + DIL2 * (t / 3 - 1), xx = x + DIL2 * (t % 3 - 1);
             bf16x8 a = inside(yy, xx) ? lds_frag(&h1[(yy * B + xx) * HS + kb]) : zero_frag();
             bf16x8 b0, b1;
             if (PRELOADED) {
@@ -358,7 +361,7 @@ __global__ __launch_bounds__(64) void block_train_kernel(const uint8_t *__restri
         // dW2[t][co][ci] += sum_p g2[p][co] h1[p + t][ci]: M = co, N = ci, K = positions
 #pragma unroll 1
         for (int t = 0; t < 9; ++t) {
-            const int dy = t / 3 - 1, dx = t % 3 - 1;
+            const int dy = DIL2 * (t / 3 - 1), dx = DIL2 * (t % 3 - 1);
 #pragma unroll
             for (int mt = 0; mt < 2; ++mt) {
                 f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
@@ -401,7 +404,7 @@ __global__ __launch_bounds__(64) void block_train_kernel(const uint8_t *__restri
             f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int t = 0; t < 9; ++t) {
-                const int yy = y - (t / 3 - 1), xx = x - (t % 3 - 1);
+                const int yy = y - DIL2 * (t / 3 - 1), xx = x - DIL2 * (t % 3 - 1);
                 bf16x8 a = inside(yy, xx) ? lds_frag(&S.g2[(yy * B + xx) * HS + kb]) : zero_frag();
                 bf16x8 b0 = as_bf16x8(reinterpret_cast<const u32x4 *>(net.w2t)[(t * 2 + 0) * 64 + lane]);
                 bf16x8 b1 = as_bf16x8(reinterpret_cast<const u32x4 *>(net.w2t)[(t * 2 + 1) * 64 + lane]);

@@ -311,3 +311,52 @@ def test_mixed_chain_matches_kaufman(gpu):
     cv = S.specific_heat(e, beta, n)
     want_c = X.specific_heat(L, L, beta)
     assert abs(cv.value - want_c) < 4.5 * cv.stderr, (cv.value, want_c, cv.stderr)
+
+
+def test_trained_network_beats_the_untrained_one(gpu):
+    # 64^2 at beta_c.  The same initial weights before and after 600 Adam
+    # steps on the plain chain's configurations (logits then scaled by 1.5,
+    # the calibrated sharpness): the trained network's blocks are accepted
+    # about 16% of the time and block sweeps alone move E between almost
+    # every pair of measurements, while the untrained network's proposals
+    # are almost never accepted (7e-5), so its block sweeps barely move the
+    # chain.  Block sweeps alone decorrelate E slowly (tau_int about 1,000
+    # block sweeps, printed), and interleaved with plain sweeps the trained
+    # network does not measurably lower tau_int of E or |M| per step at this
+    # size; the bench reports those numbers.
+    from plaquette_amd import statistics as S
+
+    L, n = 64, 64 * 64
+    c = _chain(gpu, (L, L), BETA_C, 91, _hot(n, 92))
+    c.advance(3000)
+    trained = gpu.BlockNet(93)
+    untrained = gpu.BlockNet(93)
+    trained.train(c, 600, 5, 3e-3)
+    p = trained.params
+    p[9568:9857] *= 1.5
+    trained.params = p
+    st0 = c.read()
+    c.close()
+    out = {}
+    for name, net in (("trained", trained), ("untrained", untrained)):
+        # block sweeps alone, E measured every 10 of them
+        ch = _chain(gpu, (L, L), BETA_C, 94, st0)
+        es = []
+        for _ in range(2000):
+            ch.block_sweeps(net, 10)
+            es.append(ch.measure().energy / n)
+        acc, prop, _ = ch.block_stats()
+        out[name] = (acc / prop, np.array(es))
+        ch.close()
+    trained.close()
+    untrained.close()
+    acc_t, e_t = out["trained"]
+    acc_u, e_u = out["untrained"]
+    est = S.reduce(e_t)
+    print("acceptance", acc_t, acc_u, "tau_int(E) of block sweeps, in units of 10:", est.tau_int)
+    moved_t, moved_u = (np.diff(e_t) != 0).mean(), (np.diff(e_u) != 0).mean()
+    msd_t, msd_u = np.mean(np.diff(e_t) ** 2), np.mean(np.diff(e_u) ** 2)
+    print("E changed between measurements:", moved_t, moved_u, "mean squared change:", msd_t, msd_u)
+    assert acc_t > 0.05 and acc_u < 1e-3     # measured 0.16 and 7e-5
+    assert moved_t > 0.9 and moved_u < 0.1   # the untrained network almost never moves the chain
+    assert msd_t > 10 * msd_u                # and the trained one moves E far more per block sweep

@@ -94,6 +94,26 @@ def test_logits_depend_on_the_boundary_ring_only():
     assert not np.array_equal(O.blocknet_logits(params, (L, L), st3)[b], base[b])
 
 
+def test_every_interior_site_sees_the_ring():
+    # Receptive field: layer 1 reaches depth 0 from the ring, the dilated
+    # middle layer depths 0-2, layer 3 depths 0-3, so every one of the 64
+    # logits -- the centre 2x2 included -- moves when some ring spin flips.
+    # With an undilated middle layer the centre's logits would stay at the
+    # bias for every ring.
+    params = _params(18)
+    L = 16
+    st = np.random.default_rng(19).integers(0, 2, L * L).astype(np.uint8)
+    base = O.blocknet_logits(params, (L, L), st, round_bf16=False)[0]
+    moved = np.zeros(64, bool)
+    ring = [(-1, x) for x in range(8)] + [(8, x) for x in range(8)] + [(y, -1) for y in range(8)] + \
+           [(y, 8) for y in range(8)]
+    for y, x in ring:
+        st2 = st.copy()
+        st2[(y % L) * L + (x % L)] ^= 1
+        moved |= np.abs(O.blocknet_logits(params, (L, L), st2, round_bf16=False)[0] - base) > 1e-12
+    assert moved.all(), np.flatnonzero(~moved)
+
+
 def test_unbiased_network_is_odd_under_global_flip():
     rng = np.random.default_rng(9)
     params = _params(10, zero_bias=True)
