@@ -163,6 +163,90 @@ def cpu_baseline_all_cores(seconds_target: float):
     }
 
 
+def cpu_baseline_ising2d_64(seconds_target: float):
+    """The reference CPU checkerboard path on 64^2 (PCG64 stream, f64
+    acceptance, neighbour and parity tables) on one core, bounded sample."""
+    import numpy as np
+
+    import oracle as O
+
+    shape = [64, 64]
+    n = 4096
+    st = np.random.default_rng(SEED).integers(0, 2, n).astype(np.uint8)
+    st, secs = O.pmode_ising_sweeps(shape, O.KERNEL_METROPOLIS, 0.4406868, 1.0, 0.0, SEED, st, 100)
+    sweeps = max(100, int(100 * seconds_target / max(secs, 1e-6)))
+    st, secs2 = O.pmode_ising_sweeps(shape, O.KERNEL_METROPOLIS, 0.4406868, 1.0, 0.0, SEED, st, sweeps)
+    return {
+        "value": sweeps * n / (secs2 * 1e9), "unit": "site-updates/ns", "cores": 1, "kind": "port",
+        "sample": f"64^2 Ising at beta_c, {sweeps} checkerboard Metropolis sweeps ({secs2:.1f} s on one host core; "
+                  f"C restatement of updater.rs:77-96,316-324 with the PCG64 stream)",
+    }
+
+
+def cpu_baseline_potts_sw(seconds_target: float):
+    """The oracle's sequential Swendsen-Wang sweep at 4096^2 on one core: the
+    reference CPU ClusterUpdate (updater.rs:541-613) -- bond draw, union-find
+    forest with minimum-site roots (cluster.rs:44-227), relabel per cluster --
+    driven by the keyed stream; bounded sample of whole sweeps."""
+    import numpy as np
+
+    import oracle as O
+
+    shape = [4096, 4096]
+    n = 4096 * 4096
+    beta = math.log(1 + math.sqrt(3))
+    d = O.cluster_desc(shape, q=3, beta=beta, gap=1.0, seed=SEED)
+    st = np.random.default_rng(SEED).integers(0, 3, n).astype(np.uint8)
+    t0 = time.perf_counter()
+    st, _ = O.cluster_sweeps(d, st, 0, 1)
+    one = time.perf_counter() - t0
+    sweeps = max(1, int(seconds_target / max(one, 1e-3)))
+    t0 = time.perf_counter()
+    st, _ = O.cluster_sweeps(d, st, 1, sweeps)
+    secs = time.perf_counter() - t0
+    return {
+        "value": sweeps * n / (secs * 1e9), "unit": "site-updates/ns", "cores": 1, "kind": "port",
+        "sample": f"4096^2 q=3 Potts at beta_c, {sweeps} Swendsen-Wang sweeps after one ({secs:.1f} s on one host "
+                  f"core; C restatement of updater.rs:541-613 and cluster.rs:44-227)",
+    }
+
+
+def cpu_baseline_z2_batch(seconds_target: float):
+    """The oracle's sequential Z2 link heat bath (staple table, f64 weights;
+    updater.rs:109-167, 325-343) over the 48-replica workload on one core.
+    Per replica size the table build is taken out by timing one sweep and
+    k + 1 sweeps; the rate is links per second over all 48 replicas."""
+    import numpy as np
+
+    import oracle as O
+
+    shapes, betas, seeds = z2_batch_shapes()
+    per_size = {}
+    budget = seconds_target / 4
+    for L in (8, 12, 16, 20):
+        shape = [L] * 4
+        nl = L ** 4 * 4
+        d = O.local_desc(shape, model=O.MODEL_Z2, kernel=O.KERNEL_HEAT_BATH, beta=0.44, seed=SEED)
+        st = np.zeros(nl, np.uint8)
+        t0 = time.perf_counter()
+        st = O.local_sweeps(d, st, 0, 1)
+        one = time.perf_counter() - t0
+        k = max(2, int(budget / max(one, 1e-4)))
+        t0 = time.perf_counter()
+        st = O.local_sweeps(d, st, 0, k + 1)
+        many = time.perf_counter() - t0
+        per_size[L] = max(many - one, 1e-9) / k  # seconds per sweep of one replica, table build excluded
+    links = sum(int(np.prod(s)) * 4 for s in shapes)
+    secs_per_sweep = sum(per_size[s[0]] for s in shapes)
+    return {
+        "value": links / (secs_per_sweep * 1e9), "unit": "link-updates/ns", "cores": 1, "kind": "port",
+        "sample": "48-replica 4D Z2 workload, link heat bath at beta 0.44: one replica per size timed over "
+                  + ", ".join(f"L={L}: {per_size[L] * 1e3:.1f} ms/sweep" for L in per_size)
+                  + " on one host core (C restatement of updater.rs:109-167,325-343 with the staple table, "
+                  "table build excluded), summed over the 12 replicas of each size",
+    }
+
+
 def read_traffic():
     """HBM bytes per colour-pass launch from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
@@ -224,7 +308,7 @@ def timed(P, ev, stream_of, run):
     return time.perf_counter() - t0, ms
 
 
-def bench_potts_sw(P, ev, steps, warmup):
+def bench_potts_sw(P, ev, steps, warmup, cpu_seconds=None):
     """2D q=3 Potts 4096^2 Swendsen-Wang at beta_c = ln(1 + sqrt 3); order
     parameter (label counts) recorded every sweep; cluster-size histogram of
     the last sweep, and separately the rate with the histogram of every
@@ -278,6 +362,7 @@ def bench_potts_sw(P, ev, steps, warmup):
                                   "site_updates_per_ns": round(hist_steps * n / (hist_wall * 1e9), 2),
                                   "histograms_kept": len(hist_series),
                                   "last_clusters": hist_series[-1].n_clusters if hist_series else None},
+        "cpu_baseline": cpu_baseline_potts_sw(cpu_seconds) if cpu_seconds else None,
     }
 
 
@@ -317,7 +402,7 @@ def cache_copy_gbs(ev, nbytes: int, reps: int) -> float:
     return 2.0 * nbytes * reps / (ms * 1e-3) / 1e9
 
 
-def bench_z2_batch(P, ev, steps, warmup):
+def bench_z2_batch(P, ev, steps, warmup, cpu_seconds=None):
     """48 replicas of 4D Z2 (L = 8, 12, 16, 20; 12 each), link heat bath, beta
     grid of 12 points in [0.40, 0.48] per size, replica exchange every 10
     sweeps, cold start; launch shape picked by the autotuner."""
@@ -355,10 +440,11 @@ def bench_z2_batch(P, ev, steps, warmup):
         "tuned_launch": tuned,
         "mean_plaquette_last": [round(float(x), 5) for x in (sums[-1] / n_plaq)] if len(sums) else None,
         "exchange_acceptance": round(float(acc.sum()) / max(1, float(att.sum())), 4),
+        "cpu_baseline": cpu_baseline_z2_batch(cpu_seconds) if cpu_seconds else None,
     }
 
 
-def bench_ising2d_64(P, ev, steps, warmup):
+def bench_ising2d_64(P, ev, steps, warmup, cpu_seconds=None):
     """2D Ising 64^2 at beta_c, checkerboard Metropolis, hot start, E and M
     recorded every sweep (latency bound: 4,096 sites)."""
     import numpy as np
@@ -383,6 +469,7 @@ def bench_ising2d_64(P, ev, steps, warmup):
         "config": {"workload": "2D Ising 64^2, beta=0.4406868, checkerboard Metropolis, hot start, E and |M| "
                                "every sweep", "lattice": shape},
         "mean_energy_per_site": float(e.mean()), "mean_abs_m_per_site": float(m.mean()),
+        "cpu_baseline": cpu_baseline_ising2d_64(cpu_seconds) if cpu_seconds else None,
     }
 
 
@@ -645,10 +732,10 @@ def main():
         sys.exit(1)
     ev = HipEvents()
     if args.workload == "potts-sw":
-        print(json.dumps(bench_potts_sw(P, ev, 200, 10)))
+        print(json.dumps(bench_potts_sw(P, ev, 200, 10, None if args.no_cpu else args.cpu_seconds)))
         return
     if args.workload == "z2-batch":
-        print(json.dumps(bench_z2_batch(P, ev, 500, 20)))
+        print(json.dumps(bench_z2_batch(P, ev, 500, 20, None if args.no_cpu else args.cpu_seconds)))
         return
     if args.workload == "cpu-stream":
         print(json.dumps(bench_cpu_stream(P, ev, 50, 5)))
@@ -657,7 +744,7 @@ def main():
         print(json.dumps(bench_learned(P, ev)))
         return
     if args.workload == "ising2d-64":
-        print(json.dumps(bench_ising2d_64(P, ev, 10000, 1000)))
+        print(json.dumps(bench_ising2d_64(P, ev, 10000, 1000, None if args.no_cpu else args.cpu_seconds)))
         return
     if args.workload == "cluster-small":
         print(json.dumps(bench_cluster_small(P, ev, 10000, 500)))
