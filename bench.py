@@ -247,21 +247,38 @@ def cpu_baseline_z2_batch(seconds_target: float):
     }
 
 
-def read_traffic():
-    """HBM bytes per colour-pass launch from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
-    path = os.path.join(REPO, "profiles", "ising3d_256_pass_traffic.txt")
+def read_profile_value(name: str, key: str):
+    """One `key: value` line of a committed rocprofv3 summary under profiles/."""
+    path = os.path.join(REPO, "profiles", name)
     if not os.path.exists(path):
         return None
-    vals = {}
     for line in open(path):
         if ":" in line and not line.startswith("#"):
             k, v = line.split(":", 1)
-            vals[k.strip()] = v.strip()
-    try:
-        return float(vals["hbm_bytes_per_launch"])
-    except (KeyError, ValueError):
+            if k.strip() == key:
+                try:
+                    return float(v.strip())
+                except ValueError:
+                    return None
+    return None
+
+
+def read_traffic():
+    """HBM bytes per colour-pass launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    return read_profile_value("ising3d_256_pass_traffic.txt", "hbm_bytes_per_launch")
+
+
+def valu_roof(pass_ms: float):
+    lane_ops = read_profile_value("ising3d_256_pass_counters.txt", "valu_lane_ops_per_launch")
+    peak = read_profile_value("ising3d_256_pass_counters.txt", "valu_peak_lane_ops_per_s")
+    per_site = read_profile_value("ising3d_256_pass_counters.txt", "valu_insts_per_site_update")
+    if not lane_ops or not peak:
         return None
+    achieved = lane_ops / (pass_ms * 1e-3)
+    return {"achieved": round(achieved / 1e12, 2), "peak": round(peak / 1e12, 2), "unit": "T lane-ops/s",
+            "frac": round(achieved / peak, 4), "valu_insts_per_site_update": per_site,
+            "source": "profiles/ising3d_256_pass_counters.txt"}
 
 
 def profile_passes(sweeps: int, copy_mib: int) -> None:
@@ -788,10 +805,14 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "ising_pass_x16<3,0> (one colour pass)",
+            "kernel": "ising_pass_x16u<3,0> (one colour pass)",
             "avg_launch_us": round(metro["pass_ms"] * 1e3, 3),
             "site_updates_per_ns_kernel": round((n // 2) / (metro["pass_ms"] * 1e6), 1),
         },
+        # The pass is bound by VALU issue more than by bytes: instructions per
+        # launch from the committed SQ counter summary, over the chip's measured
+        # v_mul_lo_u32 issue rate, against this run's launch time.
+        "valu_roof": valu_roof(metro["pass_ms"]),
         "metropolis": {k: (round(v, 6) if isinstance(v, float) else v) for k, v in metro.items()},
         "heat_bath": {k: (round(v, 6) if isinstance(v, float) else v) for k, v in hb.items()},
         "heat_bath_roofline_frac": round(bytes_per_launch / (hb["pass_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
