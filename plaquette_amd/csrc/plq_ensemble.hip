@@ -177,6 +177,117 @@ __global__ __launch_bounds__(256) void ens_link_pass(uint32_t *__restrict__ word
     }
 }
 
+// Index deltas of one-step moves in a colour array: moving a site along axis
+// a changes its index r * H + x0 / 2 by a delta that depends only on that
+// axis's coordinate (and on x0's parity for axis 0), so moves along several
+// axes add their deltas.
+struct EnsDeltas {
+    uint32_t p[4], m[4];  // successor / predecessor along each axis (mod 2^32)
+};
+
+template <int D>
+__device__ __forceinline__ EnsDeltas ens_deltas(const EnsQuad &Q, const LinkSite &s) {
+    EnsDeltas d;
+    const uint32_t H = (uint32_t)Q.H, L0 = (uint32_t)Q.L[0];
+    // axis 0: x0 odd moves to the next index going up, x0 even going down
+    const bool odd = (s.x0 & 1u) != 0;
+    d.p[0] = odd ? (s.x0 + 1u == L0 ? 0u - (H - 1u) : 1u) : 0u;
+    d.m[0] = odd ? 0u : (s.x0 == 0u ? H - 1u : 0u - 1u);
+#pragma unroll
+    for (int a = 1; a < 4; ++a) {
+        if (a >= D) break;
+        const uint32_t l = (uint32_t)Q.L[a];
+        const uint32_t st = (uint32_t)Q.rstride[a] * H;  // uniform
+        d.p[a] = s.x[a] + 1u == l ? 0u - (l - 1u) * st : st;
+        d.m[a] = s.x[a] == 0u ? (l - 1u) * st : 0u - st;
+    }
+    return d;
+}
+
+// One (MU, c) pass of every replica, as ens_link_pass, with the staple words
+// addressed by index deltas from the lane's own index (no per-neighbour
+// coordinate arithmetic) and the four replicas' decisions taken as in the
+// uniform-parity site pass: the class bytes cur * (K + 1) + n of all four
+// replicas in one dword, and per replica one compare of the unshifted hash
+// y against threshold * 256 - 1 into VCC and one add-with-carry.  Every
+// threshold of every beta slot must be at least 1 (the host checks).
+template <int D, int MODE, int MU>
+__global__ __launch_bounds__(256) void ens_link_pass2(uint32_t *__restrict__ words, const EnsQuad *__restrict__ quads,
+                                                      const uint16_t *__restrict__ block_quad,
+                                                      const uint32_t *__restrict__ tables,
+                                                      const uint32_t *__restrict__ beta_slot,
+                                                      const uint32_t *__restrict__ sweep_base, uint32_t sweep_off,
+                                                      int c, int per_lane) {
+    constexpr int K = 2 * (D - 1);
+    constexpr int TL = 2 * (K + 1);
+    __shared__ uint32_t T[4 * TL];  // T[j * TL + cur * (K + 1) + n] = threshold * 256 - 1
+    const uint32_t lb = ens_logical_block();
+    const int qi = block_quad[lb];
+    if (qi == NO_QUAD) return;
+    const EnsQuad Q = quads[qi];
+    if (threadIdx.x < 4 * TL) {
+        const int j = threadIdx.x / TL, e = threadIdx.x - j * TL;
+        const int rj = quads[qi].rep[j];  // runtime index: read from memory, not the register copy
+        T[threadIdx.x] = rj >= 0 ? (tables[beta_slot[rj] * TL + e] << 8) - 1u : 0u;
+    }
+    __syncthreads();
+    const uint32_t uc = (uint32_t)c, uo = 1u - uc;
+    const uint32_t kmix = sweep_mix((*sweep_base + sweep_off) * GOLDEN);
+    const uint32_t half = Q.nsites >> 1;
+    // bytes of the word that hold a replica (unused bytes keep their value)
+    uint32_t used = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) used |= Q.rep[j] >= 0 ? 0xffu << (8 * j) : 0u;
+    // the word arrays this pass reads, as uniform bases
+    const uint32_t *Wc[4], *Wo[4];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        Wc[a] = words + arr(Q, half, a, uc);
+        Wo[a] = words + arr(Q, half, a, uo);
+    }
+    uint32_t *own = words + arr(Q, half, MU, uc);
+    const char *Tb = reinterpret_cast<const char *>(T);
+    const uint32_t base = (lb - Q.block0) * blockDim.x * (uint32_t)per_lane + threadIdx.x;
+    for (int k = 0; k < per_lane; ++k) {
+        const uint32_t item = base + (uint32_t)k * blockDim.x;
+        if (item >= half) break;
+        const LinkSite s = site_of<D>(Q, item, uc);
+        const EnsDeltas dl = ens_deltas<D>(Q, s);
+        const uint32_t ixp = item + dl.p[MU];
+        uint32_t neg = 0;
+#pragma unroll
+        for (int nu = 0; nu < D; ++nu) {
+            if (nu == MU) continue;
+            const uint32_t ixpn = item + dl.p[nu];
+            const uint32_t imn = item + dl.m[nu];
+            const uint32_t imnp = imn + dl.p[MU];
+            // forward U_NU(x+MU) U_MU(x+NU) U_NU(x), backward U_MU(x-NU) U_NU(x-NU+MU) U_NU(x-NU)
+            const uint32_t fw = ld32(Wo[nu], ixp) ^ ld32(Wo[MU], ixpn) ^ ld32(Wc[nu], item);
+            const uint32_t bw = ld32(Wo[MU], imn) ^ ld32(Wc[nu], imnp) ^ ld32(Wo[nu], imn);
+            neg += fw + bw;
+        }
+        const uint32_t wcur = ld32(own, item);
+        const uint32_t link = (s.r * (uint32_t)Q.L[0] + s.x0) * (uint32_t)D + (uint32_t)MU;
+        const uint32_t cls4 = wcur * (uint32_t)(K + 1) + neg;  // per byte: cur * (K + 1) + n <= 2K + 1
+        uint32_t bits = 0;  // rejections, replica 3 first, so that replica j lands in bit j
+#pragma unroll
+        for (int j = 3; j >= 0; --j) {
+            const uint32_t x = Q.seed32[j] ^ link;
+            const uint32_t y = lowbias32_tail(lowbias32_mid(x ^ (x >> 16)) ^ kmix);
+            const uint32_t t2 =
+                *reinterpret_cast<const uint32_t *>(Tb + 4 * (j * TL) + (((cls4 >> (8 * j)) & 0xffu) << 2));
+            asm("v_cmp_gt_u32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc"
+                : "+v"(bits)
+                : "v"(y), "v"(t2)
+                : "vcc");
+        }
+        const uint32_t raw = bits * 0x00204081u;  // bit j -> bit 8 j
+        uint32_t wnew = MODE == 0 ? (wcur ^ (~raw & 0x01010101u)) : (raw & 0x01010101u);
+        wnew = (wnew & used) | (wcur & ~used);
+        if (wnew != wcur) own[item] = wnew;
+    }
+}
+
 // Signed plaquette sum of every replica into sums[r] (zeroed by the caller):
 // per site and plane (A < B), U_A(x) U_B(x+A) U_A(x+B) U_B(x)
 // (lattice.rs:420-430), negative when the XOR of the four bits is 1.
@@ -342,9 +453,15 @@ void launch_ens_unpack(uint8_t *lat, const uint32_t *words, const EnsQuad &Q, in
 template <int D, int MODE>
 static void ens_pass_mu(const EnsPassArgs &a) {
     dim3 grid(a.blocks), block(a.block_size);
-#define PLQ_ENS_MU(M)                                                                                                \
-    ens_link_pass<D, MODE, M><<<grid, block, 0, a.stream>>>(a.words, a.quads, a.block_quad, a.tables, a.beta_slot, \
-                                                            a.sweep_base, a.sweep_off, a.c, a.per_lane)
+#define PLQ_ENS_MU(M)                                                                                             \
+    if (a.fast)                                                                                                   \
+        ens_link_pass2<D, MODE, M><<<grid, block, 0, a.stream>>>(a.words, a.quads, a.block_quad, a.tables,       \
+                                                                 a.beta_slot, a.sweep_base, a.sweep_off, a.c,     \
+                                                                 a.per_lane);                                     \
+    else                                                                                                          \
+        ens_link_pass<D, MODE, M><<<grid, block, 0, a.stream>>>(a.words, a.quads, a.block_quad, a.tables,        \
+                                                                a.beta_slot, a.sweep_base, a.sweep_off, a.c,      \
+                                                                a.per_lane)
     switch (a.mu) {
         case 0: PLQ_ENS_MU(0); break;
         case 1: PLQ_ENS_MU(1); break;

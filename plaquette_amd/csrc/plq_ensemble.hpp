@@ -55,6 +55,7 @@ struct EnsPassArgs {
     uint32_t sweep_off;
     int mu, c;
     int mode;                    // 0 Metropolis, 1 heat bath
+    int fast;                    // every threshold >= 1: ens_link_pass2
     uint32_t blocks;
     int block_size;
     int per_lane;

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -85,6 +86,7 @@ struct plq_ensemble {
     int exchange = 0;
     uint32_t ex_seed32 = 0;
     int block_size = 256, per_lane = 1, use_graph = 1;
+    int fast = 0;  // ens_link_pass2 (every threshold >= 1)
     uint32_t total_blocks = 0;
     hipGraphExec_t graph_exec = nullptr;
     uint32_t graph_sweeps = 0;
@@ -151,6 +153,7 @@ void enqueue_block(plq_ensemble *e, uint32_t nsweeps, bool measure) {
     a.beta_slot = e->d_slot_of;
     a.sweep_base = e->d_counters;
     a.mode = e->kernel == PLQ_KERNEL_METROPOLIS ? 0 : 1;
+    a.fast = e->fast;
     a.blocks = e->total_blocks;
     a.block_size = e->block_size;
     a.per_lane = e->per_lane;
@@ -346,6 +349,14 @@ int plq_ensemble_create(const plq_ensemble_desc *d, const uint8_t *start, size_t
         if (!plq::build_table(tp, t, err)) return bail(plq::host_fail(PLQ_EINVAL, err));
         tables.insert(tables.end(), t.begin(), t.end());
     }
+    // The index-delta pass compares against threshold * 256 - 1, which needs
+    // every threshold in [1, 2^24]; PLQ_ENS_FAST=0 keeps the first pass (tests
+    // compare the two).
+    e->fast = 1;
+    for (uint32_t t : tables)
+        if (t == 0 || t > plq::DRAW_ONE) e->fast = 0;
+    if (const char *f = getenv("PLQ_ENS_FAST"))
+        if (atoi(f) == 0) e->fast = 0;
     // exchange groups: replicas of one shape, ordered by beta
     std::vector<uint32_t> rep_at;
     std::vector<uint32_t> ex_tables;

@@ -139,3 +139,34 @@ def test_device_exchange_samples_the_exact_laws(gpu, betas):
     # pair k is attempted in the rounds t < T with t = k (mod 2)
     assert list(att[:R - 1]) == [(T - k % 2 + 1) // 2 for k in range(R - 1)]
     assert np.all(acc[:R - 1] <= att[:R - 1])
+
+
+@pytest.mark.parametrize("kind", ["full", "mixed4d", "uneven3d"])
+@pytest.mark.parametrize("kernel", [O.KERNEL_HEAT_BATH, O.KERNEL_METROPOLIS])
+def test_index_delta_pass_equals_the_coordinate_pass(gpu, monkeypatch, kind, kernel):
+    # The index-delta link pass (default where every threshold is at least
+    # 1) and the coordinate pass (PLQ_ENS_FAST=0) are independent
+    # implementations: states, records, betas and exchange counts agree.
+    if kind == "full":
+        dim, shapes, betas, seeds = 4, [], [], []
+        for L in (8, 12, 16, 20):
+            for k in range(12):
+                shapes.append([L] * 4)
+                betas.append(0.40 + 0.08 * k / 11)
+                seeds.append(1000 * L + k)
+    else:
+        dim, shapes, betas = make_case(kind)
+        seeds = [300 + r for r in range(len(shapes))]
+    sizes = [int(np.prod(s)) * dim for s in shapes]
+    start = np.random.default_rng(17).integers(0, 2, sum(sizes)).astype(np.uint8)
+    outs = []
+    for fast in ("1", "0"):
+        monkeypatch.setenv("PLQ_ENS_FAST", fast)
+        e = gpu.Ensemble(dim, shapes, betas, seeds, start=start, kernel=kernel, measure_every=5, exchange=True,
+                         exchange_seed=3)
+        e.advance(23)
+        outs.append((e.read(), e.betas(), e.series()[0], e.exchange_counts()[0]))
+        e.close()
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert np.array_equal(a, b)
