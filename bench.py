@@ -664,25 +664,28 @@ def learned_small(P, ev, stats):
     return out
 
 
-def bench_learned(P, ev):
+def bench_learned(P, ev, steps=1000000, train=20000, with_small=True):
     """2D Ising 512^2 at beta_c: the learned block proposal against the plain
     checkerboard.  A chain from a hot start runs 2,000 sweeps, then the
-    network trains 20,000 Adam steps, one per configuration taken every 5
-    sweeps (20,000 configurations of the plain sampler), and the logit scale
-    is calibrated on block acceptance.  From the trained chain's
-    configuration a plain chain and a mixed chain each run 1,000,000 steps
-    with E and |M| recorded every 10 steps (compare_chains)."""
+    network trains `train` (20,000) Adam steps, one per configuration taken
+    every 5 sweeps (20,000 configurations of the plain sampler), and the
+    logit scale is calibrated on block acceptance.  From the trained chain's
+    configuration a plain chain and a mixed chain each run `steps`
+    (1,000,000) steps with E and |M| recorded every 10 steps
+    (compare_chains).  Then the same comparison at 64^2 (learned_small)
+    unless with_small is false.  Smaller counts, or no 64^2 part, only
+    shorten a profiled run."""
     import numpy as np
 
     from plaquette_amd import statistics as stats
 
-    L, beta, steps = 512, 0.4406868, 1000000
+    L, beta = 512, 0.4406868
     n = L * L
     start = np.random.default_rng(SEED).integers(0, 2, n).astype(np.uint8)
     c = P.ising_gpu_chain([L, L], 1.0, 0.0, beta, SEED, start)
     c.advance(2000)
     net = P.BlockNet(SEED)
-    wall_train, ms_train = timed(P, ev, c.stream, lambda: net.train(c, 20000, 5, 3e-3))
+    wall_train, ms_train = timed(P, ev, c.stream, lambda: net.train(c, train, 5, 3e-3))
     losses = net.train(c, 1, 5, 3e-3)
     cal = calibrate_sharpness(P, c, net)
     st = c.read()
@@ -695,7 +698,7 @@ def bench_learned(P, ev):
     c.close()
     runs = compare_chains(P, ev, stats, L, beta, st, net, steps, 10, SEED + 1)
     net.close()
-    small = learned_small(P, ev, stats)
+    small = learned_small(P, ev, stats) if with_small else None
     flop = (n // 64) * BLOCK_FLOP
     tflops = flop / (block_us * 1e-6) / 1e12
     mixed = runs["mixed"]
@@ -706,10 +709,10 @@ def bench_learned(P, ev):
         "scaling": "weak", "vs_baseline": None, "dtype": "bf16 network, fp32 accumulate; u8 states",
         "data": "synthetic (chain configurations)",
         "config": {"workload": "2D Ising 512^2 at beta_c = 0.4406868, 4,096 blocks of 8x8, 3-layer 3x3 CNN with "
-                               "32 channels (middle layer dilated by 2), trained on 20,000 configurations",
+                               f"32 channels (middle layer dilated by 2), trained on {train:,} configurations",
                    "lattice": [L, L]},
         "ratio_vs_plain": runs["ratio_vs_plain_abs_m"], "comparison_512": runs, "comparison_64": small,
-        "training": {"steps": 20000, "wall_s": round(wall_train, 3), "final_loss": float(losses[-1]),
+        "training": {"steps": train, "wall_s": round(wall_train, 3), "final_loss": float(losses[-1]),
                      "calibration": cal},
         "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(tflops / 2500.0, 5), "traffic": None,
@@ -731,6 +734,12 @@ def main():
     ap.add_argument("--kernel-sweeps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--learned-steps", type=int, default=1000000,
+                    help="steps of each compared chain in the learned workload")
+    ap.add_argument("--learned-train", type=int, default=20000,
+                    help="training configurations in the learned workload")
+    ap.add_argument("--learned-no-64", action="store_true",
+                    help="leave out the 64^2 comparison of the learned workload")
     args = ap.parse_args()
     if args.profile:
         profile_passes(10, 256)
@@ -758,7 +767,7 @@ def main():
         print(json.dumps(bench_cpu_stream(P, ev, 50, 5)))
         return
     if args.workload == "learned":
-        print(json.dumps(bench_learned(P, ev)))
+        print(json.dumps(bench_learned(P, ev, args.learned_steps, args.learned_train, not args.learned_no_64)))
         return
     if args.workload == "ising2d-64":
         print(json.dumps(bench_ising2d_64(P, ev, 10000, 1000, None if args.no_cpu else args.cpu_seconds)))
