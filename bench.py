@@ -519,6 +519,49 @@ def bench_cluster_small(P, ev, steps, warmup):
     }
 
 
+def bench_wolff(P, ev, steps, warmup):
+    """Wolff moves on a 2D Ising 4096^2 lattice (gap 2J) at beta 0.3, 0.4,
+    beta_c and 0.6, each from a configuration equilibrated by 300
+    Swendsen-Wang sweeps: the reference's device move (the full
+    decomposition, PLQ_WOLFF_GROW=0) against the default adaptive growth of
+    the seed's cluster (PLQ_WOLFF_GROW=1), the same chain either way.  One
+    step is one Wolff move."""
+    import numpy as np
+
+    L = 4096
+    n = L * L
+    out = {}
+    for beta in (0.3, 0.4, 0.4406868, 0.6):
+        start = np.random.default_rng(SEED).integers(0, 2, n).astype(np.uint8)
+        sw = P.GpuClusterChain.swendsen_wang([L, L], 2, 2.0, beta, SEED, start)
+        sw.advance(300)
+        st = sw.read()
+        sw.close()
+        row = {}
+        for name, mode in (("decomposition", "0"), ("adaptive_growth", "1")):
+            os.environ["PLQ_WOLFF_GROW"] = mode
+            c = P.GpuClusterChain.wolff([L, L], 2, 2.0, beta, SEED + 1, st)
+            c.advance(warmup)
+            c.sync()
+            g0, f0 = c.wolff_counts()
+            wall, _ = timed(P, ev, c.stream, lambda: (c.advance(steps), c.sync()))
+            g1, f1 = c.wolff_counts()
+            c.close()
+            row[name] = {"us_per_move": round(wall * 1e6 / steps, 2), "grown": g1 - g0, "decomposed": f1 - f0}
+        os.environ.pop("PLQ_WOLFF_GROW", None)
+        row["speedup"] = round(row["decomposition"]["us_per_move"] / row["adaptive_growth"]["us_per_move"], 3)
+        out[f"beta_{beta}"] = row
+    head = out["beta_0.4"]["adaptive_growth"]
+    return {
+        "metric": "Wolff moves/s, 2D Ising 4096^2 at beta 0.4, adaptive growth", "value": round(1e6 / head["us_per_move"], 1),
+        "unit": "moves/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": round(head["us_per_move"] / 1e3, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8", "data": "synthetic (Swendsen-Wang equilibrated configurations)",
+        "config": {"workload": "2D Ising 4096^2 Wolff moves, gap 2J, beta 0.3 / 0.4 / beta_c / 0.6", "lattice": [L, L]},
+        "by_beta": out,
+    }
+
+
 def bench_measure(P, ev, reps):
     """The device measurements: the Ising correlator of a 256^3 chain (387
     axis-displacement pair sums) and the Wilson table of a 4D Z2 chain at
@@ -726,7 +769,7 @@ def main():
                     help="run the fixed counter-collection workload instead of the benchmark")
     ap.add_argument("--workload", default="ising3d",
                     choices=["ising3d", "potts-sw", "z2-batch", "ising2d-64", "learned", "cpu-stream", "cluster-small",
-                             "measure"],
+                             "measure", "wolff"],
                     help="ising3d is the headline line; the others time the remaining configurations")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
@@ -771,6 +814,9 @@ def main():
         return
     if args.workload == "ising2d-64":
         print(json.dumps(bench_ising2d_64(P, ev, 10000, 1000, None if args.no_cpu else args.cpu_seconds)))
+        return
+    if args.workload == "wolff":
+        print(json.dumps(bench_wolff(P, ev, 300, 20)))
         return
     if args.workload == "cluster-small":
         print(json.dumps(bench_cluster_small(P, ev, 10000, 500)))

@@ -177,6 +177,14 @@ int plq_chain_cluster_series(plq_chain *chain, plq_cluster_stats *out, uint64_t 
 /* Cluster chains only: lower the labelling pass cap (gpu_cluster.rs:620-625);
  * used to force PLQ_ENOCONV from the host. */
 int plq_chain_set_pass_cap(plq_chain *chain, uint64_t cap);
+/* Wolff chains only: how many sweeps grew the seed's cluster on the device
+ * (the CPU's grow_cluster, cluster.rs:88-111, over the sweep's keyed bonds)
+ * and how many ran the full decomposition of the reference's device move
+ * (cluster_relabel.wgsl:55-66) after a growth gave up or was skipped; both
+ * zero while growth is off.  No counterpart in the reference: the states do
+ * not depend on the path, so this only reports the cost split.  Waits for the
+ * chain's stream. */
+int plq_chain_wolff_counts(plq_chain *chain, uint64_t *grown, uint64_t *full);
 int plq_chain_destroy(plq_chain *chain);
 /* PLQ_PROFILE_CPU_STREAM chains only: the stream state as four u64
  * (state lo, state hi, increment lo, increment hi) -- the two u128 fields of

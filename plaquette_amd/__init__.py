@@ -180,6 +180,7 @@ SIGNATURES = {
     "plq_chain_cluster_roots": (C.c_int, [_HANDLE, _P(C.c_int64), C.c_size_t]),
     "plq_chain_cluster_stats": (C.c_int, [_HANDLE, _P(ClusterStats)]),
     "plq_chain_set_pass_cap": (C.c_int, [_HANDLE, C.c_uint64]),
+    "plq_chain_wolff_counts": (C.c_int, [_HANDLE, _P(C.c_uint64), _P(C.c_uint64)]),
     "plq_chain_record_clusters": (C.c_int, [_HANDLE, C.c_int]),
     "plq_chain_cluster_series": (C.c_int, [_HANDLE, C.c_void_p, C.c_uint64, _P(C.c_uint64)]),
     "plq_chain_destroy": (C.c_int, [_HANDLE]),
@@ -575,6 +576,13 @@ class GpuClusterChain(_Chain):
 
     def set_pass_cap(self, cap: int) -> None:
         _check(lib().plq_chain_set_pass_cap(self._h, int(cap)))
+
+    def wolff_counts(self) -> tuple:
+        """(sweeps that grew the seed's cluster, sweeps that ran the full
+        decomposition) of a Wolff chain; (0, 0) while growth is off."""
+        grown, full = C.c_uint64(), C.c_uint64()
+        _check(lib().plq_chain_wolff_counts(self._h, C.byref(grown), C.byref(full)))
+        return int(grown.value), int(full.value)
 
     def record_clusters(self, on: bool = True) -> None:
         """Keep each recorded sweep's cluster statistics on the device (with

@@ -33,6 +33,7 @@ SOURCES = [
     "plq_block_abi.hip",
     "plq_measure.hip",
     "plq_measure_abi.hip",
+    "plq_wolff.hip",
     "plq_thresholds.cpp",
 ]
 

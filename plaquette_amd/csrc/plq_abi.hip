@@ -419,8 +419,36 @@ int enqueue_cluster_sweep(plq_chain *c) {
     }
     a.max_blocks = plq::MAX_PASS_BLOCKS;
     a.stream = c->stream;
+    // A Wolff move may grow the seed's cluster first; the decomposition then
+    // runs only where the growth gave up (gate = 1).  Runs with per-sweep
+    // cluster statistics or a lowered hop budget always decompose.
+    const bool grow = c->wolff_mode && !a.count_sizes && !c->rec_clusters && c->pass_cap >= (uint64_t)c->n_sites + 64;
+    if (grow) {
+        plq::WolffGrowArgs g{};
+        g.labels = c->labels;
+        g.cg = c->cg;
+        g.seed32 = c->seed32;
+        g.sweep = a.sweep;
+        g.bond_t = c->bond_t;
+        g.q = c->q;
+        g.forced = c->forced;
+        g.queue = c->d_queue;
+        g.qcap = c->wolff_qcap;
+        g.level_cap = c->wolff_levels;
+        g.adaptive = c->wolff_mode == 1;
+        g.gain_ticks = c->wolff_gain_ticks;
+        g.state = c->d_wolff;
+        g.partials = a.partials;
+        g.nparts = plq::cluster_relabel_blocks(a);
+        g.stream = c->stream;
+        plq::launch_wolff_grow(g);
+        a.gate = &c->d_wolff->gate;
+        a.gate_skip = 0;
+    }
     c->last_relabel_blocks = plq::launch_cluster_sweep(a);
     c->have_partition = true;
+    c->grow_pending = grow;
+    c->grow_sweep = a.sweep;
     c->sweeps_done += 1;
     bool record = c->rec_every && (c->sweeps_done % c->rec_every == 0);
     if (record) {
@@ -432,6 +460,40 @@ int enqueue_cluster_sweep(plq_chain *c) {
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(PLQ_EDEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
+    return PLQ_OK;
+}
+
+// The partition of the last sweep when that sweep may have grown its
+// cluster instead of decomposing the lattice: the labels it started from
+// (the result with the grown cluster's old label restored) decomposed again
+// with the sweep's keys into parent, all gated to run only where the growth
+// succeeded (gate = 0).  Enqueue only.
+int rebuild_grown_partition(plq_chain *c) {
+    if (!c->grow_pending) return PLQ_OK;
+    PLQ_HIP(hipMemcpyAsync(c->d_lat, c->labels, (size_t)c->n_sites, hipMemcpyDeviceToDevice, c->stream));
+    plq::launch_wolff_restore(c->d_lat, c->d_queue, c->d_wolff, c->stream);
+    plq::ClusterSweepArgs a{};
+    a.labels = c->d_lat;
+    a.parent = c->parent;
+    a.cg = c->cg;
+    a.seed32 = c->seed32;
+    a.sweep = c->grow_sweep;
+    a.bond_t = c->bond_t;
+    a.q = c->q;
+    a.seeded = c->seeded;
+    a.forced = c->forced;
+    a.cap = (uint32_t)(c->pass_cap > 0xffffffffull ? 0xffffffffull : c->pass_cap);
+    a.flag = c->d_flag;
+    a.cross = c->d_cross;
+    a.max_blocks = plq::MAX_PASS_BLOCKS;
+    a.gate = &c->d_wolff->gate;
+    a.gate_skip = 1;
+    a.label_only = true;
+    a.stream = c->stream;
+    (void)plq::launch_cluster_sweep(a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(PLQ_EDEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
+    c->grow_pending = false;
     return PLQ_OK;
 }
 
@@ -454,6 +516,7 @@ int write_lattice(plq_chain *c, const uint8_t *in, size_t n) {
     else if (c->kind == KIND_LINK) plq::launch_pack_links(c->d_lat, c->d_links, c->g, c->stream);
     PLQ_HIP(hipStreamSynchronize(c->stream));
     c->have_partition = false;
+    c->grow_pending = false;
     return PLQ_OK;
 }
 
@@ -695,6 +758,30 @@ int plq_chain_create_cluster(const plq_cluster_desc *d, const uint8_t *start, si
     c->small = ns <= plq::small_cluster_max_sites(d->dim);
     if (const char *sm = getenv("PLQ_SMALL_CLUSTER"))
         if (atoi(sm) == 0) c->small = false;
+    // Wolff moves grown from the seed (plq_wolff.hip): adaptive by default
+    // on lattices of 2^22 sites or more, where a decomposition costs several
+    // times the fixed cost of a grown move; PLQ_WOLFF_GROW=0/1/2 (off,
+    // adaptive, every sweep), PLQ_WOLFF_LEVELS and PLQ_WOLFF_QUEUE set the
+    // mode and the budgets (tests drive both paths).
+    if (c->seeded && !c->small) {
+        c->wolff_mode = ns >= (int64_t(1) << 22) ? 1 : 0;
+        if (const char *g = getenv("PLQ_WOLFF_GROW")) c->wolff_mode = std::max(0, std::min(2, atoi(g)));
+        // Measured on MI355X (2D Ising, 1024^2 to 4096^2): a Wolff sweep by
+        // the decomposition costs about 25 us + 7.5 ps per site, a level of
+        // growth about 1.2 us, and a grown sweep's gated launches about
+        // 10 us.  The level budget keeps a failed attempt near half a
+        // decomposition.
+        const double full_us = 25.0 + 7.5e-6 * (double)ns;
+        c->wolff_levels = (uint32_t)std::max(8.0, std::min(128.0, 0.5 * full_us / 1.2));
+        int rate_khz = 0;
+        if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || rate_khz <= 0)
+            rate_khz = 100000;
+        c->wolff_gain_ticks = (float)((full_us - 10.0) * 1e-3 * (double)rate_khz);
+        c->wolff_qcap = (uint32_t)std::min<int64_t>(ns, plq::WOLFF_QUEUE);
+        if (const char *v = getenv("PLQ_WOLFF_LEVELS")) c->wolff_levels = (uint32_t)std::max(1, atoi(v));
+        if (const char *v = getenv("PLQ_WOLFF_QUEUE"))
+            c->wolff_qcap = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(c->wolff_qcap, atoll(v)));
+    }
     int rc = PLQ_OK;
     auto bail = [&](int code) {
         delete c;
@@ -703,6 +790,16 @@ int plq_chain_create_cluster(const plq_cluster_desc *d, const uint8_t *start, si
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(PLQ_EDEVICE, "hipStreamCreate failed"));
     if ((rc = alloc_scratch(c)) != PLQ_OK) return bail(rc);
+    if (c->wolff_mode) {
+        if ((rc = alloc((void **)&c->d_wolff, sizeof(plq::WolffState))) != PLQ_OK ||
+            (rc = alloc((void **)&c->d_queue, sizeof(int32_t) * (size_t)c->wolff_qcap)) != PLQ_OK)
+            return bail(rc);
+        plq::WolffState st0{};
+        st0.gate = 1;
+        st0.success = 0.5f;
+        if (hipMemcpy(c->d_wolff, &st0, sizeof st0, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(PLQ_EDEVICE, "Wolff state initialisation failed"));
+    }
     if ((rc = alloc((void **)&c->labels, (size_t)ns)) != PLQ_OK) return bail(rc);
     if ((rc = alloc((void **)&c->parent, (size_t)ns * sizeof(int32_t))) != PLQ_OK) return bail(rc);
     if ((rc = alloc((void **)&c->d_flag, sizeof(int))) != PLQ_OK) return bail(rc);
@@ -1008,6 +1105,22 @@ int plq_chain_set_pass_cap(plq_chain *c, uint64_t cap) {
     return PLQ_OK;
 }
 
+int plq_chain_wolff_counts(plq_chain *c, uint64_t *grown, uint64_t *full) {
+    if (!c || !grown || !full) return fail(PLQ_EINVAL, "chain or output is null");
+    if (c->kind != KIND_CLUSTER || !c->seeded) return fail(PLQ_EINVAL, "not a Wolff chain");
+    std::lock_guard<std::mutex> lk(c->mu);
+    *grown = 0;
+    *full = 0;
+    if (!c->d_wolff) return PLQ_OK;
+    PLQ_HIP(hipSetDevice(c->device));
+    plq::WolffState st{};
+    PLQ_HIP(hipMemcpyAsync(&st, c->d_wolff, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    PLQ_HIP(hipStreamSynchronize(c->stream));
+    *grown = st.grown;
+    *full = st.full;
+    return check_flag(c);
+}
+
 int plq_chain_cluster_roots(plq_chain *c, int64_t *roots, size_t n) {
     if (!c || !roots) return fail(PLQ_EINVAL, "chain or output is null");
     if (c->kind != KIND_CLUSTER) return fail(PLQ_EINVAL, "not a cluster chain");
@@ -1016,6 +1129,7 @@ int plq_chain_cluster_roots(plq_chain *c, int64_t *roots, size_t n) {
     std::lock_guard<std::mutex> lk(c->mu);
     PLQ_HIP(hipSetDevice(c->device));
     if (!c->roots) PLQ_TRY(alloc((void **)&c->roots, sizeof(int32_t) * (size_t)c->n_sites));
+    PLQ_TRY(rebuild_grown_partition(c));
     uint32_t cap = (uint32_t)(c->pass_cap > 0xffffffffull ? 0xffffffffull : c->pass_cap);
     plq::launch_cluster_resolve(c->parent, c->roots, c->cg, cap, c->d_flag, nullptr, nullptr, c->stream);
     std::vector<int32_t> host((size_t)c->n_sites);
@@ -1033,6 +1147,7 @@ int plq_chain_cluster_stats(plq_chain *c, plq_cluster_stats *out) {
     PLQ_HIP(hipSetDevice(c->device));
     if (!c->sizes) PLQ_TRY(alloc((void **)&c->sizes, sizeof(int32_t) * (size_t)c->n_sites));
     if (!c->hist) PLQ_TRY(alloc((void **)&c->hist, sizeof(unsigned long long) * CLU_SLOT));
+    PLQ_TRY(rebuild_grown_partition(c));
     PLQ_TRY(enqueue_cluster_stats(c, c->hist));
     unsigned long long h[CLU_SLOT];
     PLQ_HIP(hipMemcpyAsync(h, c->hist, sizeof h, hipMemcpyDeviceToHost, c->stream));

@@ -61,6 +61,14 @@ struct plq_chain {
     uint64_t pass_cap = 0;
     int64_t last_relabel_blocks = 0;
     bool have_partition = false;
+    // Wolff moves grown from the seed (plq_wolff.hip)
+    int wolff_mode = 0;                    // 0 off, 1 adaptive, 2 an attempt every sweep
+    uint32_t wolff_levels = 0, wolff_qcap = 0;
+    float wolff_gain_ticks = 0;            // wall-clock ticks a grown sweep saves (adaptive mode)
+    plq::WolffState *d_wolff = nullptr;
+    int32_t *d_queue = nullptr;            // wolff_qcap sites of the grown cluster
+    bool grow_pending = false;             // the last sweep may have grown: its partition is built on demand
+    uint32_t grow_sweep = 0;
     // shared scratch
     uint8_t *d_lat = nullptr;              // lattice-order staging
     long long *d_partials = nullptr;       // 2 * MAX_PASS_BLOCKS * REC_K
@@ -100,7 +108,8 @@ struct plq_chain {
                        (void *)d_partials, (void *)d_meas, (void *)d_rec, (void *)d_block_counters, (void *)d_clu, (void *)d_cross,
                        (void *)d_hist_part,
                        (void *)d_block_logits, (void *)d_pm_cls, (void *)d_pm_state, (void *)d_pm_jt, (void *)d_pm_blk,
-                       (void *)d_pm_off, (void *)d_meas_bits, (void *)d_meas_counts, (void *)d_batch})
+                       (void *)d_pm_off, (void *)d_meas_bits, (void *)d_meas_counts, (void *)d_batch,
+                       (void *)d_wolff, (void *)d_queue})
             if (p) (void)hipFree(p);
         for (int k = 0; k < 2 * plq::MAX_D; ++k)
             if (links[k]) (void)hipFree(links[k]);

@@ -46,6 +46,10 @@ __device__ __forceinline__ void store_relaxed(int32_t *p, int32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// A Wolff growth (plq_wolff.hip) gates the kernels of its sweep: each returns
+// at once when *gate == skip (a null gate never skips).
+__device__ __forceinline__ bool gated(const int *gate, int skip) { return gate != nullptr && *gate == skip; }
+
 // ---------------------------------------------------------------- local ---
 
 __device__ __forceinline__ int32_t lds_load(const int32_t *p) {
@@ -93,13 +97,14 @@ __device__ __forceinline__ void lds_union(int32_t *P, int32_t a, int32_t b) {
 
 // n / d for n < 2^24 through a 64-bit reciprocal: q = (n * M) >> 40 with
 // M = ceil(2^40 / d) is exact while n * d < 2^40.
-__device__ __forceinline__ uint32_t cdiv(uint32_t n, uint64_t M) { return (uint32_t)(((uint64_t)n * M) >> 40); }
 
 // POW2: every box has the same power-of-two extents, so a local index splits
 // into box coordinates by shifts and masks instead of reciprocal divisions.
 template <int D, bool POW2>
 __global__ __launch_bounds__(1024) void cc_local(const uint8_t *__restrict__ labels, int32_t *__restrict__ parent,
-                                                 CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t) {
+                                                 CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t,
+                                                 const int *__restrict__ gate, int gate_skip) {
+    if (gated(gate, gate_skip)) return;
     extern __shared__ int32_t P[];
     uint32_t o[MAX_D], e[MAX_D], ls[MAX_D];
     uint64_t me[MAX_D];
@@ -404,7 +409,9 @@ __device__ __forceinline__ void x16_label(int32_t *P, uint16_t *H, int32_t *__re
 // 16: x16_label with the labels read from global memory.
 template <int D>
 __global__ __launch_bounds__(1024) void cc_local_x16(const uint8_t *__restrict__ labels, int32_t *__restrict__ parent,
-                                                     CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t) {
+                                                     CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t,
+                                                     const int *__restrict__ gate, int gate_skip) {
+    if (gated(gate, gate_skip)) return;
     extern __shared__ int4 box_lds[];  // its own name: extern LDS arrays of one name share one alignment
     int32_t *P = reinterpret_cast<int32_t *>(box_lds);
     __shared__ uint16_t H[1024];
@@ -491,7 +498,9 @@ constexpr int BND_SLOTS = 512;  // LDS pair set of a 256-lane boundary workgroup
 template <int D>
 __global__ __launch_bounds__(256) void cc_boundary(const uint8_t *__restrict__ labels, int32_t *__restrict__ parent,
                                                    CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t,
-                                                   uint32_t cap, int *__restrict__ flag) {
+                                                   uint32_t cap, int *__restrict__ flag, const int *__restrict__ gate,
+                                                   int gate_skip) {
+    if (gated(gate, gate_skip)) return;
     // The workgroup's distinct (parent, parent) pairs: face sites along a
     // face mostly pair the same two box-local roots, and a bond whose ends
     // have the same parents as another open bond joins the same two sets.
@@ -550,7 +559,8 @@ template <int D>
 __global__ __launch_bounds__(256) void cc_compress(const uint8_t *__restrict__ labels, int32_t *__restrict__ parent,
                                                    CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t,
                                                    uint32_t cap, int *__restrict__ flag, uint8_t *__restrict__ cross,
-                                                   uint8_t tag) {
+                                                   uint8_t tag, const int *__restrict__ gate, int gate_skip) {
+    if (gated(gate, gate_skip)) return;
     int64_t gsite = 0, gp = 0;
     if (!face_bond<D>(labels, cg, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, bond_seed, skey, bond_t, gsite,
                       gp))
@@ -635,22 +645,12 @@ __device__ __forceinline__ int32_t seed_cluster_root(const int32_t *__restrict__
     return r;
 }
 
-__device__ __forceinline__ uint32_t relabel_draw(uint32_t u_bits, uint32_t lab, int q, int forced) {
-    const float u = keyed_uniform_f32(u_bits);
-    if (forced) {
-        uint32_t step = (uint32_t)(u * (float)(uint32_t)(q - 1));
-        if (step >= (uint32_t)(q - 1)) step = (uint32_t)(q - 2);
-        return (lab + 1u + step) % (uint32_t)q;
-    }
-    uint32_t draw = (uint32_t)(u * (float)(uint32_t)q);
-    return draw >= (uint32_t)q ? (uint32_t)(q - 1) : draw;
-}
-
 __global__ __launch_bounds__(256) void cc_relabel(uint8_t *__restrict__ labels, const int32_t *__restrict__ parent,
                                                   int64_t N, uint32_t root_seed, uint32_t seed_seed, uint32_t skey,
                                                   uint32_t sweep, int q, int seeded, int forced, uint32_t cap,
                                                   int *__restrict__ flag, long long *__restrict__ partials,
-                                                  int64_t span) {
+                                                  int64_t span, const int *__restrict__ gate, int gate_skip) {
+    if (gated(gate, gate_skip)) return;
     __shared__ int32_t seed_root;
     if (seeded) {
         if (threadIdx.x == 0) seed_root = seed_cluster_root(parent, N, seed_seed, sweep, cap, flag);
@@ -861,7 +861,9 @@ __global__ __launch_bounds__(1024, 8) void cc_relabel_box(uint8_t *__restrict__ 
                                                        const int32_t *__restrict__ parent, CCGeom cg,
                                                        uint32_t root_seed, uint32_t seed_seed, uint32_t skey,
                                                        uint32_t sweep, int q, int seeded, int forced, uint32_t cap,
-                                                       int *__restrict__ flag, long long *__restrict__ partials) {
+                                                       int *__restrict__ flag, long long *__restrict__ partials,
+                                                       const int *__restrict__ gate, int gate_skip) {
+    if (gated(gate, gate_skip)) return;
     extern __shared__ int4 box_lds[];  // its own name: extern LDS arrays of one name share one alignment
     int32_t *P = reinterpret_cast<int32_t *>(box_lds);
     __shared__ int32_t seed_root;
@@ -1574,16 +1576,19 @@ int64_t launch_cluster_sweep(const ClusterSweepArgs &a) {
     const bool x16 = cluster_boxes_x16(cg, a.labels);
 #define PLQ_CC(DD)                                                                                          \
     if (x16)                                                                                                \
-        cc_local_x16<DD><<<gl, bl, lds, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t);     \
+        cc_local_x16<DD><<<gl, bl, lds, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t,      \
+                                                    a.gate, a.gate_skip);                                    \
     else if (pow2)                                                                                          \
-        cc_local<DD, true><<<gl, bl, lds, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t);   \
+        cc_local<DD, true><<<gl, bl, lds, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t,    \
+                                                      a.gate, a.gate_skip);                                  \
     else                                                                                                    \
-        cc_local<DD, false><<<gl, bl, lds, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t);  \
+        cc_local<DD, false><<<gl, bl, lds, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t,   \
+                                                       a.gate, a.gate_skip);                                 \
     if (faces > 0) {                                                                                        \
         cc_boundary<DD><<<gb, bb, 0, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t, a.cap, \
-                                                 a.flag);                                                   \
+                                                 a.flag, a.gate, a.gate_skip);                              \
         cc_compress<DD><<<gb, bb, 0, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t, a.cap, \
-                                                 a.flag, a.cross, cross_tag(a.sweep));                      \
+                                                 a.flag, a.cross, cross_tag(a.sweep), a.gate, a.gate_skip); \
     }
     switch (cg.D) {
         case 1: PLQ_CC(1); break;
@@ -1592,6 +1597,7 @@ int64_t launch_cluster_sweep(const ClusterSweepArgs &a) {
         default: PLQ_CC(4); break;
     }
 #undef PLQ_CC
+    if (a.label_only) return 0;
     int64_t need = (cg.N + 255) / 256;
     int64_t G = need < a.max_blocks ? need : a.max_blocks;
     if (x16 && a.count_sizes) {
@@ -1616,7 +1622,8 @@ int64_t launch_cluster_sweep(const ClusterSweepArgs &a) {
 #define PLQ_RB(DD)                                                                                            \
     cc_relabel_box<DD><<<dim3((uint32_t)Gb), bl, lds, a.stream>>>(a.labels, a.parent, cg, a.seed32 ^ ROOT_SALT, \
                                                                 a.seed32 ^ SEED_SALT, skey, a.sweep, a.q,     \
-                                                                a.seeded, a.forced, a.cap, a.flag, a.partials)
+                                                                a.seeded, a.forced, a.cap, a.flag, a.partials, \
+                                                                a.gate, a.gate_skip)
             case 1: PLQ_RB(1); break;
             case 2: PLQ_RB(2); break;
             case 3: PLQ_RB(3); break;
@@ -1629,8 +1636,17 @@ int64_t launch_cluster_sweep(const ClusterSweepArgs &a) {
     const int64_t span4 = nd > 0 ? (nd + G - 1) / G : 1;
     cc_relabel<<<dim3((uint32_t)G), dim3(256), 0, a.stream>>>(a.labels, a.parent, cg.N, a.seed32 ^ ROOT_SALT,
                                                               a.seed32 ^ SEED_SALT, skey, a.sweep, a.q, a.seeded,
-                                                              a.forced, a.cap, a.flag, a.partials, span4);
+                                                              a.forced, a.cap, a.flag, a.partials, span4, a.gate,
+                                                              a.gate_skip);
     return G;
+}
+
+int64_t cluster_relabel_blocks(const ClusterSweepArgs &a) {
+    const CCGeom &cg = a.cg;
+    if (a.label_only) return 0;
+    if (cluster_boxes_x16(cg, a.labels)) return cg.nboxes < a.max_blocks ? cg.nboxes : a.max_blocks;
+    const int64_t need = (cg.N + 255) / 256;
+    return need < a.max_blocks ? need : a.max_blocks;
 }
 
 void launch_potts_measure(const uint8_t *labels, const CCGeom &cg, int q, long long *partials, int64_t max_blocks,

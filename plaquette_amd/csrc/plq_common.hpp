@@ -72,6 +72,23 @@ __host__ __device__ __forceinline__ float keyed_uniform_f32(uint32_t m) {
     return (float)m * (1.0f / 16777216.0f);
 }
 
+// A cluster's new label from its keyed draw (cluster_relabel.wgsl:39-90):
+// Redraw takes floor(u q), ForcedChange adds 1 + floor(u (q - 1)) to the
+// cluster's label lab; both scaled in f32 and clamped.
+__host__ __device__ __forceinline__ uint32_t relabel_draw(uint32_t u_bits, uint32_t lab, int q, int forced) {
+    const float u = keyed_uniform_f32(u_bits);
+    if (forced) {
+        uint32_t step = (uint32_t)(u * (float)(uint32_t)(q - 1));
+        if (step >= (uint32_t)(q - 1)) step = (uint32_t)(q - 2);
+        return (lab + 1u + step) % (uint32_t)q;
+    }
+    uint32_t draw = (uint32_t)(u * (float)(uint32_t)q);
+    return draw >= (uint32_t)q ? (uint32_t)(q - 1) : draw;
+}
+
+// n / L by a multiplication, M = ceil(2^40 / L): exact while n * L < 2^40.
+__host__ __device__ __forceinline__ uint32_t cdiv(uint32_t n, uint64_t M) { return (uint32_t)(((uint64_t)n * M) >> 40); }
+
 // fold_seed (device.rs:761-763).
 __host__ __device__ __forceinline__ uint32_t fold_seed(uint64_t seed) {
     return (uint32_t)(seed ^ (seed >> 32));

@@ -237,11 +237,59 @@ struct ClusterSweepArgs {
     int *flag;            // set to 1 when a budget is exceeded
     long long *partials;  // REC_K per relabel block: [0, moved, counts...]
     int64_t max_blocks;
+    // Gating by a Wolff growth (plq_wolff.hip): with gate non-null every
+    // kernel of the sweep returns at once when *gate == gate_skip.
+    const int *gate;
+    int gate_skip;
+    bool label_only;      // build the partition only: no relabel, no partials
     hipStream_t stream;
 };
 // Enqueue one cluster sweep; returns the number of relabel blocks whose
 // partials were written.
 int64_t launch_cluster_sweep(const ClusterSweepArgs &a);
+// The number of relabel blocks launch_cluster_sweep gives the same arguments.
+int64_t cluster_relabel_blocks(const ClusterSweepArgs &a);
+
+// Wolff moves grown from the seed site (plq_wolff.hip).  One workgroup
+// grows the seed's cluster level by level over the sweep's keyed bonds and,
+// when it closes within the level and queue budgets, relabels it and sets
+// gate = 0, so the full decomposition of the same sweep (gated) is skipped;
+// otherwise gate = 1 and the full decomposition runs.
+struct WolffState {
+    int gate;                // 0: the last sweep grew its cluster; 1: the full decomposition ran
+    int since;               // sweeps since the last attempt (adaptive mode)
+    float cost;              // running mean of an attempt's duration, wall-clock ticks (adaptive mode)
+    float success;           // running mean of the attempts' success (adaptive mode)
+    int old_label;           // the grown cluster's label before the move
+    uint32_t size;           // the grown cluster's size (queue[0 .. size))
+    unsigned long long grown, full;  // sweeps by growth, by the full decomposition
+};
+struct WolffGrowArgs {
+    uint8_t *labels;
+    CCGeom cg;
+    uint32_t seed32;
+    uint32_t sweep;
+    uint32_t bond_t;
+    int q;
+    int forced;
+    int32_t *queue;          // min(qcap, WOLFF_QUEUE) entries: the grown cluster's sites
+    uint32_t qcap;
+    uint32_t level_cap;
+    int adaptive;            // attempt only where it is expected to pay, and every 32nd or 256th sweep
+    float gain_ticks;        // wall-clock ticks a grown sweep saves against the decomposition
+    WolffState *state;
+    long long *partials;     // relabel partials: nparts rows of REC_K
+    int64_t nparts;
+    hipStream_t stream;
+};
+// LDS of the growth: the cluster's sites and the set of sites reached.
+constexpr uint32_t WOLFF_SET_BITS = 13;
+constexpr uint32_t WOLFF_SET_SLOTS = 1u << WOLFF_SET_BITS;
+constexpr uint32_t WOLFF_QUEUE = 6144;
+void launch_wolff_grow(const WolffGrowArgs &a);
+// dst[queue[i]] = the grown cluster's old label for i < size, when gate == 0:
+// the labels the last grown sweep started from, given its result in dst.
+void launch_wolff_restore(uint8_t *dst, const int32_t *queue, const WolffState *state, hipStream_t s);
 void launch_potts_measure(const uint8_t *labels, const CCGeom &cg, int q, long long *partials,
                           int64_t max_blocks, long long *rec, hipStream_t s);
 // Canonical roots of the current forest, and (when sizes/hist are non-null)
