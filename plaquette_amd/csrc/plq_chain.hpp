@@ -69,6 +69,13 @@ struct plq_chain {
     int32_t *d_queue = nullptr;            // wolff_qcap sites of the grown cluster
     bool grow_pending = false;             // the last sweep may have grown: its partition is built on demand
     uint32_t grow_sweep = 0;
+    // hipGraph replays of a uniform-pass site chain's sweeps
+    int site_graph = 0;                    // replays on (PLQ_SITE_GRAPH=1; a failed capture turns them off)
+    uint32_t *d_ctr = nullptr;             // replay counters: sweep, record slot
+    bool ctr_valid = false;                // d_ctr holds sweeps_done and the next record slot
+    hipGraphExec_t site_exec = nullptr;
+    uint32_t exec_sweeps = 0;              // sweeps per replay
+    bool exec_records = false;             // a replay ends with a record
     // shared scratch
     uint8_t *d_lat = nullptr;              // lattice-order staging
     long long *d_partials = nullptr;       // 2 * MAX_PASS_BLOCKS * REC_K
@@ -102,6 +109,7 @@ struct plq_chain {
     ~plq_chain() {
         (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
+        if (site_exec) (void)hipGraphExecDestroy(site_exec);
         for (auto p : {(void *)col[0], (void *)col[1], (void *)d_links, (void *)d_table, (void *)labels,
                        (void *)parent, (void *)roots, (void *)sizes, (void *)ext, (void *)hist, (void *)d_flag,
                        (void *)d_lat,
@@ -109,7 +117,7 @@ struct plq_chain {
                        (void *)d_hist_part,
                        (void *)d_block_logits, (void *)d_pm_cls, (void *)d_pm_state, (void *)d_pm_jt, (void *)d_pm_blk,
                        (void *)d_pm_off, (void *)d_meas_bits, (void *)d_meas_counts, (void *)d_batch,
-                       (void *)d_wolff, (void *)d_queue})
+                       (void *)d_wolff, (void *)d_queue, (void *)d_ctr})
             if (p) (void)hipFree(p);
         for (int k = 0; k < 2 * plq::MAX_D; ++k)
             if (links[k]) (void)hipFree(links[k]);

@@ -61,6 +61,8 @@ struct SitePassArgs {
     int color;
     uint32_t seed;
     uint32_t skey;       // sweep * 0x9e3779b9
+    const uint32_t *ctr; // uniform pass only, non-null in a graph: sweep = ctr[0] + sweep_off
+    uint32_t sweep_off;
     int q;
     const uint32_t *table;
     MeasArgs m;
@@ -80,6 +82,13 @@ void launch_finalize_record(const long long *pa, int64_t na, const long long *pb
                             hipStream_t s);
 void launch_finalize_columns(const long long *pa, int64_t na, const long long *pb, int64_t nb, int nk, long long *rec,
                              hipStream_t s);
+// The same into slot ctr[1] of the record ring at rec_base (graph replays).
+void launch_finalize_columns_ring(const long long *pa, int64_t na, const long long *pb, int64_t nb, int nk,
+                                  long long *rec_base, const uint32_t *ctr, hipStream_t s);
+// A site chain's graph counters: ctr[0] = sweep, ctr[1] = record slot; the
+// tick advances them by one replay (sweeps, and records into a ring of cap).
+void launch_site_ctr_set(uint32_t *ctr, uint32_t sweep, uint32_t slot, hipStream_t s);
+void launch_site_ctr_tick(uint32_t *ctr, uint32_t sweeps, uint32_t records, uint32_t cap, hipStream_t s);
 
 // One-workgroup chain of a small power-of-two Ising lattice (plq_small.hip):
 // nsweeps sweeps in one launch with the lattice held in LDS.

@@ -105,10 +105,14 @@ __device__ __forceinline__ void emit_partials(const MeasArgs &m, long long bond,
 // Fold the column-major partials of the two colour passes (na and nb blocks)
 // into a record: one workgroup per record column, each column a contiguous
 // run read coalesced; columns >= nk are cleared.
+// With ctr non-null the record goes to slot ctr[1] of a ring of cap records
+// at rec (a replayed graph's record).
 __global__ __launch_bounds__(256) void finalize_columns_kernel(const long long *__restrict__ pa, int64_t na,
                                                                const long long *__restrict__ pb, int64_t nb, int nk,
-                                                               long long *__restrict__ rec) {
+                                                               long long *__restrict__ rec,
+                                                               const uint32_t *__restrict__ ctr) {
     __shared__ long long lds[16];
+    if (ctr) rec += (size_t)ctr[1] * REC_K;
     const int k = blockIdx.x;
     if (k >= nk) {
         if (threadIdx.x == 0) rec[k] = 0;
@@ -663,12 +667,13 @@ __device__ __forceinline__ void x16u_chunk(const uint4 *__restrict__ oth, uint4 
     }
     const uint32_t sv[4] = {s.x, s.y, s.z, s.w};
     // Rejections as bits, site 15 first, so that site i lands in bit i: a
-    // compare into VCC and an add-with-carry per site.  Class byte d + 8 s,
-    // its threshold at LDS byte 4 (d + 8 s).
+    // compare into VCC and an add-with-carry per site.  Class byte d + CS s
+    // (d <= 2D < CS), its threshold at LDS byte 4 (d + CS s).
+    constexpr int CS_LOG = D == 4 ? 4 : 3;
     uint32_t bits = 0;
 #pragma unroll
     for (int j = 3; j >= 0; --j) {
-        const uint32_t cls4 = (sv[j] << 3) + ds[j];
+        const uint32_t cls4 = (sv[j] << CS_LOG) + ds[j];
 #pragma unroll
         for (int k = 3; k >= 0; --k) {
             const uint32_t t2 = *reinterpret_cast<const uint32_t *>(Tb + (((cls4 >> (8 * k)) & 0xffu) << 2));
@@ -715,21 +720,26 @@ __device__ __forceinline__ void x16u_chunk(const uint4 *__restrict__ oth, uint4 
 template <int D, int MODE>
 __global__ __launch_bounds__(256) void ising_pass_x16u(const uint4 *__restrict__ oth, uint4 *__restrict__ own,
                                                        X16UGeom ug, int c, uint32_t seed, uint32_t skey,
+                                                       const uint32_t *__restrict__ ctr, uint32_t sweep_off,
                                                        const uint32_t *__restrict__ table, MeasArgs m, uint32_t work,
                                                        uint32_t span) {
     constexpr int NN = 2 * D;
     constexpr int K = NN + 1;
-    __shared__ uint32_t T[2 * 8];  // T[d + 8 s] = threshold * 256 - 1
+    // T[d + CS s] = threshold * 256 - 1: the class stride CS exceeds the
+    // largest down-neighbour count 2D (8 in 4D takes a stride of 16)
+    constexpr int CS = D == 4 ? 16 : 8;
+    __shared__ uint32_t T[2 * CS];
     __shared__ long long lds[16];
     if (threadIdx.x < 2 * K) {
         int s = threadIdx.x / K, d = threadIdx.x % K;
-        T[d + 8 * s] = (table[threadIdx.x] << 8) - 1u;
+        T[d + CS * s] = (table[threadIdx.x] << 8) - 1u;
     }
     __syncthreads();
     const uint32_t begin = (uint32_t)logical_block() * span;
     uint32_t end = begin + span;
     if (end > work) end = work;
-    const uint32_t kmix = sweep_mix(skey);
+    // a replayed graph reads its sweep from the device counter
+    const uint32_t kmix = sweep_mix(ctr ? (ctr[0] + sweep_off) * GOLDEN : skey);
     const char *Tb = reinterpret_cast<const char *>(T);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave in block
@@ -825,7 +835,32 @@ void launch_finalize_record(const long long *pa, int64_t na, const long long *pb
 
 void launch_finalize_columns(const long long *pa, int64_t na, const long long *pb, int64_t nb, int nk, long long *rec,
                              hipStream_t s) {
-    finalize_columns_kernel<<<dim3(REC_K), dim3(256), 0, s>>>(pa, na, pb, nb, nk, rec);
+    finalize_columns_kernel<<<dim3(REC_K), dim3(256), 0, s>>>(pa, na, pb, nb, nk, rec, nullptr);
+}
+
+void launch_finalize_columns_ring(const long long *pa, int64_t na, const long long *pb, int64_t nb, int nk,
+                                  long long *rec_base, const uint32_t *ctr, hipStream_t s) {
+    finalize_columns_kernel<<<dim3(REC_K), dim3(256), 0, s>>>(pa, na, pb, nb, nk, rec_base, ctr);
+}
+
+// Device counters of a site chain's graph replays: ctr[0] the sweep, ctr[1]
+// the record slot.
+__global__ void site_ctr_set(uint32_t *ctr, uint32_t sweep, uint32_t slot) {
+    ctr[0] = sweep;
+    ctr[1] = slot;
+}
+
+__global__ void site_ctr_tick(uint32_t *ctr, uint32_t sweeps, uint32_t records, uint32_t cap) {
+    ctr[0] += sweeps;
+    if (records) ctr[1] = (uint32_t)(((uint64_t)ctr[1] + records) % cap);
+}
+
+void launch_site_ctr_set(uint32_t *ctr, uint32_t sweep, uint32_t slot, hipStream_t s) {
+    site_ctr_set<<<dim3(1), dim3(1), 0, s>>>(ctr, sweep, slot);
+}
+
+void launch_site_ctr_tick(uint32_t *ctr, uint32_t sweeps, uint32_t records, uint32_t cap, hipStream_t s) {
+    site_ctr_tick<<<dim3(1), dim3(1), 0, s>>>(ctr, sweeps, records, cap);
 }
 
 static int64_t pass_work(const SitePassArgs &a) { return (int64_t)a.g.R * a.g.H / a.vec; }
@@ -877,11 +912,13 @@ static void dispatch_x16u(const SitePassArgs &a, int blocks, int64_t work, int64
     const uint4 *oth = (const uint4 *)a.oth;
     uint4 *own = (uint4 *)a.own;
     if (a.mode == 0)
-        ising_pass_x16u<D, 0><<<grid, block, 0, a.stream>>>(oth, own, a.ug, a.color, a.seed, a.skey, a.table, a.m,
-                                                            (uint32_t)work, (uint32_t)span);
+        ising_pass_x16u<D, 0><<<grid, block, 0, a.stream>>>(oth, own, a.ug, a.color, a.seed, a.skey, a.ctr,
+                                                            a.sweep_off, a.table, a.m, (uint32_t)work,
+                                                            (uint32_t)span);
     else
-        ising_pass_x16u<D, 1><<<grid, block, 0, a.stream>>>(oth, own, a.ug, a.color, a.seed, a.skey, a.table, a.m,
-                                                            (uint32_t)work, (uint32_t)span);
+        ising_pass_x16u<D, 1><<<grid, block, 0, a.stream>>>(oth, own, a.ug, a.color, a.seed, a.skey, a.ctr,
+                                                            a.sweep_off, a.table, a.m, (uint32_t)work,
+                                                            (uint32_t)span);
 }
 
 template <int D>

@@ -166,6 +166,9 @@ X16U_CASES = [
     ([256, 16, 8], HB, 1, 0.2216546, -0.05),
     ([64, 64, 4], HB, 0, 0.3, 0.0),
     ([128, 32, 4, 2], M, 1, 0.15, 0.1),
+    ([128, 32, 4, 2], HB, 0, 0.2216546, 0.05),  # 4D: 9 neighbour classes per spin state
+    ([64, 64, 2, 2], HB, 1, 0.3, 0.0),
+    ([64, 64, 2, 2], M, 0, 0.1, 0.3),          # strong field: the 8-aligned classes differ per spin
     ([512, 32], HB, 1, 0.4406868, 0.0),
     ([2048, 8], M, 0, 0.4406868, 0.2),
     ([256, 16, 8], M, 1, 5.0, 0.0),
@@ -189,6 +192,57 @@ def test_uniform_parity_pass_is_bit_exact(gpu, monkeypatch, case):
             st = O.local_sweeps(d, st, t, 1)
             assert (series[t].bond_sum, series[t].spin_sum) == O.ising_sums(shape, st), t
         assert np.array_equal(got, st), f"{(got != st).sum()} sites differ"
+
+
+GRAPH_CASES = [
+    # shape, kernel, record every (0: off), advances
+    ([256, 16, 8], M, 10, [3, 7, 25, 40, 1, 24]),
+    ([512, 32], HB, 1, [5, 2, 9]),
+    ([64, 64, 4], M, 0, [15, 33, 16, 2]),
+    ([128, 32, 4, 2], HB, 64, [70, 64, 130]),
+    ([256, 16, 8], HB, 100, [150, 60]),  # beyond 64 sweeps per record: no replays
+]
+
+
+@pytest.mark.parametrize("case", GRAPH_CASES, ids=[f"{'x'.join(map(str, c[0]))}-{c[1]}-every{c[2]}" for c in GRAPH_CASES])
+def test_graph_replays_of_site_sweeps_are_bit_exact(gpu, monkeypatch, case):
+    # With PLQ_SITE_GRAPH=1, uniform-pass chains replay whole blocks of
+    # sweeps (the record interval, or 16 without records) from a captured
+    # graph, keyed by device counters, and launch the sweeps around them
+    # directly.  Advances of odd lengths shift the phase; a set_sweep and a
+    # new record ring midway invalidate the counters and the graph.  States
+    # and records equal the direct launches (the default) and the oracle.
+    shape, kernel, every, steps = case
+    n = int(np.prod(shape))
+    start = start_config(n, 2, True, 31)
+    outs = []
+    for graph in ("1", "0"):
+        monkeypatch.setenv("PLQ_SITE_GRAPH", graph)
+        c = gpu.ising_gpu_chain(shape, 1.0, 0.05, 0.2216546, 12, start, kernel=kernel)
+        if every:
+            c.record(every, 400)
+        for k in steps:
+            c.advance(k)
+        mid = c.read()
+        c.set_sweep(1000)
+        if every:
+            c.record(every, 400)
+        c.advance(2 * (every or 16) + 3)
+        outs.append((mid, c.read(), [(o.sweep, o.bond_sum, o.spin_sum) for o in c.series()]))
+        c.close()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    assert outs[0][2] == outs[1][2]
+    d = O.local_desc(shape, kernel=kernel, beta=0.2216546, h=0.05, seed=12)
+    mid = O.local_sweeps(d, start, 0, sum(steps))
+    assert np.array_equal(outs[0][0], mid)
+    more = 2 * (every or 16) + 3
+    assert np.array_equal(outs[0][1], O.local_sweeps(d, mid, 1000, more))
+    if every:
+        sweeps = [r[0] for r in outs[0][2]]
+        assert sweeps == [s for s in range(1001, 1001 + more) if s % every == 0]
+        for sw, bond, spin in outs[0][2]:
+            assert (bond, spin) == O.ising_sums(shape, O.local_sweeps(d, mid, 1000, sw - 1000)), sw
 
 
 SMALL_CASES = [([64, 64], M), ([64, 64], HB), ([16], M), ([2, 4], HB), ([16, 16, 16], M), ([8, 8, 4, 4], HB),
