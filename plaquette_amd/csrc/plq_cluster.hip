@@ -264,8 +264,8 @@ struct X16Box {
 //      to H[c].
 //   2. Unions, after a barrier, between run starts: the axis-0 bond out of
 //      the segment's last site, and each open bond of the other axes unless
-//      an earlier bond of the segment joins the same two runs (the runs of
-//      the row one step up are read from its segment's H).
+//      the bond one site back joins the same two runs (the runs of the row
+//      one step up are read from its segment's H).
 //   3. After a barrier, the roots of the run starts; every site of a run
 //      stores the run's root as a global index (four 16-byte stores).
 // The draws are the reference's (bond = site * D + axis); for D = 1, 2, 4
@@ -363,14 +363,9 @@ __device__ __forceinline__ void x16_label(int32_t *P, uint16_t *H, int32_t *__re
             uint32_t v = vmask[mu];
             if (!v) continue;
             const uint32_t sn = ((~(uint32_t)H[nbr_l0[mu] >> 4] << 1) | 1u) & 0xffffu;  // the neighbour's run starts
-            // keep the first bond of the segment between each pair of runs
-            uint32_t need = 0;
-            for (uint32_t w = v; w; w &= w - 1u) {
-                const uint32_t k = (uint32_t)__builtin_ctz(w);
-                const uint32_t le = (2u << k) - 1u;
-                const uint32_t m = max(31u - (uint32_t)__clz((int)(starts & le)), 31u - (uint32_t)__clz((int)(sn & le)));
-                need |= (v & ((1u << k) - (1u << m))) ? 0u : 1u << k;
-            }
+            // drop a bond that continues both runs of the bond one site back:
+            // that bond joins the same two runs
+            uint32_t need = v & ~((v << 1) & ~starts & ~sn);
             while (need) {
                 const uint32_t k = (uint32_t)__builtin_ctz(need);
                 need &= need - 1u;
