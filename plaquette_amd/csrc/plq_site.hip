@@ -639,13 +639,25 @@ __device__ __forceinline__ void x16u_chunk(const uint4 *__restrict__ oth, uint4 
     if (!late) load();
     // The draws (site0 = 2A + P has bits 1..4 clear); y = the final hash
     // before its 8-bit shift, so that m < T  <=>  y <= T * 256 - 1.
+    //
+    // Site k hashes v0 ^ 2k, v0 = x ^ (x >> 16) with x = seed ^ (2A + P).  Bits
+    // 1..4 of v0, b, are bits 1..4 of seed ^ (seed >> 16) xor bits 16..19 of
+    // A, and every lane of the wave has the same A >> 16 (its rows differ
+    // below bit 16 - sh), so b is wave-uniform.  Then
+    // (v0 ^ 2k) * C1 = (v0 & ~30) * C1 + (b ^ k) * 2 C1 (mod 2^32): the first
+    // multiply of every site is one add of a wave-uniform offset.
     uint32_t y[16];
     __builtin_amdgcn_sched_barrier(0);
     {
         const uint32_t bx = seed ^ (2u * A + (uint32_t)P);
         const uint32_t v0 = bx ^ (bx >> 16);
+        const uint32_t b = __builtin_amdgcn_readfirstlane(((seed >> 1) ^ (seed >> 17) ^ (wv.base >> 16)) & 15u);
+        const uint32_t q0 = (v0 & ~30u) * 0x7feb352du;
 #pragma unroll
-        for (int k = 0; k < 16; ++k) y[k] = lowbias32_tail(lowbias32_mid(v0 ^ (2u * (uint32_t)k)) ^ kmix);
+        for (int k = 0; k < 16; ++k) {
+            const uint32_t t = q0 + (b ^ (uint32_t)k) * 0xffd66a5au;  // 2 * 0x7feb352d (mod 2^32)
+            y[k] = lowbias32_tail(((t ^ (t >> 15)) * 0x846ca68bu) ^ kmix);
+        }
     }
     __builtin_amdgcn_sched_barrier(0);
     if (late) load();
@@ -692,17 +704,19 @@ __device__ __forceinline__ void x16u_chunk(const uint4 *__restrict__ oth, uint4 
     }
     stb(own, A, make_uint4(ov[0], ov[1], ov[2], ov[3]));
     if (on) {
-        down += __popc(ov[0]) + __popc(ov[1]) + __popc(ov[2]) + __popc(ov[3]);
+        const int dn = (int)(__popc(ov[0]) + __popc(ov[1]) + __popc(ov[2]) + __popc(ov[3]));
+        down += dn;
         total += 16;
         if (c == 1) {
+            // sum over the 16 sites of (1 - 2 s)(NN - 2 d) = 16 NN - 2 sum d - 2 NN sum s + 4 sum s d,
+            // with the byte sums of d and of d where s = 1 taken by v_sad_u8
+            uint32_t dsum = 0, sd = 0;
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    int sn = (int)((ov[j] >> (8 * k)) & 1u);
-                    int dn = (int)((ds[j] >> (8 * k)) & 0xffu);
-                    bond += (1 - 2 * sn) * (NN - 2 * dn);
-                }
+            for (int j = 0; j < 4; ++j) {
+                dsum = __builtin_amdgcn_sad_u8(ds[j], 0u, dsum);
+                sd = __builtin_amdgcn_sad_u8(ds[j] & (ov[j] * 0xffu), 0u, sd);
+            }
+            bond += 16 * NN - 2 * (int)dsum - 2 * NN * dn + 4 * (int)sd;
         }
     }
 }
