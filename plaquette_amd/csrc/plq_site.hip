@@ -688,7 +688,16 @@ __device__ __forceinline__ void x16u_chunk(const uint4 *__restrict__ oth, uint4 
         const uint32_t cls4 = (sv[j] << CS_LOG) + ds[j];
 #pragma unroll
         for (int k = 3; k >= 0; --k) {
-            const uint32_t t2 = *reinterpret_cast<const uint32_t *>(Tb + (((cls4 >> (8 * k)) & 0xffu) << 2));
+            // the class byte scaled to its LDS address; byte 0 as one SDWA op too
+            // (the compiler folds (x & 0xff) << 2 into a shift and a mask)
+            uint32_t addr;
+            if (k == 0)
+                asm("v_lshlrev_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_0"
+                    : "=v"(addr)
+                    : "v"(2u), "v"(cls4));
+            else
+                addr = ((cls4 >> (8 * k)) & 0xffu) << 2;
+            const uint32_t t2 = *reinterpret_cast<const uint32_t *>(Tb + addr);
             asm("v_cmp_gt_u32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc"
                 : "+v"(bits)
                 : "v"(y[4 * j + k]), "v"(t2)

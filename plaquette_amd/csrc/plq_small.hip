@@ -135,6 +135,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void small_chain_kernel(uint8_t *__r
     // Per-lane site words: own[c][k] = 2*offset + parity (= lattice index),
     // nb[c][k][mu] = (offset of the - neighbour << 16) | offset of the + one.
     uint32_t own[2][K];
+    uint32_t h1[2][K];  // the first hash round of the site's draw, the same every sweep
     uint32_t nb[2][K][D];
     const uint32_t lh = sg.lg[0] - 1;  // log2(L0 / 2)
     const uint32_t L0m = (1u << sg.lg[0]) - 1u;
@@ -145,6 +146,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void small_chain_kernel(uint8_t *__r
             uint32_t w = tid + (uint32_t)k * SMALL_THREADS;
             if (w >= half) {
                 own[c][k] = NO_SITE;
+                h1[c][k] = 0;
 #pragma unroll
                 for (int mu = 0; mu < D; ++mu) nb[c][k][mu] = 0;
                 continue;
@@ -158,6 +160,10 @@ __global__ __launch_bounds__(SMALL_THREADS) void small_chain_kernel(uint8_t *__r
             uint32_t x0 = 2 * i + p;
             uint32_t rowbase = r << lh;
             own[c][k] = 2 * w + p;
+            {
+                const uint32_t x = seed ^ own[c][k];
+                h1[c][k] = lowbias32_mid(x ^ (x >> 16));
+            }
             uint32_t xp = (x0 + 1) & L0m, xm = (x0 - 1) & L0m;
             nb[c][k][0] = (rowbase + (xp >> 1)) | ((rowbase + (xm >> 1)) << 16);
 #pragma unroll
@@ -173,7 +179,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void small_chain_kernel(uint8_t *__r
 
     SmallRecorder records(rec, rec_cap, rec_first, every, phase, half);
     for (uint32_t t = 0; t < nsweeps; ++t) {
-        const uint32_t skey = (sweep0 + t) * GOLDEN;
+        const uint32_t kmix = sweep_mix((sweep0 + t) * GOLDEN);  // keyed_bits_k's merged xor-shift
         const bool record = records.recording();
         int bond = 0, ndown = 0;
 #pragma unroll
@@ -196,7 +202,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void small_chain_kernel(uint8_t *__r
                 bool hit;
                 if (th == 0u) hit = false;
                 else if (th >= DRAW_ONE) hit = true;
-                else hit = keyed_bits_k(seed, op, skey) < th;
+                else hit = (lowbias32_tail(h1[c][k] ^ kmix) >> 8) < th;  // keyed_bits_k(seed, op, sweep key)
                 const int nxt = MODE == 0 ? (hit ? 1 - cur : cur) : (hit ? 0 : 1);
                 mine[off] = (uint8_t)nxt;
                 if (record) {
@@ -266,7 +272,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void small_chain_x4(uint8_t *__restr
         rowp[mu - 1] = (rp << lh4) + j;
         rowm[mu - 1] = (rm << lh4) + j;
     }
-    uint32_t p[2], adj[2], v0[2];
+    uint32_t p[2], adj[2], u[2][4];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
         p[c] = (prs + (uint32_t)c) & 1u;
@@ -274,9 +280,12 @@ __global__ __launch_bounds__(SMALL_THREADS) void small_chain_x4(uint8_t *__restr
         adj[c] = (r << lh4) + ((j + (p[c] ? 1u : (1u << lh4) - 1u)) & ((1u << lh4) - 1u));
         // site k of the dword is site0 + 2k, and site0 has bits 1..4 clear, so
         // the first xor-shift of the draw, (x ^ x >> 16) with x = seed ^ site,
-        // is v0 ^ 2k
+        // is v0 ^ 2k.  The first hash round does not depend on the sweep: it
+        // is computed once, and a sweep's draw is the second round of it.
         const uint32_t bx = seed ^ ((r << sg.lg[0]) + (j << 3) + p[c]);
-        v0[c] = bx ^ (bx >> 16);
+        const uint32_t v0 = bx ^ (bx >> 16);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) u[c][k] = lowbias32_mid(v0 ^ (2u * (uint32_t)k));
     }
     const char *Tb = reinterpret_cast<const char *>(T);
     __syncthreads();
@@ -301,7 +310,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void small_chain_x4(uint8_t *__restr
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const uint32_t tk = *reinterpret_cast<const uint32_t *>(Tb + ((addr4 >> (8 * k)) & 0xffu));
-                    const uint32_t m = lowbias32_tail(lowbias32_mid(v0[c] ^ (2u * (uint32_t)k)) ^ kmix) >> 8;
+                    const uint32_t m = lowbias32_tail(u[c][k] ^ kmix) >> 8;
                     diff[k] = m - tk;  // negative iff m < t (both below 2^31)
                 }
                 const uint32_t lo = __builtin_amdgcn_perm(diff[1], diff[0], 0x0c0c0703u);
