@@ -435,6 +435,12 @@ def bench_z2_batch(P, ev, steps, warmup, cpu_seconds=None):
     wall, ms = timed(P, ev, e.stream, lambda: (e.advance(steps), e.sync()))
     sums, _ = e.series()
     acc, att = e.exchange_counts()
+    # the same sweeps with every colour pass of a 10-sweep block in one
+    # cooperative launch (grid barriers between the passes), for comparison
+    e.set_launch(tuned["block_size"], tuned["per_lane"], 2)
+    e.advance(warmup)
+    e.sync()
+    coop_wall, _ = timed(P, ev, e.stream, lambda: (e.advance(steps), e.sync()))
     e.close()
     n_plaq = np.array([int(np.prod(s)) * 6 for s in shapes])
     rate = steps * links / (wall * 1e9)
@@ -455,6 +461,10 @@ def bench_z2_batch(P, ev, steps, warmup, cpu_seconds=None):
                                    "500 times back to back (bytes read plus written); one launch per copy, "
                                    "so launch time is in it, as it is in the eight passes of a sweep"},
         "tuned_launch": tuned,
+        "cooperative_launch": {"us_per_sweep": round(coop_wall * 1e6 / steps, 2),
+                               "link_updates_per_ns": round(steps * links / (coop_wall * 1e9), 2),
+                               "note": "every pass of a 10-sweep block in one cooperative launch, grid barriers "
+                                       "between the passes"},
         "mean_plaquette_last": [round(float(x), 5) for x in (sums[-1] / n_plaq)] if len(sums) else None,
         "exchange_acceptance": round(float(acc.sum()) / max(1, float(att.sum())), 4),
         "cpu_baseline": cpu_baseline_z2_batch(cpu_seconds) if cpu_seconds else None,

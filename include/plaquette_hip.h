@@ -282,7 +282,11 @@ int plq_ensemble_series(plq_ensemble *e, int64_t *sums, uint32_t *slots, uint64_
 /* Accepted swaps and attempts per exchange pair (pairs in group order). */
 int plq_ensemble_exchange_counts(plq_ensemble *e, uint64_t *accepted, uint64_t *attempted, size_t npairs,
                                  size_t *npairs_out);
-/* Launch shape: lanes per block, links per lane, hipGraph replay on/off. */
+/* Launch shape: lanes per block, links per lane, and how a block of sweeps
+ * is launched: use_graph 0 one launch per colour pass, 1 those launches
+ * replayed from a captured hipGraph, 2 every pass of the block in one
+ * cooperative launch with grid barriers between the passes (bit for bit the
+ * same; measured slower on MI355X, so the autotuner does not pick it there). */
 int plq_ensemble_set_launch(plq_ensemble *e, int32_t block_size, int32_t per_lane, int32_t use_graph);
 /* Time the candidate launch shapes on a scratch copy of the state and keep
  * the fastest; the chain's state and counters are left as they were. */

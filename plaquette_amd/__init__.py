@@ -703,13 +703,13 @@ class Ensemble:
         return acc[:n.value], att[:n.value]
 
     def set_launch(self, block_size=256, per_lane=1, use_graph=True):
-        _check(lib().plq_ensemble_set_launch(self._h, int(block_size), int(per_lane), int(bool(use_graph))))
+        _check(lib().plq_ensemble_set_launch(self._h, int(block_size), int(per_lane), int(use_graph)))
 
     def tune(self) -> dict:
         bs, pl, g = C.c_int32(0), C.c_int32(0), C.c_int32(0)
         us = C.c_double(0)
         _check(lib().plq_ensemble_tune(self._h, C.byref(bs), C.byref(pl), C.byref(g), C.byref(us)))
-        return {"block_size": bs.value, "per_lane": pl.value, "use_graph": bool(g.value),
+        return {"block_size": bs.value, "per_lane": pl.value, "use_graph": g.value,
                 "us_per_sweep": us.value}
 
     @property

@@ -22,6 +22,7 @@
 //
 // Each block belongs to one quad (blocks never straddle quads), so the
 // geometry and the four threshold tables are uniform per block.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include "plq_common.hpp"
@@ -211,19 +212,19 @@ __device__ __forceinline__ EnsDeltas ens_deltas(const EnsQuad &Q, const LinkSite
 // replicas in one dword, and per replica one compare of the unshifted hash
 // y against threshold * 256 - 1 into VCC and one add-with-carry.  Every
 // threshold of every beta slot must be at least 1 (the host checks).
+// The body of one logical block lb of a (MU, c) pass, for ens_link_pass2
+// (one launch per pass) and ens_sweeps_coop (every pass of a block of
+// sweeps in one launch).  T is the block's LDS threshold table.
 template <int D, int MODE, int MU>
-__global__ __launch_bounds__(256) void ens_link_pass2(uint32_t *__restrict__ words, const EnsQuad *__restrict__ quads,
-                                                      const uint16_t *__restrict__ block_quad,
-                                                      const uint32_t *__restrict__ tables,
-                                                      const uint32_t *__restrict__ beta_slot,
-                                                      const uint32_t *__restrict__ sweep_base, uint32_t sweep_off,
-                                                      int c, int per_lane) {
+__device__ __forceinline__ void ens_pass2_block(uint32_t *__restrict__ words, const EnsQuad *__restrict__ quads,
+                                                const uint16_t *__restrict__ block_quad,
+                                                const uint32_t *__restrict__ tables,
+                                                const uint32_t *__restrict__ beta_slot, uint32_t kmix, int c,
+                                                int per_lane, uint32_t lb, uint32_t *T) {
     constexpr int K = 2 * (D - 1);
     constexpr int TL = 2 * (K + 1);
-    __shared__ uint32_t T[4 * TL];  // T[j * TL + cur * (K + 1) + n] = threshold * 256 - 1
-    const uint32_t lb = ens_logical_block();
     const int qi = block_quad[lb];
-    if (qi == NO_QUAD) return;
+    if (qi == NO_QUAD) return;  // the same for the whole block
     const EnsQuad Q = quads[qi];
     if (threadIdx.x < 4 * TL) {
         const int j = threadIdx.x / TL, e = threadIdx.x - j * TL;
@@ -232,7 +233,6 @@ __global__ __launch_bounds__(256) void ens_link_pass2(uint32_t *__restrict__ wor
     }
     __syncthreads();
     const uint32_t uc = (uint32_t)c, uo = 1u - uc;
-    const uint32_t kmix = sweep_mix((*sweep_base + sweep_off) * GOLDEN);
     const uint32_t half = Q.nsites >> 1;
     // bytes of the word that hold a replica (unused bytes keep their value)
     uint32_t used = 0;
@@ -285,6 +285,62 @@ __global__ __launch_bounds__(256) void ens_link_pass2(uint32_t *__restrict__ wor
         uint32_t wnew = MODE == 0 ? (wcur ^ (~raw & 0x01010101u)) : (raw & 0x01010101u);
         wnew = (wnew & used) | (wcur & ~used);
         if (wnew != wcur) own[item] = wnew;
+    }
+}
+
+// One (MU, c) pass of every replica, as ens_link_pass, with the staple words
+// addressed by index deltas from the lane's own index (no per-neighbour
+// coordinate arithmetic) and the four replicas' decisions taken as in the
+// uniform-parity site pass: the class bytes cur * (K + 1) + n of all four
+// replicas in one dword, and per replica one compare of the unshifted hash
+// y against threshold * 256 - 1 into VCC and one add-with-carry.  Every
+// threshold of every beta slot must be at least 1 (the host checks).
+template <int D, int MODE, int MU>
+__global__ __launch_bounds__(256) void ens_link_pass2(uint32_t *__restrict__ words, const EnsQuad *__restrict__ quads,
+                                                      const uint16_t *__restrict__ block_quad,
+                                                      const uint32_t *__restrict__ tables,
+                                                      const uint32_t *__restrict__ beta_slot,
+                                                      const uint32_t *__restrict__ sweep_base, uint32_t sweep_off,
+                                                      int c, int per_lane) {
+    constexpr int TL = 2 * (2 * (D - 1) + 1);
+    __shared__ uint32_t T[4 * TL];  // T[j * TL + cur * (K + 1) + n] = threshold * 256 - 1
+    const uint32_t kmix = sweep_mix((*sweep_base + sweep_off) * GOLDEN);
+    ens_pass2_block<D, MODE, MU>(words, quads, block_quad, tables, beta_slot, kmix, c, per_lane, ens_logical_block(),
+                                 T);
+}
+
+// Every colour pass of nsweeps sweeps of every replica in one cooperative
+// launch: the grid's blocks walk the logical blocks of a pass, and a grid
+// barrier separates consecutive passes (a link's staples hold links of the
+// other passes).  Bit for bit the per-pass launches; measured slower, since a
+// grid barrier costs more than a launch boundary on this GPU.
+template <int D, int MODE>
+__global__ __launch_bounds__(256) void ens_sweeps_coop(uint32_t *__restrict__ words, const EnsQuad *__restrict__ quads,
+                                                       const uint16_t *__restrict__ block_quad,
+                                                       const uint32_t *__restrict__ tables,
+                                                       const uint32_t *__restrict__ beta_slot,
+                                                       const uint32_t *__restrict__ sweep_base, uint32_t nsweeps,
+                                                       int per_lane, uint32_t nblocks) {
+    constexpr int TL = 2 * (2 * (D - 1) + 1);
+    __shared__ uint32_t T[4 * TL];
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    const uint32_t sweep0 = *sweep_base;  // the counter ticks after the launch
+    for (uint32_t k = 0; k < nsweeps; ++k) {
+        const uint32_t kmix = sweep_mix((sweep0 + k) * GOLDEN);
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu)
+            for (int c = 0; c < 2; ++c) {
+                for (uint32_t lb = blockIdx.x; lb < nblocks; lb += gridDim.x) {
+                    __syncthreads();  // the previous logical block is done with T
+                    switch (mu) {
+                        case 0: ens_pass2_block<D, MODE, 0>(words, quads, block_quad, tables, beta_slot, kmix, c, per_lane, lb, T); break;
+                        case 1: ens_pass2_block<D, MODE, 1>(words, quads, block_quad, tables, beta_slot, kmix, c, per_lane, lb, T); break;
+                        case 2: ens_pass2_block<D, MODE, (2 < D ? 2 : 0)>(words, quads, block_quad, tables, beta_slot, kmix, c, per_lane, lb, T); break;
+                        default: ens_pass2_block<D, MODE, (3 < D ? 3 : 0)>(words, quads, block_quad, tables, beta_slot, kmix, c, per_lane, lb, T); break;
+                    }
+                }
+                grid.sync();
+            }
     }
 }
 
@@ -469,6 +525,41 @@ static void ens_pass_mu(const EnsPassArgs &a) {
         default: PLQ_ENS_MU(3 < D ? 3 : 0); break;
     }
 #undef PLQ_ENS_MU
+}
+
+template <int D, int MODE>
+static hipError_t ens_coop(const EnsPassArgs &a, uint32_t nsweeps) {
+    int bpc = 0, cus = 0, dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, ens_sweeps_coop<D, MODE>, a.block_size, 0);
+    if (e != hipSuccess) return e;
+    if (bpc < 1) return hipErrorInvalidConfiguration;
+    // one block per CU: a grid barrier's cost grows with the blocks it joins
+    // (measured 1.2 ms per sweep with every resident block, against 0.3 ms)
+    uint32_t grid = (uint32_t)cus;
+    if (grid > a.blocks) grid = a.blocks;
+    uint32_t *words = a.words;
+    const EnsQuad *quads = a.quads;
+    const uint16_t *block_quad = a.block_quad;
+    const uint32_t *tables = a.tables, *beta_slot = a.beta_slot, *sweep_base = a.sweep_base;
+    int per_lane = a.per_lane;
+    uint32_t nblocks = a.blocks;
+    void *args[] = {&words, &quads, &block_quad, &tables, &beta_slot, &sweep_base, &nsweeps, &per_lane, &nblocks};
+    return hipLaunchCooperativeKernel((const void *)ens_sweeps_coop<D, MODE>, dim3(grid), dim3(a.block_size), args, 0,
+                                      a.stream);
+}
+
+hipError_t launch_ens_sweeps_coop(const EnsPassArgs &a, uint32_t nsweeps) {
+    switch (a.D * 2 + a.mode) {
+        case 4: return ens_coop<2, 0>(a, nsweeps);
+        case 5: return ens_coop<2, 1>(a, nsweeps);
+        case 6: return ens_coop<3, 0>(a, nsweeps);
+        case 7: return ens_coop<3, 1>(a, nsweeps);
+        case 8: return ens_coop<4, 0>(a, nsweeps);
+        default: return ens_coop<4, 1>(a, nsweeps);
+    }
 }
 
 void launch_ens_pass(const EnsPassArgs &a) {

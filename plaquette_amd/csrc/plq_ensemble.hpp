@@ -66,6 +66,9 @@ struct EnsPassArgs {
 void launch_ens_pack(const uint8_t *lat, uint32_t *words, const EnsQuad &Q, int j, int D, hipStream_t s);
 void launch_ens_unpack(uint8_t *lat, const uint32_t *words, const EnsQuad &Q, int j, int D, hipStream_t s);
 void launch_ens_pass(const EnsPassArgs &a);
+// Every colour pass of nsweeps sweeps in one cooperative launch with grid
+// barriers between the passes (ens_link_pass2's body; a.fast must hold).
+hipError_t launch_ens_sweeps_coop(const EnsPassArgs &a, uint32_t nsweeps);
 void launch_ens_plaquette(const uint32_t *words, const EnsQuad *quads, const uint16_t *block_quad, int nrep, int D,
                           uint32_t blocks, int block_size, int per_lane, unsigned long long *sums, hipStream_t s);
 void launch_ens_exchange(const EnsGroupPair *pairs, int npairs, uint32_t *slot_of, uint32_t *rep_at,

@@ -158,7 +158,18 @@ void enqueue_block(plq_ensemble *e, uint32_t nsweeps, bool measure) {
     a.block_size = e->block_size;
     a.per_lane = e->per_lane;
     a.stream = e->stream;
-    for (uint32_t k = 0; k < nsweeps; ++k) {
+    // use_graph = 2: the block's sweeps in one cooperative launch (grid
+    // barriers between the passes); a launch the device refuses falls back
+    // to per-pass launches for good
+    bool coop_done = false;
+    if (e->use_graph == 2 && e->fast && nsweeps > 0) {
+        if (plq::launch_ens_sweeps_coop(a, nsweeps) == hipSuccess) coop_done = true;
+        else {
+            (void)hipGetLastError();
+            e->use_graph = 0;
+        }
+    }
+    for (uint32_t k = 0; k < nsweeps && !coop_done; ++k) {
         a.sweep_off = k;
         for (int mu = 0; mu < e->D; ++mu)
             for (int c = 0; c < 2; ++c) {
@@ -216,7 +227,7 @@ int advance(plq_ensemble *e, uint64_t sweeps) {
     const bool measure = e->measure_every != 0;
     while (sweeps > 0) {
         uint64_t into = measure ? e->sweeps_done % e->measure_every : 0;
-        if (e->use_graph && into == 0 && sweeps >= block) {
+        if (e->use_graph == 1 && into == 0 && sweeps >= block) {
             if (!e->graph_exec || e->graph_sweeps != block || e->graph_measures != measure)
                 ENS_TRY(build_graph(e, block, measure));
             ENS_HIP(hipGraphLaunch(e->graph_exec, e->stream));
@@ -567,9 +578,10 @@ int plq_ensemble_set_launch(plq_ensemble *e, int32_t block_size, int32_t per_lan
     std::lock_guard<std::mutex> lk(e->mu);
     ENS_HIP(hipSetDevice(e->device));
     ENS_HIP(hipStreamSynchronize(e->stream));
+    if (use_graph < 0 || use_graph > 2) return plq::host_fail(PLQ_EINVAL, "use_graph must be 0, 1 or 2");
     e->block_size = block_size;
     e->per_lane = per_lane;
-    e->use_graph = use_graph ? 1 : 0;
+    e->use_graph = use_graph;
     return layout_blocks(e);
 }
 
@@ -606,7 +618,7 @@ int plq_ensemble_tune(plq_ensemble *e, int32_t *block_size, int32_t *per_lane, i
     int bbs = saved_bs, bpl = saved_pl, bg = saved_g;
     const uint64_t block = e->measure_every ? e->measure_every : 10;
     const uint64_t trial = block * 3;
-    for (int g : {1, 0})
+    for (int g : {1, 0, 2})
         for (int bs : {64, 128, 256})
             for (int pl : {1, 2, 4}) {
                 e->block_size = bs;

@@ -1,6 +1,7 @@
 """Batched Z2 ensembles with replica exchange: bit-exact against the oracle
 (states, beta assignments, recorded plaquette sums, exchange counts) and
-independent of the launch shape and of hipGraph replay (GPU)."""
+independent of the launch shape, of hipGraph replay and of the cooperative
+one-launch-per-block mode (GPU)."""
 import numpy as np
 import pytest
 
@@ -25,7 +26,7 @@ def make_case(kind):
 
 @pytest.mark.parametrize("kind", ["mixed4d", "uneven3d", "plane"])
 @pytest.mark.parametrize("kernel", [O.KERNEL_HEAT_BATH, O.KERNEL_METROPOLIS])
-@pytest.mark.parametrize("graph", [True, False])
+@pytest.mark.parametrize("graph", [1, 0, 2], ids=["graph", "direct", "coop"])
 def test_ensemble_is_bit_exact(gpu, kind, kernel, graph):
     dim, shapes, betas = make_case(kind)
     seeds = [100 + r for r in range(len(shapes))]
@@ -63,7 +64,7 @@ def test_launch_shapes_and_tuning_leave_the_chain_unchanged(gpu):
     sizes = [int(np.prod(s)) * dim for s in shapes]
     start = np.random.default_rng(3).integers(0, 2, sum(sizes)).astype(np.uint8)
     outs = []
-    for bs, pl, g in ((256, 1, True), (64, 4, False), (128, 2, True)):
+    for bs, pl, g in ((256, 1, 1), (64, 4, 0), (128, 2, 1), (64, 2, 2)):
         e = gpu.Ensemble(dim, shapes, betas, seeds, start=start, measure_every=5, exchange=True)
         e.set_launch(bs, pl, g)
         e.advance(23)
@@ -97,16 +98,17 @@ def test_full_batched_workload_runs_and_agrees_across_launch_shapes(gpu):
             seeds.append(1000 * L + k)
     sizes = [int(np.prod(s)) * 4 for s in shapes]
     outs = []
-    for bs, pl, g in ((256, 1, True), (128, 4, False)):
+    for bs, pl, g in ((256, 1, 1), (128, 4, 0), (256, 1, 2)):
         e = gpu.Ensemble(4, shapes, betas, seeds, measure_every=10, exchange=True, exchange_seed=5)
         e.set_launch(bs, pl, g)
         e.advance(40)
         outs.append((e.read(), e.betas(), e.series()[0], e.exchange_counts()[0]))
         e.close()
-    assert np.array_equal(outs[0][0], outs[1][0])
-    assert np.array_equal(outs[0][1], outs[1][1])
-    assert np.array_equal(outs[0][2], outs[1][2])
-    assert np.array_equal(outs[0][3], outs[1][3])
+    for o in outs[1:]:
+        assert np.array_equal(outs[0][0], o[0])
+        assert np.array_equal(outs[0][1], o[1])
+        assert np.array_equal(outs[0][2], o[2])
+        assert np.array_equal(outs[0][3], o[3])
     # mean plaquette per replica from a cold start stays in the ordered phase
     last = outs[0][2][-1]
     n_plaq = np.array([int(np.prod(s)) * 6 for s in shapes])
