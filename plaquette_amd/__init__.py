@@ -703,6 +703,9 @@ class Ensemble:
         return acc[:n.value], att[:n.value]
 
     def set_launch(self, block_size=256, per_lane=1, use_graph=True):
+        """Lanes per block, links per lane, and the launch mode of a sweep block:
+        use_graph 0 (or False) one launch per colour pass, 1 (or True) a captured
+        hipGraph replay, 2 one cooperative launch with grid barriers."""
         _check(lib().plq_ensemble_set_launch(self._h, int(block_size), int(per_lane), int(use_graph)))
 
     def tune(self) -> dict:
