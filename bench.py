@@ -163,16 +163,15 @@ def cpu_baseline_all_cores(seconds_target: float):
     }
 
 
-def cpu_baseline_ising2d_64(seconds_target: float):
+def cpu_baseline_ising2d_64(seconds_target: float, start):
     """The reference CPU checkerboard path on 64^2 (PCG64 stream, f64
-    acceptance, neighbour and parity tables) on one core, bounded sample."""
-    import numpy as np
-
+    acceptance, neighbour and parity tables) on one core, bounded sample,
+    from the benchmarked hot start."""
     import oracle as O
 
     shape = [64, 64]
     n = 4096
-    st = np.random.default_rng(SEED).integers(0, 2, n).astype(np.uint8)
+    st = start.copy()
     st, secs = O.pmode_ising_sweeps(shape, O.KERNEL_METROPOLIS, 0.4406868, 1.0, 0.0, SEED, st, 100)
     sweeps = max(100, int(100 * seconds_target / max(secs, 1e-6)))
     st, secs2 = O.pmode_ising_sweeps(shape, O.KERNEL_METROPOLIS, 0.4406868, 1.0, 0.0, SEED, st, sweeps)
@@ -183,20 +182,19 @@ def cpu_baseline_ising2d_64(seconds_target: float):
     }
 
 
-def cpu_baseline_potts_sw(seconds_target: float):
+def cpu_baseline_potts_sw(seconds_target: float, start):
     """The oracle's sequential Swendsen-Wang sweep at 4096^2 on one core: the
     reference CPU ClusterUpdate (updater.rs:541-613) -- bond draw, union-find
     forest with minimum-site roots (cluster.rs:44-227), relabel per cluster --
-    driven by the keyed stream; bounded sample of whole sweeps."""
-    import numpy as np
-
+    driven by the keyed stream; bounded sample of whole sweeps from the
+    benchmarked hot start."""
     import oracle as O
 
     shape = [4096, 4096]
     n = 4096 * 4096
     beta = math.log(1 + math.sqrt(3))
     d = O.cluster_desc(shape, q=3, beta=beta, gap=1.0, seed=SEED)
-    st = np.random.default_rng(SEED).integers(0, 3, n).astype(np.uint8)
+    st = start.copy()
     t0 = time.perf_counter()
     st, _ = O.cluster_sweeps(d, st, 0, 1)
     one = time.perf_counter() - t0
@@ -338,7 +336,8 @@ def bench_potts_sw(P, ev, steps, warmup, cpu_seconds=None):
     shape = [4096, 4096]
     n = 4096 * 4096
     beta = math.log(1 + math.sqrt(3))
-    start = np.random.default_rng(SEED).integers(0, 3, n).astype(np.uint8)
+    # the run config's hot start: the seeded stream's draws (configuration.rs:80-87)
+    start = P.hot_start(SEED, 3, n)[0]
     c = P.GpuClusterChain.swendsen_wang(shape, 3, 1.0, beta, SEED, start)
     c.advance(warmup)
     c.sync()
@@ -379,7 +378,7 @@ def bench_potts_sw(P, ev, steps, warmup, cpu_seconds=None):
                                   "site_updates_per_ns": round(hist_steps * n / (hist_wall * 1e9), 2),
                                   "histograms_kept": len(hist_series),
                                   "last_clusters": hist_series[-1].n_clusters if hist_series else None},
-        "cpu_baseline": cpu_baseline_potts_sw(cpu_seconds) if cpu_seconds else None,
+        "cpu_baseline": cpu_baseline_potts_sw(cpu_seconds, start) if cpu_seconds else None,
     }
 
 
@@ -478,7 +477,8 @@ def bench_ising2d_64(P, ev, steps, warmup, cpu_seconds=None):
 
     shape = [64, 64]
     n = 4096
-    start = np.random.default_rng(SEED).integers(0, 2, n).astype(np.uint8)
+    # the run config's hot start: the seeded stream's draws (configuration.rs:80-87, ising/run_config.rs:237-251)
+    start = P.hot_start(SEED, 2, n)[0]
     c = P.ising_gpu_chain(shape, 1.0, 0.0, 0.4406868, SEED, start)
     c.advance(warmup)
     c.sync()
@@ -496,7 +496,7 @@ def bench_ising2d_64(P, ev, steps, warmup, cpu_seconds=None):
         "config": {"workload": "2D Ising 64^2, beta=0.4406868, checkerboard Metropolis, hot start, E and |M| "
                                "every sweep", "lattice": shape},
         "mean_energy_per_site": float(e.mean()), "mean_abs_m_per_site": float(m.mean()),
-        "cpu_baseline": cpu_baseline_ising2d_64(cpu_seconds) if cpu_seconds else None,
+        "cpu_baseline": cpu_baseline_ising2d_64(cpu_seconds, start) if cpu_seconds else None,
     }
 
 
