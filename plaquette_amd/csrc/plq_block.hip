@@ -16,8 +16,10 @@
 //
 // The network runs on the matrix cores: the 32 -> 32 layer is a 64 x 32 x 288
 // product per block, done as 72 v_mfma_f32_16x16x32_bf16 with BF16 weights
-// and activations and FP32 accumulation; the 1 -> 32 and 32 -> 1 layers are
-// 288 FP32 FMAs per lane.  The acceptance test uses only IEEE single adds,
+// and activations and FP32 accumulation.  In the pass kernel the 1 -> 32
+// layer is 8 more (K = the 9 taps, zero-padded) and the 32 -> 1 layer 4 more
+// (per position the channel sums of each tap, then 9 shifted adds); training
+// runs those two layers as 288 FP32 FMAs per lane.  The acceptance test uses only IEEE single adds,
 // multiplies and divides with a fixed summation order and the deterministic
 // exponential plq_expf, so given the logits it is reproducible bit for bit
 // on any IEEE machine.
@@ -153,6 +155,82 @@ __device__ __forceinline__ void layer2(const BlockNetView &n, const bf16x8 *wB, 
     }
 }
 
+// The matrix-core forms of layers 1 and 3 in the pass kernel.  Their B
+// fragments (n.w13f) are packed by block_pack_kernel:
+//   layer 1, tile nt (channels 16 nt ..): lane l, element j is W1 at tap
+//     3 (l >> 4) + j, channel 16 nt + (l & 15), for j < 3 and l < 48 (else 0);
+//   layer 3: lane l, element j is W3 at tap l & 15 (below 9, else 0),
+//     channel 8 (l >> 4) + j.
+constexpr int W13F_L3 = 2 * 64 * 8;  // offset of the layer-3 fragments (uint16)
+constexpr int W13F = 3 * 64 * 8;     // uint16 entries of n.w13f
+constexpr int CBS = 17;              // row stride (floats) of layer 3's per-tap sums
+
+// Layer 1 (1 -> 32, 3x3, tanh) on the matrix cores.  A rows are the 16
+// positions of tile mt, K the 9 taps (three per group of 16 lanes, the rest
+// zero), B columns 16 output channels.  The patch values (+-1, 0) are exact
+// in BF16 and the weights are the BF16-rounded W1 of the vector form, so the
+// products are the same; only the order of the FP32 sums differs.  The
+// accumulator starts at the channel's bias.
+__device__ __forceinline__ void layer1_mfma(const BlockNetView &n, const float *patch, int lane, uint16_t *h1) {
+    const int row = lane & 15, kq = lane >> 4, col = lane & 15;
+    const bf16x8 b0 = as_bf16x8(reinterpret_cast<const u32x4 *>(n.w13f)[0 * 64 + lane]);
+    const bf16x8 b1 = as_bf16x8(reinterpret_cast<const u32x4 *>(n.w13f)[1 * 64 + lane]);
+    const float bias0 = n.b1[col], bias1 = n.b1[16 + col];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        const int p = 16 * mt + row, y = p >> 3, x = p & 7;
+        uint32_t lo = 0u, hi = 0u;
+        if (kq < 3) {
+            // taps 3 kq .. 3 kq + 2: patch rows y + kq, columns x .. x + 2
+            const float *pr = &patch[(y + kq) * 10 + x];
+            lo = (uint32_t)bf16_bits(pr[0]) | ((uint32_t)bf16_bits(pr[1]) << 16);
+            hi = (uint32_t)bf16_bits(pr[2]);
+        }
+        const bf16x8 a = as_bf16x8(u32x4{lo, hi, 0u, 0u});
+        f32x4 acc0 = {bias0, bias0, bias0, bias0}, acc1 = {bias1, bias1, bias1, bias1};
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc1, 0, 0, 0);
+        // D: lane holds column (lane & 15), rows 4 * (lane >> 4) + i
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int q = 16 * mt + 4 * kq + i;
+            h1[q * HS + col] = bf16_bits(tanh_fast(acc0[i]));
+            h1[q * HS + 16 + col] = bf16_bits(tanh_fast(acc1[i]));
+        }
+    }
+}
+
+// Layer 3 (32 -> 1, 3x3) on the matrix cores, first half: per position p and
+// tap t the channel sum C[p][t] = sum_c W3[t][c] h2[p][c], one 16 x 16 x 32
+// product per tile of 16 positions (B columns are the taps, 9 of 16 used),
+// into cbuf[p * CBS + t] (a row stride of 17 floats keeps the position-wise
+// reads of the second half on distinct LDS banks).
+__device__ __forceinline__ void layer3_mfma_taps(const BlockNetView &n, const uint16_t *h2, int lane, float *cbuf) {
+    const int row = lane & 15, kb = (lane >> 4) * 8, col = lane & 15;
+    const bf16x8 b = as_bf16x8(reinterpret_cast<const u32x4 *>(n.w13f + W13F_L3)[lane]);
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        const bf16x8 a = lds_frag(&h2[(16 * mt + row) * HS + kb]);
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) cbuf[(16 * mt + 4 * (lane >> 4) + i) * CBS + col] = acc[i];
+    }
+}
+
+// Second half: the logit of the lane's position, the bias plus C[p + tap
+// offset][t] over the taps whose position lies in the block, in tap order.
+__device__ __forceinline__ float layer3_mfma_sum(const BlockNetView &n, const float *cbuf, int lane) {
+    const int y = lane >> 3, x = lane & 7;
+    float s = n.b3[0];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const int yy = y + t / 3 - 1, xx = x + t % 3 - 1;
+        if (inside(yy, xx)) s += cbuf[(yy * B + xx) * CBS + t];
+    }
+    return s;
+}
+
 // Layer 3 (32 -> 1, 3x3): the logit of the lane's position.
 __device__ __forceinline__ float layer3(const BlockNetView &n, const uint16_t *h2, int lane) {
     const int y = lane >> 3, x = lane & 7;
@@ -219,12 +297,17 @@ __global__ __launch_bounds__(256) void block_pass_kernel(uint8_t *__restrict__ c
         cur = site_state(c0, c1, L0, L1, X, Y);
     }
     __syncthreads();
-    if (active) layer1(net, W.patch, lane, W.h1);
+    if (active) layer1_mfma(net, W.patch, lane, W.h1);
     __syncthreads();
     if (active) layer2<true>(net, wB, W.h1, lane, W.h2);
     __syncthreads();
+    // layer 3's per-tap channel sums go where h1 was (h1 is dead after layer 2)
+    static_assert(B * B * CBS * sizeof(float) <= sizeof(W.h1), "layer-3 sums fit in h1");
+    float *cbuf = reinterpret_cast<float *>(W.h1);
+    if (active) layer3_mfma_taps(net, W.h2, lane, cbuf);
+    __syncthreads();
     if (active) {
-        const float l = layer3(net, W.h2, lane);
+        const float l = layer3_mfma_sum(net, cbuf, lane);
         const int64_t b = (int64_t)by * nb0 + bx;
         if (logits_out) logits_out[b * 64 + lane] = l;
         if (mode == 0) {
@@ -516,11 +599,25 @@ __global__ void block_adam_kernel(float *__restrict__ w, float *__restrict__ m, 
 // (w2f[t][nt][lane][j] = W2[t][16 nt + (lane & 15)][8 (lane >> 4) + j]) and
 // for the input gradient (w2t[t][nt][lane][j] = W2[t][8 (lane >> 4) + j][16 nt + (lane & 15)]).
 __global__ void block_pack_kernel(const float *__restrict__ w, float *__restrict__ w1r, float *__restrict__ w3r,
-                                  uint16_t *__restrict__ w2f, uint16_t *__restrict__ w2t) {
+                                  uint16_t *__restrict__ w2f, uint16_t *__restrict__ w2t, uint16_t *__restrict__ w13f) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e < 9 * CH) {
         w1r[e] = bf16_float(bf16_bits(w[BN_OFF_W1 + e]));
         w3r[e] = bf16_float(bf16_bits(w[BN_OFF_W3 + e]));
+    }
+    if (e < W13F) {
+        const int j = e & 7, lane = (e >> 3) & 63;
+        uint16_t v = 0;
+        if (e < W13F_L3) {
+            // layer 1, channel tile nt: tap 3 (lane >> 4) + j, channel 16 nt + (lane & 15)
+            const int nt = e >> 9, kq = lane >> 4, t = 3 * kq + j;
+            if (kq < 3 && j < 3) v = bf16_bits(w[BN_OFF_W1 + t * CH + 16 * nt + (lane & 15)]);
+        } else {
+            // layer 3: tap lane & 15, channel 8 (lane >> 4) + j
+            const int t = lane & 15, ch = 8 * (lane >> 4) + j;
+            if (t < 9) v = bf16_bits(w[BN_OFF_W3 + t * CH + ch]);
+        }
+        w13f[e] = v;
     }
     if (e < 9 * 2 * 64 * 8) {
         const int j = e & 7, lane = (e >> 3) & 63, nt = (e >> 9) & 1, t = e >> 10;
@@ -532,8 +629,9 @@ __global__ void block_pack_kernel(const float *__restrict__ w, float *__restrict
 
 }  // namespace
 
-void launch_block_pack(const float *master, float *w1r, float *w3r, uint16_t *w2f, uint16_t *w2t, hipStream_t s) {
-    block_pack_kernel<<<dim3((9 * 2 * 64 * 8 + 255) / 256), dim3(256), 0, s>>>(master, w1r, w3r, w2f, w2t);
+void launch_block_pack(const float *master, float *w1r, float *w3r, uint16_t *w2f, uint16_t *w2t, uint16_t *w13f,
+                       hipStream_t s) {
+    block_pack_kernel<<<dim3((9 * 2 * 64 * 8 + 255) / 256), dim3(256), 0, s>>>(master, w1r, w3r, w2f, w2t, w13f);
 }
 
 void launch_block_pass(const BlockPassArgs &a) {

@@ -22,6 +22,7 @@ struct plq_blocknet {
     float *w = nullptr, *m = nullptr, *v = nullptr;  // FP32 parameters and Adam moments
     float *w1r = nullptr, *w3r = nullptr;            // BF16-rounded W1, W3
     uint16_t *w2f = nullptr, *w2t = nullptr;         // W2 fragments (forward, input gradient)
+    uint16_t *w13f = nullptr;                        // W1 and W3 fragments of the pass kernel
     float *grad = nullptr;                           // BN_PARAMS + 1
     float *partials = nullptr;                       // per-workgroup gradient sums, then per-run sums
     float *losses = nullptr;                         // per training step
@@ -37,13 +38,14 @@ struct plq_blocknet {
         n.b3 = w + plq::BN_OFF_B3;
         n.w2f = w2f;
         n.w2t = w2t;
+        n.w13f = w13f;
         return n;
     }
 
     ~plq_blocknet() {
         (void)hipSetDevice(device);
         (void)hipDeviceSynchronize();
-        for (void *p : {(void *)w, (void *)m, (void *)v, (void *)w1r, (void *)w3r, (void *)w2f, (void *)w2t,
+        for (void *p : {(void *)w, (void *)m, (void *)v, (void *)w1r, (void *)w3r, (void *)w2f, (void *)w2t, (void *)w13f,
                         (void *)grad, (void *)partials, (void *)losses})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
@@ -114,7 +116,7 @@ int ensure_chain_buffers(plq_chain *c) {
 }
 
 int pack(plq_blocknet *n, hipStream_t s) {
-    plq::launch_block_pack(n->w, n->w1r, n->w3r, n->w2f, n->w2t, s);
+    plq::launch_block_pack(n->w, n->w1r, n->w3r, n->w2f, n->w2t, n->w13f, s);
     return launch_status();
 }
 
@@ -183,6 +185,7 @@ int plq_blocknet_create(uint64_t seed, plq_blocknet **out) {
     if ((rc = alloc((void **)&n->w3r, 9 * plq::BN_CHANNELS * sizeof(float))) != PLQ_OK) return bail(rc);
     if ((rc = alloc((void **)&n->w2f, plq::BLOCK_W2_FRAG * sizeof(uint16_t))) != PLQ_OK) return bail(rc);
     if ((rc = alloc((void **)&n->w2t, plq::BLOCK_W2_FRAG * sizeof(uint16_t))) != PLQ_OK) return bail(rc);
+    if ((rc = alloc((void **)&n->w13f, plq::BLOCK_W13_FRAG * sizeof(uint16_t))) != PLQ_OK) return bail(rc);
     if ((rc = alloc((void **)&n->grad, (P + 1) * sizeof(float))) != PLQ_OK) return bail(rc);
     if ((rc = alloc((void **)&n->partials, (size_t)(plq::BLOCK_TRAIN_GROUPS + plq::BLOCK_TRAIN_GROUPS / 16) *
                                                plq::BLOCK_PART_STRIDE * sizeof(float))) != PLQ_OK)

@@ -136,6 +136,7 @@ constexpr int BLOCK_W2_FRAG = 9 * 2 * 64 * 8;         // bf16 entries of one pac
 struct BlockNetView {
     const float *w1, *b1, *b2, *w3, *b3;
     const uint16_t *w2f, *w2t;
+    const uint16_t *w13f;  // MFMA B fragments of layers 1 and 3 (3 * 64 * 8 BF16)
 };
 
 struct BlockPassArgs {
@@ -158,7 +159,9 @@ struct BlockGradArgs {
     float *grad;          // BN_PARAMS + 1: mean gradient, then the mean loss
     hipStream_t stream;
 };
-void launch_block_pack(const float *master, float *w1r, float *w3r, uint16_t *w2f, uint16_t *w2t, hipStream_t s);
+void launch_block_pack(const float *master, float *w1r, float *w3r, uint16_t *w2f, uint16_t *w2t, uint16_t *w13f,
+                       hipStream_t s);
+constexpr int BLOCK_W13_FRAG = 3 * 64 * 8;  // uint16 entries of BlockNetView::w13f
 void launch_block_pass(const BlockPassArgs &a);
 int block_pass_groups(int L0, int L1);  // workgroups of a mode-0 pass
 int block_train_groups(int L0, int L1);
