@@ -184,6 +184,8 @@ MIXED_CASES = [
     ((32, 32), BETA_C, 1.0, 0.0, 1, O.KERNEL_METROPOLIS),
     ((64, 48), 0.35, 0.8, 0.1, 2, O.KERNEL_HEAT_BATH),
     ((32, 64), 0.6, 1.0, -0.15, 3, O.KERNEL_METROPOLIS),
+    # config 5's lattice: 4,096 blocks, the uniform-parity colour passes
+    ((512, 512), BETA_C, 1.0, 0.0, 1, O.KERNEL_METROPOLIS),
 ]
 
 
