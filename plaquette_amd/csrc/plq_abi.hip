@@ -445,6 +445,13 @@ int enqueue_cluster_sweep(plq_chain *c) {
         a.gate = &c->d_wolff->gate;
         a.gate_skip = 0;
     }
+    // A recorded Swendsen-Wang sweep counts its record in the relabel, into
+    // the slot rec_slot gives below (rec_count does not change before then).
+    const bool fused = record_next && c->relabel_measures && plq::cluster_relabel_measures(a);
+    if (fused) {
+        a.meas_part = c->d_partials;
+        a.meas_rec = c->d_rec + (c->rec_count % c->rec_cap) * plq::REC_K;
+    }
     c->last_relabel_blocks = plq::launch_cluster_sweep(a);
     c->have_partition = true;
     c->grow_pending = grow;
@@ -452,7 +459,8 @@ int enqueue_cluster_sweep(plq_chain *c) {
     c->sweeps_done += 1;
     bool record = c->rec_every && (c->sweeps_done % c->rec_every == 0);
     if (record) {
-        enqueue_measure(c, rec_slot(c));
+        long long *rec = rec_slot(c);
+        if (!fused) enqueue_measure(c, rec);
         if (c->rec_clusters) {
             const uint64_t slot = (c->rec_count - 1) % c->rec_cap;
             PLQ_TRY(enqueue_cluster_stats(c, c->d_clu + slot * CLU_SLOT));
@@ -763,6 +771,10 @@ int plq_chain_create_cluster(const plq_cluster_desc *d, const uint8_t *start, si
     c->small = ns <= plq::small_cluster_max_sites(d->dim);
     if (const char *sm = getenv("PLQ_SMALL_CLUSTER"))
         if (atoi(sm) == 0) c->small = false;
+    // PLQ_RELABEL_MEASURE=0 records SW sweeps with a separate measurement
+    // pass (tests compare the two).
+    if (const char *rm = getenv("PLQ_RELABEL_MEASURE"))
+        if (atoi(rm) == 0) c->relabel_measures = false;
     // Wolff moves grown from the seed (plq_wolff.hip): adaptive by default
     // on lattices of 2^22 sites or more, where a decomposition costs several
     // times the fixed cost of a grown move; PLQ_WOLFF_GROW=0/1/2 (off,

@@ -61,6 +61,7 @@ struct plq_chain {
     uint64_t pass_cap = 0;
     int64_t last_relabel_blocks = 0;
     bool have_partition = false;
+    bool relabel_measures = true;          // a recorded SW sweep's relabel also counts its record
     // Wolff moves grown from the seed (plq_wolff.hip)
     int wolff_mode = 0;                    // 0 off, 1 adaptive, 2 an attempt every sweep
     uint32_t wolff_levels = 0, wolff_qcap = 0;

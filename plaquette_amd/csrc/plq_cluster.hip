@@ -253,6 +253,34 @@ struct X16Box {
     }
 };
 
+// Membership of a global site in a box of power-of-two extents, with few
+// scalar operands: the coordinate bits above the box extents (mask out)
+// are the box origin's (g0) exactly for the box's sites, and the box-local
+// index gathers each coordinate's low bits (shifted right by sh[mu], masked
+// by msk[mu]).
+template <int D>
+struct BoxMember {
+    uint32_t g0, out, sh[MAX_D], msk[MAX_D];
+    __device__ __forceinline__ BoxMember(const CCGeom &cg, const X16Box<D> &bx) {
+        g0 = 0;
+        out = 0;
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) {
+            g0 |= bx.o[mu] << bx.Lsh[mu];
+            out |= (((uint32_t)cg.L[mu] - 1u) & ~(bx.e[mu] - 1u)) << bx.Lsh[mu];
+            sh[mu] = bx.Lsh[mu] - bx.lsh[mu];
+            msk[mu] = (bx.e[mu] - 1u) << bx.lsh[mu];
+        }
+    }
+    // whether site p lies in the box, and its box-local index if so
+    __device__ __forceinline__ bool local(uint32_t p, uint32_t &l) const {
+        l = 0;
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) l |= (p >> sh[mu]) & msk[mu];
+        return ((p ^ g0) & out) == 0u;
+    }
+};
+
 // The box labelling of cc_local, sixteen sites per lane: lane c owns the
 // box-local indices 16c .. 16c+15, one segment of an axis-0 row.  `me` holds
 // the lane's 16 labels; up(l, g) returns the 16 labels at box-local index l
@@ -740,11 +768,14 @@ __device__ __forceinline__ void box_load_parents(int32_t *P, const int32_t *__re
 }
 
 // The new labels of a lane's 16 sites (w, in place), the roots found as
-// the sites are visited; returns the relabelled sites.
+// the sites are visited; returns the relabelled sites.  hops collects, as
+// bits, the lengths of the walks taken (1: the parent is the root, 2: its
+// parent is); the caller checks the longest against the hop budget once
+// (an integer, not a lane mask held through the loop).
 template <int D>
-__device__ __forceinline__ int box_relabel16(const int32_t *P, const CCGeom &cg, const X16Box<D> &bx, uint32_t l0,
+__device__ __forceinline__ int box_relabel16(const int32_t *P, const BoxMember<D> &bm, uint32_t l0,
                                              uint32_t (&w)[4], uint32_t root_seed, uint32_t skey, int q, int seeded,
-                                             int forced, int32_t seed_root, uint32_t cap, bool &over) {
+                                             int forced, int32_t seed_root, uint32_t &hops) {
     int32_t prev = -1;
     uint32_t nl = NO_LABEL;
     int cnt = 0;
@@ -755,16 +786,9 @@ __device__ __forceinline__ int box_relabel16(const int32_t *P, const CCGeom &cg,
         if (p != prev) {
             prev = p;
             // p inside the box: its parent, from LDS; else p is a root
-            uint32_t local = 0;
-            bool inside = true;
-#pragma unroll
-            for (int mu = 0; mu < D; ++mu) {
-                const uint32_t y = (((uint32_t)p >> bx.Lsh[mu]) & ((uint32_t)cg.L[mu] - 1u)) - bx.o[mu];
-                inside = inside && y < bx.e[mu];
-                local |= y << bx.lsh[mu];
-            }
-            const int32_t root = inside ? P[local] : p;
-            over = over || (root != p ? 2u : 1u) > cap;
+            uint32_t local;
+            const int32_t root = bm.local((uint32_t)p, local) ? P[local] : p;
+            hops |= root != p ? 2u : 1u;
             const uint32_t lab = (w[k >> 2] >> (8 * (k & 3))) & 0xffu;
             nl = (seeded && root != seed_root) ? NO_LABEL
                                                : relabel_draw(keyed_bits_k(root_seed, (uint32_t)root, skey), lab, q, forced);
@@ -778,11 +802,15 @@ __device__ __forceinline__ int box_relabel16(const int32_t *P, const CCGeom &cg,
     return cnt;
 }
 
+// Whether the walks box_relabel16 / box_roots16 recorded in hops exceed
+// the hop budget.
+__device__ __forceinline__ bool hops_over(uint32_t hops, uint32_t cap) { return (hops >= 2u ? 2u : hops) > cap; }
+
 // The cluster roots of a lane's 16 sites, kept for the counting that
 // follows (the relabel-and-count and sizes kernels).
 template <int D>
-__device__ __forceinline__ void box_roots16(const int32_t *P, const CCGeom &cg, const X16Box<D> &bx, uint32_t l0,
-                                            int32_t (&root)[16], uint32_t cap, bool &over) {
+__device__ __forceinline__ void box_roots16(const int32_t *P, const BoxMember<D> &bm, uint32_t l0,
+                                            int32_t (&root)[16], uint32_t &hops) {
     int32_t prev = -1, r = -1;
     int4 q4;
 #pragma unroll
@@ -791,16 +819,9 @@ __device__ __forceinline__ void box_roots16(const int32_t *P, const CCGeom &cg, 
         if (p != prev) {
             prev = p;
             // p inside the box: its parent, from LDS; else p is a root
-            uint32_t local = 0;
-            bool inside = true;
-#pragma unroll
-            for (int mu = 0; mu < D; ++mu) {
-                const uint32_t y = (((uint32_t)p >> bx.Lsh[mu]) & ((uint32_t)cg.L[mu] - 1u)) - bx.o[mu];
-                inside = inside && y < bx.e[mu];
-                local |= y << bx.lsh[mu];
-            }
-            r = inside ? P[local] : p;
-            over = over || (r != p ? 2u : 1u) > cap;
+            uint32_t local;
+            r = bm.local((uint32_t)p, local) ? P[local] : p;
+            hops |= r != p ? 2u : 1u;
         }
         root[k] = r;
     }
@@ -867,7 +888,7 @@ __global__ __launch_bounds__(1024, 8) void cc_relabel_box(uint8_t *__restrict__ 
     const uint32_t c = threadIdx.x;
     const uint32_t l0 = c << 4;
     long long moved = 0;
-    bool over = false;
+    uint32_t hops = 0;
     for (int64_t box = blockIdx.x; box < cg.nboxes; box += gridDim.x) {
         const X16Box<D> bx(cg, (uint32_t)box);
         const bool act = c < (bx.nloc >> 4);
@@ -879,13 +900,189 @@ __global__ __launch_bounds__(1024, 8) void cc_relabel_box(uint8_t *__restrict__ 
             uint4 *dst = reinterpret_cast<uint4 *>(labels + gsite0);
             const uint4 l4 = *dst;
             uint32_t w[4] = {l4.x, l4.y, l4.z, l4.w};
-            const int cnt = box_relabel16<D>(P, cg, bx, l0, w, root_seed, skey, q, seeded, forced, seed_root, cap, over);
+            const int cnt =
+                box_relabel16<D>(P, BoxMember<D>(cg, bx), l0, w, root_seed, skey, q, seeded, forced, seed_root, hops);
             if (cnt) *dst = make_uint4(w[0], w[1], w[2], w[3]);
             moved += cnt;
         }
     }
-    if (over) atomicOr(flag, 1);
+    if (hops_over(hops, cap)) atomicOr(flag, 1);
     box_partials(moved, wsum, partials);
+}
+
+// Bytes of the halo of forward neighbours outside a box that
+// cc_relabel_box_meas stages in LDS (at most one per lane, and at most
+// 3 nloc, so labels and halo fit the nloc parents' LDS).
+constexpr int RELABEL_HALO_MAX = 1024;
+
+// cc_relabel_box for a Swendsen-Wang sweep (every cluster redrawn, no seed
+// cluster, no forced change, no gate), with the Potts energy sums of the new
+// labels (potts_measure_vec's columns: agreeing forward bonds, label counts)
+// counted in the same pass, so a recorded sweep reads its labels once.
+// After the relabel the box's new labels (bytes 0 .. nloc) and the halo
+// (from byte nloc) replace its parents in LDS.  A forward neighbour outside
+// the box belongs to another workgroup's box, which may be relabelling it
+// now, but its new label is fixed: its cluster root is parent[parent[s]]
+// (the forest after cc_compress, which no kernel of the relabel writes) and
+// the redraw is keyed on the root alone.  Those neighbours form the halo: per
+// axis mu >= 1 with the box narrower than the lattice, the face beyond the
+// box's last layer (nloc / e[mu] bytes, axis 0 contiguous inside it), then
+// axis 0's (nloc / e[0] bytes, one per box row).  Partials per block: the
+// relabel's row [0, moved, 0 ...] in partials, and the measure's columns
+// [agree, 0, counts[0..q)] column-major in mpart (column k of block b at
+// mpart[k * gridDim.x + b]).
+//
+// Halo index h: the global site of its neighbour, and its byte in LDS.
+template <int D>
+__device__ __forceinline__ uint32_t halo_site(const CCGeom &cg, const X16Box<D> &bx, uint32_t h) {
+    uint32_t off = 0;
+#pragma unroll
+    for (int k = 1; k <= D; ++k) {
+        const int mu = k < D ? k : 0;  // axes 1 .. D-1, then axis 0
+        if (bx.e[mu] == (uint32_t)cg.L[mu]) continue;
+        const uint32_t lg = (uint32_t)(31 - __clz((int)bx.e[mu]));
+        const uint32_t n = bx.nloc >> lg;
+        if (h < off + n) {
+            const uint32_t f = h - off;
+            const uint32_t lo = f & ((1u << bx.lsh[mu]) - 1u);
+            const uint32_t l = lo | ((bx.e[mu] - 1u) << bx.lsh[mu]) | ((f >> bx.lsh[mu]) << (bx.lsh[mu] + lg));
+            const uint32_t mu_out = (bx.o[mu] + bx.e[mu]) & ((uint32_t)cg.L[mu] - 1u);
+            return bx.locate(cg, l) + (mu_out - (bx.o[mu] + bx.e[mu] - 1u)) * (uint32_t)cg.stride[mu];
+        }
+        off += n;
+    }
+    return 0;
+}
+
+// LDS byte offsets of the forward neighbours lane c's segment (box-local
+// l0 .. l0 + 15) compares with: nx the byte after the segment along axis 0
+// (the next segment's first, the row's first when the box is as wide as the
+// lattice, or the row's halo byte), up[mu] the 16 bytes one step along axis
+// mu >= 1 (16-byte aligned).
+template <int D>
+__device__ __forceinline__ void meas_offsets(const CCGeom &cg, const X16Box<D> &bx, uint32_t l0, uint32_t &nx,
+                                             uint32_t (&up)[MAX_D]) {
+    uint32_t hoff = bx.nloc;
+#pragma unroll
+    for (int mu = 1; mu < D; ++mu) {
+        const uint32_t lg = (uint32_t)(31 - __clz((int)bx.e[mu]));
+        const uint32_t y = (l0 >> bx.lsh[mu]) & (bx.e[mu] - 1u);
+        if (y + 1u < bx.e[mu])
+            up[mu] = l0 + (1u << bx.lsh[mu]);
+        else if (bx.e[mu] == (uint32_t)cg.L[mu])
+            up[mu] = l0 - (y << bx.lsh[mu]);
+        else
+            up[mu] = hoff + ((l0 & ((1u << bx.lsh[mu]) - 1u)) | ((l0 >> (bx.lsh[mu] + lg)) << bx.lsh[mu]));
+        if (bx.e[mu] != (uint32_t)cg.L[mu]) hoff += bx.nloc >> lg;
+    }
+    const uint32_t lg0 = (uint32_t)(31 - __clz((int)bx.e[0]));
+    const uint32_t x0 = l0 & (bx.e[0] - 1u);
+    nx = x0 + 16u < bx.e[0] ? l0 + 16u : bx.e[0] == (uint32_t)cg.L[0] ? l0 + 16u - bx.e[0] : hoff + (l0 >> lg0);
+}
+
+template <int D>
+__global__ __launch_bounds__(1024, 8) void cc_relabel_box_meas(uint8_t *__restrict__ labels,
+                                                            const int32_t *__restrict__ parent, CCGeom cg,
+                                                            uint32_t root_seed, uint32_t skey, int q, uint32_t cap,
+                                                            int *__restrict__ flag, long long *__restrict__ partials,
+                                                            long long *__restrict__ mpart, uint32_t nhalo) {
+    extern __shared__ int4 box_lds[];
+    int32_t *P = reinterpret_cast<int32_t *>(box_lds);
+    uint8_t *R = reinterpret_cast<uint8_t *>(box_lds);  // after the relabel: new labels, then the halo
+    __shared__ int wsum[16][3 + MAX_Q_DEV];
+    const uint32_t c = threadIdx.x;
+    const uint32_t l0 = c << 4;
+    const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u);
+    int moved = 0, agree = 0;
+    if (lane == 0)
+        for (int a = 0; a < q; ++a) wsum[wave][3 + a] = 0;  // each wave's label counts, summed over its boxes
+    uint32_t hops = 0;
+    for (int64_t box = blockIdx.x; box < cg.nboxes; box += gridDim.x) {
+        // the box geometry is dead after the relabel: the count uses the
+        // LDS offsets computed here
+        const X16Box<D> bx(cg, (uint32_t)box);
+        const bool act = c < (bx.nloc >> 4);
+        const uint32_t gsite0 = bx.locate(cg, l0);
+        // lanes past the halo load their own first site's parents instead:
+        // no mask held over the relabel
+        const uint32_t hs = c < nhalo ? halo_site<D>(cg, bx, c) : gsite0;
+        const uint32_t hb = bx.nloc + c;
+        uint32_t nx = 0, up[MAX_D] = {};
+        meas_offsets<D>(cg, bx, l0, nx, up);
+        __syncthreads();  // the previous box's readers are done with the LDS
+        // the halo site's parent beside the box's parents, its root during
+        // the relabel (a barrier waits for every load in flight)
+        const int32_t hp = parent[hs];
+        if (act) box_load_parents<D>(P, parent, l0, gsite0);
+        __syncthreads();
+        const int32_t hr = parent[hp];
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+        if (act) {
+            uint4 *dst = reinterpret_cast<uint4 *>(labels + gsite0);
+            const uint4 l4 = *dst;
+            w[0] = l4.x;
+            w[1] = l4.y;
+            w[2] = l4.z;
+            w[3] = l4.w;
+            moved += box_relabel16<D>(P, BoxMember<D>(cg, bx), l0, w, root_seed, skey, q, 0, 0, -1, hops);
+            *dst = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+        const uint32_t hl = relabel_draw(keyed_bits_k(root_seed, (uint32_t)hr, skey), 0u, q, 0);
+        __syncthreads();  // every lane is done with P
+        if (act) reinterpret_cast<uint4 *>(box_lds)[c] = make_uint4(w[0], w[1], w[2], w[3]);
+        if (c < nhalo) R[hb] = (uint8_t)hl;
+        __syncthreads();
+        if (act) {
+            const uint32_t n0 = R[nx];
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                const uint32_t nxt = d < 3 ? w[d < 3 ? d + 1 : 0] : n0;
+                agree += zero_bytes(w[d] ^ ((w[d] >> 8) | (nxt << 24)));
+            }
+#pragma unroll
+            for (int mu = 1; mu < D; ++mu) {
+                const uint4 u = *reinterpret_cast<const uint4 *>(R + up[mu]);
+                agree += zero_bytes(w[0] ^ u.x) + zero_bytes(w[1] ^ u.y) + zero_bytes(w[2] ^ u.z) +
+                         zero_bytes(w[3] ^ u.w);
+            }
+        }
+        // label counts of the box: a wave sum per label, kept by lane 0 (a
+        // loop that is not unrolled: unrolled, the per-label conditions
+        // took scalar registers from the relabel)
+#pragma unroll 1
+        for (int a = 0; a < q; ++a) {
+            int n = 0;
+            if (act)
+#pragma unroll
+                for (int d = 0; d < 4; ++d) n += zero_bytes(w[d] ^ (0x01010101u * (uint32_t)a));
+            const int s = wave_sum_i32(n);
+            if (lane == 0) wsum[wave][3 + a] += s;
+        }
+    }
+    if (hops_over(hops, cap)) atomicOr(flag, 1);
+    // block sums: value 0 the moved sites, 1 + k measure column k
+    const int nv = 3 + q;
+    {
+        const int s0 = wave_sum_i32(moved), s1 = wave_sum_i32(agree);
+        if (lane == 0) {
+            wsum[wave][0] = s0;
+            wsum[wave][1] = s1;
+            wsum[wave][2] = 0;
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nv) {
+        long long t = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += wsum[i][threadIdx.x];
+        if (threadIdx.x == 0) {
+            long long *dst = partials + (int64_t)blockIdx.x * REC_K;
+            dst[0] = 0;
+            dst[1] = t;
+            for (int k = 2; k < REC_K; ++k) dst[k] = 0;
+        } else {
+            mpart[(int64_t)(threadIdx.x - 1) * gridDim.x + blockIdx.x] = t;
+        }
+    }
 }
 
 
@@ -1383,7 +1580,7 @@ __global__ __launch_bounds__(1024, 8) void cc_sizes_box(const int32_t *__restric
     const uint32_t c = threadIdx.x;
     const uint32_t l0 = c << 4;
     LaneHist lh;
-    bool over = false;
+    uint32_t hops = 0;  // not checked: the partition is final
     for (int64_t box = blockIdx.x; box < cg.nboxes; box += gridDim.x) {
         const X16Box<D> bx(cg, (uint32_t)box);
         const bool act = c < (bx.nloc >> 4);
@@ -1392,7 +1589,7 @@ __global__ __launch_bounds__(1024, 8) void cc_sizes_box(const int32_t *__restric
         if (act) box_load_parents<D>(P, parent, l0, gsite0);
         __syncthreads();
         int32_t root[16];
-        if (act) box_roots16<D>(P, cg, bx, l0, root, 0xffffffffu, over);
+        if (act) box_roots16<D>(P, BoxMember<D>(cg, bx), l0, root, hops);
         box_count16<D>(P, okey, ocnt, cg, bx, act, l0, gsite0, root, sizes, ext, cross, tag, lh, hb);
     }
     lh.flush(hb, &nclu, &largest);
@@ -1434,7 +1631,7 @@ __global__ __launch_bounds__(1024, 8) void cc_relabel_count(uint8_t *__restrict_
     const uint32_t l0 = c << 4;
     LaneHist lh;
     long long moved = 0;
-    bool over = false;
+    uint32_t hops = 0;
     for (int64_t box = blockIdx.x; box < cg.nboxes; box += gridDim.x) {
         const X16Box<D> bx(cg, (uint32_t)box);
         const bool act = c < (bx.nloc >> 4);
@@ -1444,7 +1641,7 @@ __global__ __launch_bounds__(1024, 8) void cc_relabel_count(uint8_t *__restrict_
         __syncthreads();
         int32_t root[16];
         if (act) {
-            box_roots16<D>(P, cg, bx, l0, root, cap, over);
+            box_roots16<D>(P, BoxMember<D>(cg, bx), l0, root, hops);
             uint4 *dst = reinterpret_cast<uint4 *>(labels + gsite0);
             const uint4 l4 = *dst;
             uint32_t w[4] = {l4.x, l4.y, l4.z, l4.w};
@@ -1454,7 +1651,7 @@ __global__ __launch_bounds__(1024, 8) void cc_relabel_count(uint8_t *__restrict_
         }
         box_count16<D>(P, okey, ocnt, cg, bx, act, l0, gsite0, root, sizes, ext, cross, tag, lh, hb);
     }
-    if (over) atomicOr(flag, 1);
+    if (hops_over(hops, cap)) atomicOr(flag, 1);
     lh.flush(hb, &nclu, &largest);
     __syncthreads();
     hist_row(hb, nclu, largest, hist, blockIdx.x);
@@ -1556,6 +1753,15 @@ static int64_t face_sites(const CCGeom &cg) {
     return total;
 }
 
+// Halo bytes of cc_relabel_box_meas: the faces beyond each box along the
+// axes where the box is narrower than the lattice.
+static uint32_t relabel_halo_sites(const CCGeom &cg) {
+    uint32_t n = 0;
+    for (int mu = 0; mu < cg.D; ++mu)
+        if (cg.B[mu] < cg.L[mu]) n += (uint32_t)(cg.box_sites / cg.B[mu]);
+    return n;
+}
+
 int64_t launch_cluster_sweep(const ClusterSweepArgs &a) {
     const CCGeom &cg = a.cg;
     uint32_t skey = a.sweep * GOLDEN;
@@ -1611,6 +1817,23 @@ int64_t launch_cluster_sweep(const ClusterSweepArgs &a) {
         }
         return Gb;
     }
+    if (x16 && a.meas_rec && cluster_relabel_measures(a)) {
+        const int64_t Gb = cg.nboxes < a.max_blocks ? cg.nboxes : a.max_blocks;
+        const uint32_t nhalo = relabel_halo_sites(cg);
+        switch (cg.D) {
+#define PLQ_RM(DD)                                                                                              \
+    cc_relabel_box_meas<DD><<<dim3((uint32_t)Gb), bl, lds, a.stream>>>(a.labels, a.parent, cg, a.seed32 ^ ROOT_SALT, \
+                                                                     skey, a.q, a.cap, a.flag, a.partials,        \
+                                                                     a.meas_part, nhalo)
+            case 1: PLQ_RM(1); break;
+            case 2: PLQ_RM(2); break;
+            case 3: PLQ_RM(3); break;
+            default: PLQ_RM(4); break;
+#undef PLQ_RM
+        }
+        launch_finalize_columns(a.meas_part, Gb, nullptr, 0, 2 + a.q, a.meas_rec, a.stream);
+        return Gb;
+    }
     if (x16) {
         const int64_t Gb = cg.nboxes < a.max_blocks ? cg.nboxes : a.max_blocks;
         switch (cg.D) {
@@ -1634,6 +1857,13 @@ int64_t launch_cluster_sweep(const ClusterSweepArgs &a) {
                                                               a.forced, a.cap, a.flag, a.partials, span4, a.gate,
                                                               a.gate_skip);
     return G;
+}
+
+bool cluster_relabel_measures(const ClusterSweepArgs &a) {
+    const uint32_t nhalo = relabel_halo_sites(a.cg);
+    return !a.label_only && !a.count_sizes && !a.seeded && !a.forced && a.gate == nullptr && a.q >= 1 &&
+           a.q <= MAX_Q_DEV && cluster_boxes_x16(a.cg, a.labels) && nhalo <= (uint32_t)RELABEL_HALO_MAX &&
+           nhalo <= 3u * (uint32_t)a.cg.box_sites;
 }
 
 int64_t cluster_relabel_blocks(const ClusterSweepArgs &a) {

@@ -254,6 +254,12 @@ struct ClusterSweepArgs {
     const int *gate;
     int gate_skip;
     bool label_only;      // build the partition only: no relabel, no partials
+    // With meas_rec non-null and cluster_relabel_measures(a), the relabel
+    // also counts the Potts record of the new labels (potts_measure_vec's
+    // columns) into meas_part (REC_K * max_blocks entries), folded into
+    // meas_rec after it, so no separate measurement is needed.
+    long long *meas_part;
+    long long *meas_rec;
     hipStream_t stream;
 };
 // Enqueue one cluster sweep; returns the number of relabel blocks whose
@@ -261,6 +267,10 @@ struct ClusterSweepArgs {
 int64_t launch_cluster_sweep(const ClusterSweepArgs &a);
 // The number of relabel blocks launch_cluster_sweep gives the same arguments.
 int64_t cluster_relabel_blocks(const ClusterSweepArgs &a);
+// Whether the relabel of a sweep with these arguments can count the sweep's
+// Potts record itself (the per-box relabel of a Swendsen-Wang sweep: no seed
+// cluster, no forced change, no gate, no size counting; q <= MAX_Q_DEV).
+bool cluster_relabel_measures(const ClusterSweepArgs &a);
 
 // Wolff moves grown from the seed site (plq_wolff.hip).  One workgroup
 // grows the seed's cluster level by level over the sweep's keyed bonds and,

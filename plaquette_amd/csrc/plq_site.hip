@@ -118,9 +118,18 @@ __global__ __launch_bounds__(256) void finalize_columns_kernel(const long long *
         if (threadIdx.x == 0) rec[k] = 0;
         return;
     }
-    long long v = 0;
-    for (int64_t b = threadIdx.x; b < na; b += blockDim.x) v += pa[k * na + b];
-    for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) v += pb[k * nb + b];
+    // eight loads in flight per lane: a loop of dependent steps would wait
+    // out each load's latency in turn
+    auto col = [](const long long *__restrict__ src, int64_t n) -> long long {
+        long long acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        int64_t b = threadIdx.x;
+        for (; b + 7 * 256 < n; b += 8 * 256)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc[u] += src[b + u * 256];
+        for (; b < n; b += 256) acc[0] += src[b];
+        return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    };
+    const long long v = col(pa + k * na, na) + col(pb + k * nb, nb);
     long long t = block_sum(v, lds);
     if (threadIdx.x == 0) rec[k] = t;
 }

@@ -460,6 +460,46 @@ def test_cluster_advance_of_many_sweeps_is_bit_exact(gpu, case):
     c.close()
 
 
+RELABEL_MEASURE_CASES = [
+    # shape, q, beta, sweeps
+    ([256, 256], 3, 1.0050525, 6),    # 4 boxes of 128x128: halo rows and columns
+    ([32, 32, 32], 2, 0.2216546, 5),  # boxes of 32x32x16: the halo of axis 2 only
+    ([16, 2048], 2, 0.4406868, 5),    # boxes as wide as the lattice: axis 0 wraps inside the box
+    ([65536], 3, 2.0, 5),             # four 1D boxes: one halo byte each
+    ([16, 8, 8, 16], 3, 0.9, 5),      # one box spanning every axis: no halo
+    ([512, 256], 16, 1.2, 4),         # q = 16, every count column
+]
+
+
+@pytest.mark.parametrize("shape,q,beta,sweeps", RELABEL_MEASURE_CASES,
+                         ids=["x".join(map(str, c[0])) + f"-q{c[1]}" for c in RELABEL_MEASURE_CASES])
+def test_sw_record_counted_in_the_relabel(gpu, monkeypatch, shape, q, beta, sweeps):
+    # A recorded Swendsen-Wang sweep counts its record in the per-box
+    # relabel, reading the new labels of forward neighbours in other boxes
+    # from their cluster roots; PLQ_RELABEL_MEASURE=0 records with the
+    # separate measurement pass.  Both series, sweep by sweep, against the
+    # oracle's sums, and the labels against each other.
+    n = int(np.prod(shape))
+    start = start_config(n, q, True, 17)
+    outs = []
+    for fused in ("1", "0"):
+        monkeypatch.setenv("PLQ_RELABEL_MEASURE", fused)
+        c = gpu.GpuClusterChain.swendsen_wang(shape, q, 1.0, beta, 23, start)
+        c.record(1, sweeps)
+        c.advance(sweeps)
+        outs.append(([(o.sweep, o.bond_sum, tuple(o.counts[a] for a in range(q))) for o in c.series()], c.read()))
+        c.close()
+    assert outs[0][0] == outs[1][0]
+    assert np.array_equal(outs[0][1], outs[1][1])
+    d = O.cluster_desc(shape, q=q, beta=beta, gap=1.0, seed=23)
+    st = start
+    for t in range(sweeps):
+        st, _ = O.cluster_sweeps(d, st, t, 1)
+        agree, counts = O.potts_sums(shape, q, st)
+        assert outs[0][0][t][1:] == (agree, tuple(counts)), f"sweep {t}"
+    assert np.array_equal(outs[0][1], st)
+
+
 @pytest.mark.parametrize("shape,q,seeded", [([256, 256], 3, False), ([32, 32, 32], 2, True), ([16, 2048], 2, False),
                                             ([90, 91], 3, False)])
 def test_cluster_stats_after_every_sweep(gpu, shape, q, seeded):
@@ -653,22 +693,23 @@ def test_cpu_stream_state_continues_in_a_new_chain(gpu):
 
 # -- the configurations at their full sizes ------------------------------------------
 
-@pytest.mark.parametrize("start_kind", ["cold", "hot"])
-def test_potts_sw_at_4096_squared_is_bit_exact(gpu, start_kind):
+@pytest.mark.parametrize("start_kind,stats", [("cold", True), ("hot", True), ("hot", False)])
+def test_potts_sw_at_4096_squared_is_bit_exact(gpu, start_kind, stats):
     # q = 3 Swendsen-Wang at beta_c on the full 4096^2 lattice (1,024 boxes
     # of 128x128).  From a cold start the first sweeps have one percolating
     # cluster whose root every box's boundary unions meet.  Labels, the last
-    # partition, the recorded observables and the recorded cluster
-    # statistics of every sweep against the oracle.
+    # partition, the recorded observables and (with stats) the recorded
+    # cluster statistics of every sweep against the oracle.  Without the
+    # statistics the relabel counts each record itself.
     shape, q, beta = [4096, 4096], 3, 1.0050525
     n = 4096 * 4096
     start = start_config(n, q, start_kind == "hot", 5)
     k = 3
     c = gpu.GpuClusterChain.new(shape, q, 1.0, gpu.Extent.ALL, gpu.Relabel.REDRAW, beta, 2026, start)
     c.record(1, k)
-    c.record_clusters(True)
+    c.record_clusters(stats)
     c.advance(k)
-    got_series, got_stats = c.series(), c.cluster_series()
+    got_series, got_stats = c.series(), (c.cluster_series() if stats else None)
     got, got_roots = c.read(), c.roots()
     c.close()
     d = O.cluster_desc(shape, q=q, beta=beta, gap=1.0, seed=2026)
@@ -681,6 +722,8 @@ def test_potts_sw_at_4096_squared_is_bit_exact(gpu, start_kind):
         sizes = np.bincount(roots)
         sizes = sizes[sizes > 0]
         largest = max(largest, int(sizes.max()))
+        if not stats:
+            continue
         s = got_stats[t]
         assert s.n_clusters == sizes.size and s.largest == sizes.max(), t
         hist = np.bincount(np.floor(np.log2(sizes)).astype(int), minlength=40)
