@@ -205,16 +205,36 @@ plq::SitePassArgs site_args(plq_chain *c, int color, uint32_t sweep, int mode, i
     return a;
 }
 
+// Enqueue the fold of a record left pending by the last sweep, if any.
+void flush_fold(plq_chain *c) {
+    const plq::RecordFold f = c->fold;
+    if (!f.rec) return;
+    plq::launch_finalize_columns(f.pa, f.na, f.pb, f.nb, f.nk, f.rec, c->stream);
+    c->fold = plq::RecordFold{};
+}
+
 // Enqueue the two colour passes of one sweep (mode 2: measure only, no
-// update); with rec non-null, fold their partial sums into rec.
+// update); with rec non-null, fold their partial sums into rec.  A pending
+// fold goes to the colour-0 pass (launch_site_pass runs it first where that
+// pass cannot take it); between direct sweeps (fold_defer) this sweep's own
+// fold waits for the next sweep.
 void enqueue_site_passes(plq_chain *c, uint32_t sweep, int mode, long long *rec) {
     plq::SitePassArgs a0 = site_args(c, 0, sweep, mode, rec != nullptr);
     plq::SitePassArgs a1 = site_args(c, 1, sweep, mode, rec != nullptr);
+    a0.m.fold = c->fold;
+    c->fold = plq::RecordFold{};
     plq::launch_site_pass(a0);
     plq::launch_site_pass(a1);
-    if (rec)
-        plq::launch_finalize_columns(a0.m.part, plq::site_pass_blocks(a0), a1.m.part, plq::site_pass_blocks(a1),
-                                     2 + c->q, rec, c->stream);
+    if (!rec) return;
+    plq::RecordFold f{};
+    f.pa = a0.m.part;
+    f.na = plq::site_pass_blocks(a0);
+    f.pb = a1.m.part;
+    f.nb = plq::site_pass_blocks(a1);
+    f.nk = 2 + c->q;
+    f.rec = rec;
+    c->fold = f;
+    if (!c->fold_defer) flush_fold(c);
 }
 
 // Enqueue a measurement of the current site field into rec (REC_K entries).
@@ -451,8 +471,13 @@ int enqueue_cluster_sweep(plq_chain *c) {
     if (fused) {
         a.meas_part = c->d_partials;
         a.meas_rec = c->d_rec + (c->rec_count % c->rec_cap) * plq::REC_K;
+        a.defer_fold = c->fold_defer;
     }
+    // a pending fold goes to this sweep's first kernel
+    a.fold = c->fold;
+    c->fold = plq::RecordFold{};
     c->last_relabel_blocks = plq::launch_cluster_sweep(a);
+    if (fused && a.defer_fold) c->fold = plq::cluster_record_fold(a, c->last_relabel_blocks);
     c->have_partition = true;
     c->grow_pending = grow;
     c->grow_sweep = a.sweep;
@@ -999,11 +1024,15 @@ int enqueue_sweeps(plq_chain *c, uint64_t sweeps) {
     if (c->kind == KIND_CLUSTER && c->small && !c->rec_clusters && c->pass_cap >= (uint64_t)c->n_sites + 64 &&
         sweeps > 0)
         return enqueue_small_cluster_sweeps(c, sweeps);
-    for (uint64_t t = 0; t < sweeps; ++t) {
-        int rc = c->kind == KIND_CLUSTER ? enqueue_cluster_sweep(c) : enqueue_local_sweep(c);
-        if (rc != PLQ_OK) return rc;
-    }
-    return PLQ_OK;
+    // Direct sweeps: a sweep's record is folded by the next sweep's first
+    // kernel, and the last one's after the loop.
+    c->fold_defer = true;
+    int rc = PLQ_OK;
+    for (uint64_t t = 0; t < sweeps && rc == PLQ_OK; ++t)
+        rc = c->kind == KIND_CLUSTER ? enqueue_cluster_sweep(c) : enqueue_local_sweep(c);
+    c->fold_defer = false;
+    flush_fold(c);
+    return rc;
 }
 
 }  // namespace

@@ -62,6 +62,10 @@ struct plq_chain {
     int64_t last_relabel_blocks = 0;
     bool have_partition = false;
     bool relabel_measures = true;          // a recorded SW sweep's relabel also counts its record
+    // A record whose fold waits for the next sweep's first kernel (while
+    // fold_defer: inside one enqueue of direct sweeps, flushed at its end).
+    plq::RecordFold fold{};
+    bool fold_defer = false;
     // Wolff moves grown from the seed (plq_wolff.hip)
     int wolff_mode = 0;                    // 0 off, 1 adaptive, 2 an attempt every sweep
     uint32_t wolff_levels = 0, wolff_qcap = 0;

@@ -433,11 +433,16 @@ __device__ __forceinline__ void x16_label(int32_t *P, uint16_t *H, int32_t *__re
 template <int D>
 __global__ __launch_bounds__(1024) void cc_local_x16(const uint8_t *__restrict__ labels, int32_t *__restrict__ parent,
                                                      CCGeom cg, uint32_t bond_seed, uint32_t skey, uint32_t bond_t,
-                                                     const int *__restrict__ gate, int gate_skip) {
+                                                     const int *__restrict__ gate, int gate_skip, RecordFold fold) {
     if (gated(gate, gate_skip)) return;
     extern __shared__ int4 box_lds[];  // its own name: extern LDS arrays of one name share one alignment
     int32_t *P = reinterpret_cast<int32_t *>(box_lds);
     __shared__ uint16_t H[1024];
+    if (fold.rec) {
+        // the previous recorded sweep's record, folded here instead of in a launch of its own
+        __shared__ long long red[16];
+        fold_record(fold, red);
+    }
     const X16Box<D> bx(cg, blockIdx.x);
     uint4 me = make_uint4(0u, 0u, 0u, 0u);
     if (threadIdx.x < (bx.nloc >> 4)) me = *reinterpret_cast<const uint4 *>(labels + bx.locate(cg, threadIdx.x << 4));
@@ -1775,10 +1780,17 @@ int64_t launch_cluster_sweep(const ClusterSweepArgs &a) {
     dim3 gb((uint32_t)((faces + 255) / 256)), bb(256);
     // sixteen sites per lane: whole 16-site segments of axis-0 rows, one per lane
     const bool x16 = cluster_boxes_x16(cg, a.labels);
+    // an earlier record's fold rides on cc_local_x16 when it has a workgroup
+    // per column and no gate; else it is launched first
+    RecordFold fold = a.fold;
+    if (fold.rec && !(x16 && a.gate == nullptr && cg.nboxes >= fold.nk)) {
+        launch_finalize_columns(fold.pa, fold.na, fold.pb, fold.nb, fold.nk, fold.rec, a.stream);
+        fold = RecordFold{};
+    }
 #define PLQ_CC(DD)                                                                                          \
     if (x16)                                                                                                \
         cc_local_x16<DD><<<gl, bl, lds, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t,      \
-                                                    a.gate, a.gate_skip);                                    \
+                                                    a.gate, a.gate_skip, fold);                              \
     else if (pow2)                                                                                          \
         cc_local<DD, true><<<gl, bl, lds, a.stream>>>(a.labels, a.parent, cg, bond_seed, skey, a.bond_t,    \
                                                       a.gate, a.gate_skip);                                  \
@@ -1831,7 +1843,7 @@ int64_t launch_cluster_sweep(const ClusterSweepArgs &a) {
             default: PLQ_RM(4); break;
 #undef PLQ_RM
         }
-        launch_finalize_columns(a.meas_part, Gb, nullptr, 0, 2 + a.q, a.meas_rec, a.stream);
+        if (!a.defer_fold) launch_finalize_columns(a.meas_part, Gb, nullptr, 0, 2 + a.q, a.meas_rec, a.stream);
         return Gb;
     }
     if (x16) {
@@ -1864,6 +1876,15 @@ bool cluster_relabel_measures(const ClusterSweepArgs &a) {
     return !a.label_only && !a.count_sizes && !a.seeded && !a.forced && a.gate == nullptr && a.q >= 1 &&
            a.q <= MAX_Q_DEV && cluster_boxes_x16(a.cg, a.labels) && nhalo <= (uint32_t)RELABEL_HALO_MAX &&
            nhalo <= 3u * (uint32_t)a.cg.box_sites;
+}
+
+RecordFold cluster_record_fold(const ClusterSweepArgs &a, int64_t blocks) {
+    RecordFold f{};
+    f.pa = a.meas_part;
+    f.na = blocks;
+    f.nk = 2 + a.q;
+    f.rec = a.meas_rec;
+    return f;
 }
 
 int64_t cluster_relabel_blocks(const ClusterSweepArgs &a) {

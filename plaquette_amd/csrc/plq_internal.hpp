@@ -27,12 +27,58 @@ struct SwarGeom {
     uint32_t ll1, ll2;
 };
 
+// A record fold (launch_finalize_columns' work) carried into the first
+// kernel of the next sweep instead of a launch of its own: workgroups
+// k < nk of that kernel sum column k of the column-major partials of up to
+// two passes (pa: na blocks, pb: nb blocks) into rec[k], and workgroup 0
+// clears rec[nk .. REC_K).  rec == nullptr: nothing to fold.
+struct RecordFold {
+    const long long *pa;
+    int64_t na;
+    const long long *pb;
+    int64_t nb;
+    int nk;
+    long long *rec;
+};
+
+// The fold, called by every thread of a workgroup (it synchronises the
+// workgroup); lds holds at least 16 entries.  Column sums keep eight loads
+// in flight per lane.
+__device__ __forceinline__ void fold_record(const RecordFold &f, long long *lds) {
+    const int k = (int)blockIdx.x;
+    if (k == 0)
+        for (int j = f.nk + (int)threadIdx.x; j < REC_K; j += (int)blockDim.x) f.rec[j] = 0;
+    if (k >= f.nk) return;
+    auto col = [](const long long *__restrict__ src, int64_t n) -> long long {
+        long long acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        const int64_t step = blockDim.x;
+        int64_t b = threadIdx.x;
+        for (; b + 7 * step < n; b += 8 * step)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc[u] += src[b + u * step];
+        for (; b < n; b += step) acc[0] += src[b];
+        return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    };
+    long long v = col(f.pa + k * f.na, f.na);
+    if (f.pb) v += col(f.pb + k * f.nb, f.nb);
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63u) == 0u) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += lds[w];
+        f.rec[k] = t;
+    }
+    __syncthreads();  // lds is free again
+}
+
 // Fused measurement of a colour pass: partial sums per block, column-major
 // (column k of block b at part[k * blocks + b]), folded into a record by
 // launch_finalize_columns after the colour-1 pass.
 struct MeasArgs {
     long long *part;       // this pass's partials
     int on;                // this pass measures
+    RecordFold fold;       // ising_pass_x16u only: an earlier record's fold
 };
 
 // Geometry of the sixteen-sites-per-lane pass whose waves hold rows of one
@@ -260,6 +306,12 @@ struct ClusterSweepArgs {
     // meas_rec after it, so no separate measurement is needed.
     long long *meas_part;
     long long *meas_rec;
+    // An earlier record's fold for this sweep's first kernel (cc_local_x16;
+    // other paths run launch_finalize_columns first), and with defer_fold
+    // the fused record above is left for the caller to fold later: the
+    // caller takes cluster_record_fold(a, blocks) as its pending fold.
+    RecordFold fold;
+    bool defer_fold;
     hipStream_t stream;
 };
 // Enqueue one cluster sweep; returns the number of relabel blocks whose
@@ -271,6 +323,9 @@ int64_t cluster_relabel_blocks(const ClusterSweepArgs &a);
 // Potts record itself (the per-box relabel of a Swendsen-Wang sweep: no seed
 // cluster, no forced change, no gate, no size counting; q <= MAX_Q_DEV).
 bool cluster_relabel_measures(const ClusterSweepArgs &a);
+// The fold of a fused record left by a sweep with defer_fold (blocks: what
+// launch_cluster_sweep returned).
+RecordFold cluster_record_fold(const ClusterSweepArgs &a, int64_t blocks);
 
 // Wolff moves grown from the seed site (plq_wolff.hip).  One workgroup
 // grows the seed's cluster level by level over the sweep's keyed bonds and,

@@ -762,6 +762,7 @@ __global__ __launch_bounds__(256) void ising_pass_x16u(const uint4 *__restrict__
     constexpr int CS = D == 4 ? 16 : 8;
     __shared__ uint32_t T[2 * CS];
     __shared__ long long lds[16];
+    if (m.fold.rec) fold_record(m.fold, lds);  // an earlier record, folded here instead of in a launch of its own
     if (threadIdx.x < 2 * K) {
         int s = threadIdx.x / K, d = threadIdx.x % K;
         T[d + CS * s] = (table[threadIdx.x] << 8) - 1u;
@@ -967,11 +968,21 @@ static void dispatch_x16(const SitePassArgs &a, int blocks, int64_t work, int64_
 #undef PLQ_X16
 }
 
-void launch_site_pass(const SitePassArgs &a) {
+void launch_site_pass(const SitePassArgs &a_in) {
     int blocks;
     int64_t span;
-    int64_t work = pass_work(a);
-    grid_for_work(work, a.max_blocks, &blocks, &span);
+    int64_t work = pass_work(a_in);
+    grid_for_work(work, a_in.max_blocks, &blocks, &span);
+    // An earlier record's fold rides on the uniform pass when that pass
+    // writes no partials of its own and has a workgroup per column; else it
+    // is launched first.
+    SitePassArgs a = a_in;
+    const bool x16u = a.vec == 16 && a.uniform && a.mode != 2 && span % 64 == 0 && a.g.D >= 2;
+    if (a.m.fold.rec && (!x16u || a.m.on || blocks < a.m.fold.nk)) {
+        launch_finalize_columns(a.m.fold.pa, a.m.fold.na, a.m.fold.pb, a.m.fold.nb, a.m.fold.nk, a.m.fold.rec,
+                                a.stream);
+        a.m.fold = RecordFold{};
+    }
 #define PLQ_BY_D(FN)                                   \
     switch (a.g.D) {                                   \
         case 1: FN<1>(a, blocks, work, span); break;   \
@@ -979,7 +990,7 @@ void launch_site_pass(const SitePassArgs &a) {
         case 3: FN<3>(a, blocks, work, span); break;   \
         default: FN<4>(a, blocks, work, span); break;  \
     }
-    if (a.vec == 16 && a.uniform && a.mode != 2 && span % 64 == 0 && a.g.D >= 2) {
+    if (x16u) {
         switch (a.g.D) {
             case 2: dispatch_x16u<2>(a, blocks, work, span); break;
             case 3: dispatch_x16u<3>(a, blocks, work, span); break;
