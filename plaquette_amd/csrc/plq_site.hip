@@ -88,17 +88,27 @@ __device__ __forceinline__ int64_t logical_block() {
 // Write this block's partial sums of a record, column-major: column k of
 // block b at part[k * gridDim.x + b], columns [bond, 0, counts[0..q)].
 // finalize_columns_kernel folds both colour passes' columns into the record.
-__device__ __forceinline__ void emit_partials(const MeasArgs &m, long long bond, const int *counts, int q,
-                                              long long *lds) {
+// Every lane of the block calls it, at the kernel's end: a DPP sum per wave
+// and value (each lane's values, and a wave's sums, fit in an int), one
+// barrier, then thread k adds value k over the waves.
+__device__ __forceinline__ void emit_partials(const MeasArgs &m, long long bond, const int *counts, int q) {
+    __shared__ int red[16][1 + MAX_Q_DEV];
     const int64_t G = gridDim.x;
-    long long t = block_sum(bond, lds);
-    if (threadIdx.x == 0) {
-        m.part[blockIdx.x] = t;
-        m.part[G + blockIdx.x] = 0;
-    }
+    const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u);
+    const int s0 = wave_sum_i32((int)bond);
+    if (lane == 0) red[wave][0] = s0;
     for (int a = 0; a < q; ++a) {
-        t = block_sum((long long)counts[a], lds);
-        if (threadIdx.x == 0) m.part[(2 + a) * G + blockIdx.x] = t;
+        const int s = wave_sum_i32(counts[a]);
+        if (lane == 0) red[wave][1 + a] = s;
+    }
+    __syncthreads();
+    const int k = (int)threadIdx.x;
+    if (k <= q) {
+        long long t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w][k];
+        m.part[(k == 0 ? 0 : 1 + k) * G + blockIdx.x] = t;
+    } else if (k == q + 1) {
+        m.part[G + blockIdx.x] = 0;
     }
 }
 
@@ -175,7 +185,6 @@ __global__ __launch_bounds__(256) void site_pass_generic(const uint8_t *__restri
                                                          const uint32_t *__restrict__ table,
                                                          MeasArgs m,
                                                          int64_t work, int64_t span) {
-    __shared__ long long lds[16];
     constexpr int NN = 2 * D;
     const int64_t lb = logical_block();
     const int64_t begin = lb * span;
@@ -275,7 +284,7 @@ __global__ __launch_bounds__(256) void site_pass_generic(const uint8_t *__restri
             }
         }
     }
-    if (m.on) emit_partials(m, bond, counts, q, lds);
+    if (m.on) emit_partials(m, bond, counts, q);
 }
 
 // ------------------------------------------------------ SWAR Ising pass ---
@@ -293,7 +302,6 @@ __global__ __launch_bounds__(256) void ising_pass_swar(const uint32_t *__restric
     constexpr int NN = 2 * D;
     constexpr int K = NN + 1;
     __shared__ uint32_t T[2 * K];
-    __shared__ long long lds[16];
     if (threadIdx.x < 2 * K) T[threadIdx.x] = table[threadIdx.x];
     __syncthreads();
     // 32-bit offsets throughout (N / 8 dwords < 2^32): every load is an SGPR
@@ -376,7 +384,7 @@ __global__ __launch_bounds__(256) void ising_pass_swar(const uint32_t *__restric
     }
     if (m.on) {
         int counts[2] = {total - down, down};
-        emit_partials(m, (long long)bond, counts, 2, lds);
+        emit_partials(m, (long long)bond, counts, 2);
     }
 }
 
@@ -479,7 +487,6 @@ __global__ __launch_bounds__(256) void ising_pass_x16(const uint4 *__restrict__ 
     constexpr int NN = 2 * D;
     constexpr int K = NN + 1;
     __shared__ uint32_t T[2 * 16];  // T[d * 2 + s], byte address (d << 3) | (s << 2)
-    __shared__ long long lds[16];
     if (threadIdx.x < 2 * K) {
         int s = threadIdx.x / K, d = threadIdx.x % K;
         T[d * 2 + s] = table[threadIdx.x];
@@ -578,7 +585,7 @@ __global__ __launch_bounds__(256) void ising_pass_x16(const uint4 *__restrict__ 
     }
     if (m.on) {
         int counts[2] = {total - down, down};
-        emit_partials(m, (long long)bond, counts, 2, lds);
+        emit_partials(m, (long long)bond, counts, 2);
     }
 }
 
@@ -802,7 +809,7 @@ __global__ __launch_bounds__(256) void ising_pass_x16u(const uint4 *__restrict__
     }
     if (m.on) {
         int counts[2] = {total - down, down};
-        emit_partials(m, (long long)bond, counts, 2, lds);
+        emit_partials(m, (long long)bond, counts, 2);
     }
 }
 
