@@ -77,12 +77,15 @@ def run_kernel(P, ev, kernel, steps, warmup, kernel_sweeps):
 
     n = int(np.prod(SHAPE))
     chain = P.ising_gpu_chain(SHAPE, 1.0, 0.0, BETA, SEED, np.zeros(n, np.uint8), kernel=kernel)
-    chain.advance(warmup)
-    chain.sync()
-    # whole job: K sweeps with E, M reduced every 10 sweeps
-    chain.record(10, steps // 10 + 1)
+    # whole job: K sweeps with E, M reduced every 10 sweeps.  The records and
+    # events are set up before the warm-up, so the timed sweeps follow the
+    # warm-up's synchronisation with no host work between (the recording
+    # phase counts from sweep 0: the timed sweeps record as they would after
+    # a set-up between warm-up and timing).
+    chain.record(10, (steps + warmup) // 10 + 1)
     stream = chain.stream()
     e0, e1 = ev.event(), ev.event()
+    chain.advance(warmup)
     chain.sync()
     t0 = time.perf_counter()
     ev.record(e0, stream)
@@ -91,7 +94,7 @@ def run_kernel(P, ev, kernel, steps, warmup, kernel_sweeps):
     chain.sync()
     wall = time.perf_counter() - t0
     gpu_ms = ev.elapsed_ms(e0, e1)
-    series = chain.series()
+    series = [o for o in chain.series() if o.sweep > warmup]  # the timed sweeps' records
     # kernel-only region: colour passes alone (no measurement launches)
     chain.record(0, 0)
     chain.sync()
