@@ -271,9 +271,10 @@ int ensure_stats_buffers(plq_chain *c) {
 
 int enqueue_cluster_stats(plq_chain *c, unsigned long long *dst) {
     PLQ_TRY(ensure_stats_buffers(c));
-    PLQ_HIP(hipMemsetAsync(dst, 0, sizeof(unsigned long long) * 42, c->stream));
     if (plq::cluster_boxes_x16(c->cg, c->labels)) {
-        // sizes counted per box from the finished forest (by the sweep's relabel when it was recorded)
+        // sizes counted per box from the finished forest (by the sweep's
+        // relabel when it was recorded); cc_hist_reduce writes every entry
+        // of dst
         plq::launch_cluster_sizes_box(c->parent, c->cg, c->sizes, c->ext, c->d_cross, c->cross_sweep,
                                       c->d_hist_part, c->d_partials + plq::MAX_PASS_BLOCKS * plq::REC_K,
                                       c->last_relabel_blocks, dst, plq::MAX_PASS_BLOCKS, c->sizes_counted,
@@ -281,6 +282,7 @@ int enqueue_cluster_stats(plq_chain *c, unsigned long long *dst) {
         c->sizes_counted = false;  // the counts are consumed (ext cleared)
         return PLQ_OK;
     } else {
+        PLQ_HIP(hipMemsetAsync(dst, 0, sizeof(unsigned long long) * 42, c->stream));
         if (!c->roots) PLQ_TRY(alloc((void **)&c->roots, sizeof(int32_t) * (size_t)c->n_sites));
         PLQ_HIP(hipMemsetAsync(c->sizes, 0, sizeof(int32_t) * (size_t)c->n_sites, c->stream));
         uint32_t cap = (uint32_t)(c->pass_cap > 0xffffffffull ? 0xffffffffull : c->pass_cap);
@@ -485,7 +487,13 @@ int enqueue_cluster_sweep(plq_chain *c) {
     bool record = c->rec_every && (c->sweeps_done % c->rec_every == 0);
     if (record) {
         long long *rec = rec_slot(c);
-        if (!fused) enqueue_measure(c, rec);
+        // between direct sweeps the measurement's fold waits for the next
+        // sweep's first kernel
+        if (!fused && c->fold_defer)
+            plq::launch_potts_measure(c->labels, c->cg, c->q, c->d_partials, plq::MAX_PASS_BLOCKS, rec, c->stream,
+                                      &c->fold);
+        else if (!fused)
+            enqueue_measure(c, rec);
         if (c->rec_clusters) {
             const uint64_t slot = (c->rec_count - 1) % c->rec_cap;
             PLQ_TRY(enqueue_cluster_stats(c, c->d_clu + slot * CLU_SLOT));

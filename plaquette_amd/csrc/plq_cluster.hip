@@ -1896,7 +1896,8 @@ int64_t cluster_relabel_blocks(const ClusterSweepArgs &a) {
 }
 
 void launch_potts_measure(const uint8_t *labels, const CCGeom &cg, int q, long long *partials, int64_t max_blocks,
-                          long long *rec, hipStream_t s) {
+                          long long *rec, hipStream_t s, RecordFold *defer) {
+    if (defer) *defer = RecordFold{};
     if (cg.L[0] % 4 == 0) {
         // sixteen sites per lane and step where rows hold whole 16-byte vectors
         const bool wide = cg.L[0] % 16 == 0 && (reinterpret_cast<uintptr_t>(labels) & 15u) == 0;
@@ -1917,7 +1918,14 @@ void launch_potts_measure(const uint8_t *labels, const CCGeom &cg, int q, long l
             default: PLQ_PM(4); break;
         }
 #undef PLQ_PM
-        launch_finalize_columns(partials, G, nullptr, 0, 2 + q, rec, s);
+        if (defer) {
+            defer->pa = partials;
+            defer->na = G;
+            defer->nk = 2 + q;
+            defer->rec = rec;
+        } else {
+            launch_finalize_columns(partials, G, nullptr, 0, 2 + q, rec, s);
+        }
         return;
     }
     int64_t need = (cg.N + 255) / 256;

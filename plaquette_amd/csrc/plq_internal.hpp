@@ -367,8 +367,11 @@ void launch_wolff_grow(const WolffGrowArgs &a);
 // dst[queue[i]] = the grown cluster's old label for i < size, when gate == 0:
 // the labels the last grown sweep started from, given its result in dst.
 void launch_wolff_restore(uint8_t *dst, const int32_t *queue, const WolffState *state, hipStream_t s);
+// With defer non-null, a measurement whose partials are column-major leaves
+// their fold in *defer instead of launching it (defer->rec stays null when
+// the fold was launched).
 void launch_potts_measure(const uint8_t *labels, const CCGeom &cg, int q, long long *partials,
-                          int64_t max_blocks, long long *rec, hipStream_t s);
+                          int64_t max_blocks, long long *rec, hipStream_t s, RecordFold *defer = nullptr);
 // Canonical roots of the current forest, and (when sizes/hist are non-null)
 // cluster sizes and the size histogram (hist[0..39] by floor(log2 size),
 // hist[40] cluster count, hist[41] largest size).
