@@ -415,14 +415,12 @@ __global__ void ens_zero(unsigned long long *__restrict__ p, int n) {
 // delta = (beta_k+1 - beta_k) J x >= 0, or the keyed draw is below the
 // host-built threshold for x (f64 exp, as every other acceptance here).
 // One thread per pair; pairs of one round are disjoint.
-__global__ void ens_exchange(const EnsGroupPair *__restrict__ pairs, int npairs, uint32_t *__restrict__ slot_of,
-                             uint32_t *__restrict__ rep_at, const unsigned long long *__restrict__ sums,
-                             const uint32_t *__restrict__ ex_tables, const uint32_t *__restrict__ round_base,
-                             uint32_t round_off, uint32_t ex_seed, unsigned long long *__restrict__ accepted) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
+__device__ __forceinline__ void ens_exchange_pair(int p, uint32_t round, const EnsGroupPair *__restrict__ pairs,
+                                                  uint32_t *__restrict__ slot_of, uint32_t *__restrict__ rep_at,
+                                                  const unsigned long long *__restrict__ sums,
+                                                  const uint32_t *__restrict__ ex_tables, uint32_t ex_seed,
+                                                  unsigned long long *__restrict__ accepted) {
     const EnsGroupPair P = pairs[p];
-    const uint32_t round = *round_base + round_off;
     if ((uint32_t)P.k % 2u != round % 2u) return;
     const uint32_t a = rep_at[P.rep_at0 + P.k];
     const uint32_t b = rep_at[P.rep_at0 + P.k + 1];
@@ -446,23 +444,44 @@ __global__ void ens_exchange(const EnsGroupPair *__restrict__ pairs, int npairs,
     }
 }
 
+
 // counters[0] = sweep base, counters[1] = exchange round, counters[2] = records
 __global__ void ens_tick(uint32_t *__restrict__ counters, uint32_t sweeps, uint32_t rounds) {
     counters[0] += sweeps;
     counters[1] += rounds;
 }
 
-// Copy the current sums and beta slots into the measurement ring.
-__global__ void ens_record(const unsigned long long *__restrict__ sums, const uint32_t *__restrict__ slot_of, int nrep,
-                           long long *__restrict__ ring_sums, uint32_t *__restrict__ ring_slots, uint32_t cap,
-                           uint32_t *__restrict__ counters) {
+
+// The measurement tail of a block of sweeps in one workgroup (one launch
+// where four small ones ran): the sums and beta slots into the ring, then
+// (npairs > 0) one exchange round, then the sums cleared for the next
+// block's ens_plaquette, and the counters advanced as ens_tick does.
+// Barriers order the phases: the record reads the slots before the exchange
+// swaps them, and every thread reads the counters before thread 0 moves
+// them.
+__global__ __launch_bounds__(256) void ens_post(unsigned long long *__restrict__ sums, uint32_t *__restrict__ slot_of,
+                                                int nrep, long long *__restrict__ ring_sums,
+                                                uint32_t *__restrict__ ring_slots, uint32_t cap,
+                                                uint32_t *__restrict__ counters, const EnsGroupPair *__restrict__ pairs,
+                                                int npairs, uint32_t *__restrict__ rep_at,
+                                                const uint32_t *__restrict__ ex_tables, uint32_t ex_seed,
+                                                unsigned long long *__restrict__ accepted, uint32_t sweeps) {
     const uint32_t slot = counters[2] % cap;
+    const uint32_t round = counters[1];
     for (int i = threadIdx.x; i < nrep; i += blockDim.x) {
         ring_sums[(size_t)slot * nrep + i] = (long long)sums[i];
         ring_slots[(size_t)slot * nrep + i] = slot_of[i];
     }
     __syncthreads();
-    if (threadIdx.x == 0) counters[2] += 1;
+    for (int p = threadIdx.x; p < npairs; p += blockDim.x)
+        ens_exchange_pair(p, round, pairs, slot_of, rep_at, sums, ex_tables, ex_seed, accepted);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nrep; i += blockDim.x) sums[i] = 0ull;
+    if (threadIdx.x == 0) {
+        counters[0] += sweeps;
+        counters[1] += npairs > 0 ? 1u : 0u;
+        counters[2] += 1u;
+    }
 }
 
 // lattice order (link = site * D + dir) <-> byte j of the quad's words
@@ -573,9 +592,22 @@ void launch_ens_pass(const EnsPassArgs &a) {
     }
 }
 
-void launch_ens_plaquette(const uint32_t *words, const EnsQuad *quads, const uint16_t *block_quad, int nrep, int D,
-                          uint32_t blocks, int block_size, int per_lane, unsigned long long *sums, hipStream_t s) {
+void launch_ens_zero(unsigned long long *sums, int nrep, hipStream_t s) {
     ens_zero<<<dim3((nrep + 255) / 256), dim3(256), 0, s>>>(sums, nrep);
+}
+
+void launch_ens_post(unsigned long long *sums, uint32_t *slot_of, int nrep, long long *ring_sums, uint32_t *ring_slots,
+                     uint32_t cap, uint32_t *counters, const EnsGroupPair *pairs, int npairs, uint32_t *rep_at,
+                     const uint32_t *ex_tables, uint32_t ex_seed, unsigned long long *accepted, uint32_t sweeps,
+                     hipStream_t s) {
+    ens_post<<<dim3(1), dim3(256), 0, s>>>(sums, slot_of, nrep, ring_sums, ring_slots, cap, counters, pairs, npairs,
+                                           rep_at, ex_tables, ex_seed, accepted, sweeps);
+}
+
+void launch_ens_plaquette(const uint32_t *words, const EnsQuad *quads, const uint16_t *block_quad, int nrep, int D,
+                          uint32_t blocks, int block_size, int per_lane, unsigned long long *sums, hipStream_t s,
+                          bool zero_first) {
+    if (zero_first) launch_ens_zero(sums, nrep, s);
     dim3 grid(blocks), block(block_size);
     switch (D) {
         case 2: ens_plaquette<2><<<grid, block, 0, s>>>(words, quads, block_quad, sums, per_lane); break;
@@ -584,21 +616,10 @@ void launch_ens_plaquette(const uint32_t *words, const EnsQuad *quads, const uin
     }
 }
 
-void launch_ens_exchange(const EnsGroupPair *pairs, int npairs, uint32_t *slot_of, uint32_t *rep_at,
-                         const unsigned long long *sums, const uint32_t *ex_tables, const uint32_t *round_base,
-                         uint32_t round_off, uint32_t ex_seed, unsigned long long *accepted, hipStream_t s) {
-    if (npairs <= 0) return;
-    ens_exchange<<<dim3((npairs + 63) / 64), dim3(64), 0, s>>>(pairs, npairs, slot_of, rep_at, sums, ex_tables,
-                                                             round_base, round_off, ex_seed, accepted);
-}
 
 void launch_ens_tick(uint32_t *counters, uint32_t sweeps, uint32_t rounds, hipStream_t s) {
     ens_tick<<<dim3(1), dim3(1), 0, s>>>(counters, sweeps, rounds);
 }
 
-void launch_ens_record(const unsigned long long *sums, const uint32_t *slot_of, int nrep, long long *ring_sums,
-                       uint32_t *ring_slots, uint32_t cap, uint32_t *counters, hipStream_t s) {
-    ens_record<<<dim3(1), dim3(256), 0, s>>>(sums, slot_of, nrep, ring_sums, ring_slots, cap, counters);
-}
 
 }  // namespace plq

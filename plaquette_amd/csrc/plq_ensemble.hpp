@@ -69,13 +69,18 @@ void launch_ens_pass(const EnsPassArgs &a);
 // Every colour pass of nsweeps sweeps in one cooperative launch with grid
 // barriers between the passes (ens_link_pass2's body; a.fast must hold).
 hipError_t launch_ens_sweeps_coop(const EnsPassArgs &a, uint32_t nsweeps);
+// Per-replica signed plaquette sums added into sums (cleared first with
+// zero_first).
 void launch_ens_plaquette(const uint32_t *words, const EnsQuad *quads, const uint16_t *block_quad, int nrep, int D,
-                          uint32_t blocks, int block_size, int per_lane, unsigned long long *sums, hipStream_t s);
-void launch_ens_exchange(const EnsGroupPair *pairs, int npairs, uint32_t *slot_of, uint32_t *rep_at,
-                         const unsigned long long *sums, const uint32_t *ex_tables, const uint32_t *round_base,
-                         uint32_t round_off, uint32_t ex_seed, unsigned long long *accepted, hipStream_t s);
+                          uint32_t blocks, int block_size, int per_lane, unsigned long long *sums, hipStream_t s,
+                          bool zero_first);
+void launch_ens_zero(unsigned long long *sums, int nrep, hipStream_t s);
+// A measured block's tail in one launch (ens_post): record, exchange round
+// (npairs > 0), sums cleared, counters advanced by sweeps and the round.
+void launch_ens_post(unsigned long long *sums, uint32_t *slot_of, int nrep, long long *ring_sums, uint32_t *ring_slots,
+                     uint32_t cap, uint32_t *counters, const EnsGroupPair *pairs, int npairs, uint32_t *rep_at,
+                     const uint32_t *ex_tables, uint32_t ex_seed, unsigned long long *accepted, uint32_t sweeps,
+                     hipStream_t s);
 void launch_ens_tick(uint32_t *counters, uint32_t sweeps, uint32_t rounds, hipStream_t s);
-void launch_ens_record(const unsigned long long *sums, const uint32_t *slot_of, int nrep, long long *ring_sums,
-                       uint32_t *ring_slots, uint32_t cap, uint32_t *counters, hipStream_t s);
 
 }  // namespace plq

@@ -178,20 +178,16 @@ void enqueue_block(plq_ensemble *e, uint32_t nsweeps, bool measure) {
                 plq::launch_ens_pass(a);
             }
     }
-    uint32_t rounds = 0;
     if (measure) {
+        // the sums are zero here: ens_post clears them after each record
         plq::launch_ens_plaquette(e->d_words, e->d_quads, e->d_block_rep, e->nrep, e->D, e->total_blocks, e->block_size,
-                                  e->per_lane, e->d_sums, e->stream);
-        plq::launch_ens_record(e->d_sums, e->d_slot_of, e->nrep, e->d_ring_sums, e->d_ring_slots,
-                               (uint32_t)e->rec_cap, e->d_counters, e->stream);
-        if (e->exchange) {
-            plq::launch_ens_exchange(e->d_pairs, (int)e->pairs.size(), e->d_slot_of, e->d_rep_at, e->d_sums,
-                                     e->d_ex_tables, e->d_counters + 1, 0, e->ex_seed32, e->d_accepted,
-                                     e->stream);
-            rounds = 1;
-        }
+                                  e->per_lane, e->d_sums, e->stream, false);
+        plq::launch_ens_post(e->d_sums, e->d_slot_of, e->nrep, e->d_ring_sums, e->d_ring_slots, (uint32_t)e->rec_cap,
+                             e->d_counters, e->d_pairs, e->exchange ? (int)e->pairs.size() : 0, e->d_rep_at,
+                             e->d_ex_tables, e->ex_seed32, e->d_accepted, nsweeps, e->stream);
+        return;
     }
-    plq::launch_ens_tick(e->d_counters, nsweeps, rounds, e->stream);
+    plq::launch_ens_tick(e->d_counters, nsweeps, 0, e->stream);
 }
 
 void account(plq_ensemble *e, uint32_t nsweeps, bool measure) {
@@ -455,6 +451,7 @@ int plq_ensemble_create(const plq_ensemble_desc *d, const uint8_t *start, size_t
         (ex_tables.size() &&
          hipMemcpy(e->d_ex_tables, ex_tables.data(), sizeof(uint32_t) * ex_tables.size(), hipMemcpyHostToDevice)) ||
         hipMemset(e->d_accepted, 0, sizeof(unsigned long long) * std::max<size_t>(1, e->pairs.size())) ||
+        hipMemset(e->d_sums, 0, sizeof(unsigned long long) * e->nrep) ||
         hipMemset(e->d_counters, 0, sizeof(uint32_t) * 4) ||
         hipMemset(e->d_words, 0, sizeof(uint32_t) * e->nwords) ||  // unused quad bytes stay 0
         hipMemcpy(e->d_lat, start, n, hipMemcpyHostToDevice))
@@ -500,10 +497,11 @@ int plq_ensemble_plaquette_sums(plq_ensemble *e, int64_t *sums, size_t n) {
     std::lock_guard<std::mutex> lk(e->mu);
     ENS_HIP(hipSetDevice(e->device));
     plq::launch_ens_plaquette(e->d_words, e->d_quads, e->d_block_rep, e->nrep, e->D, e->total_blocks, e->block_size, e->per_lane,
-                              e->d_sums, e->stream);
+                              e->d_sums, e->stream, true);
     std::vector<unsigned long long> h(e->nrep);
     ENS_HIP(hipMemcpyAsync(h.data(), e->d_sums, sizeof(unsigned long long) * e->nrep, hipMemcpyDeviceToHost,
                            e->stream));
+    plq::launch_ens_zero(e->d_sums, e->nrep, e->stream);  // a measured block's plaquette adds into zeroed sums
     ENS_HIP(hipStreamSynchronize(e->stream));
     for (int r = 0; r < e->nrep; ++r) sums[r] = (int64_t)h[r];
     return PLQ_OK;
