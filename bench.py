@@ -72,6 +72,13 @@ class HipEvents:
         return ms.value
 
 
+def fused_sweep() -> bool:
+    """Whether the 256^3 chain runs one launch per sweep (the fused two-colour
+    sweep, the library's default for this lattice; PLQ_FUSED_SWEEP=0 keeps
+    two colour-pass launches per sweep)."""
+    return os.environ.get("PLQ_FUSED_SWEEP", "1") != "0"
+
+
 def run_kernel(P, ev, kernel, steps, warmup, kernel_sweeps):
     import numpy as np
 
@@ -95,7 +102,7 @@ def run_kernel(P, ev, kernel, steps, warmup, kernel_sweeps):
     wall = time.perf_counter() - t0
     gpu_ms = ev.elapsed_ms(e0, e1)
     series = [o for o in chain.series() if o.sweep > warmup]  # the timed sweeps' records
-    # kernel-only region: colour passes alone (no measurement launches)
+    # kernel-only region: the sweep launches alone (no records)
     chain.record(0, 0)
     chain.sync()
     k0, k1 = ev.event(), ev.event()
@@ -103,14 +110,14 @@ def run_kernel(P, ev, kernel, steps, warmup, kernel_sweeps):
     chain.advance(kernel_sweeps)
     ev.record(k1, stream)
     chain.sync()
-    pass_ms = ev.elapsed_ms(k0, k1) / (2 * kernel_sweeps)
+    launch_ms = ev.elapsed_ms(k0, k1) / ((1 if fused_sweep() else 2) * kernel_sweeps)
     chain.close()
     return {
         "wall_s": wall,
         "gpu_ms": gpu_ms,
         "site_updates_per_ns": steps * n / (wall * 1e9),
         "site_updates_per_ns_gpu_clock": steps * n / (gpu_ms * 1e6),
-        "pass_ms": pass_ms,
+        "launch_ms": launch_ms,
         "last_energy_per_site": series[-1].energy / n if series else None,
         "last_abs_m_per_site": abs(series[-1].spin_sum) / n if series else None,
         "records": len(series),
@@ -264,29 +271,39 @@ def read_profile_value(name: str, key: str):
     return None
 
 
+def kernel_files():
+    """The committed rocprofv3 summaries of the launch that dominates the
+    3D job: the fused sweep, or the colour pass with PLQ_FUSED_SWEEP=0."""
+    if fused_sweep():
+        return "ising3d_256_sweep_traffic.txt", "ising3d_256_sweep_counters.txt"
+    return "ising3d_256_pass_traffic.txt", "ising3d_256_pass_counters.txt"
+
+
 def read_traffic():
-    """HBM bytes per colour-pass launch from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
-    return read_profile_value("ising3d_256_pass_traffic.txt", "hbm_bytes_per_launch")
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    return read_profile_value(kernel_files()[0], "hbm_bytes_per_launch")
 
 
-def valu_roof(pass_ms: float):
-    lane_ops = read_profile_value("ising3d_256_pass_counters.txt", "valu_lane_ops_per_launch")
-    peak = read_profile_value("ising3d_256_pass_counters.txt", "valu_peak_lane_ops_per_s")
-    per_site = read_profile_value("ising3d_256_pass_counters.txt", "valu_insts_per_site_update")
+def valu_roof(launch_ms: float):
+    name = kernel_files()[1]
+    lane_ops = read_profile_value(name, "valu_lane_ops_per_launch")
+    peak = read_profile_value(name, "valu_peak_lane_ops_per_s")
+    per_site = read_profile_value(name, "valu_insts_per_site_update")
     if not lane_ops or not peak:
         return None
-    achieved = lane_ops / (pass_ms * 1e-3)
+    achieved = lane_ops / (launch_ms * 1e-3)
     return {"achieved": round(achieved / 1e12, 2), "peak": round(peak / 1e12, 2), "unit": "T lane-ops/s",
             "frac": round(achieved / peak, 4), "valu_insts_per_site_update": per_site,
-            "source": "profiles/ising3d_256_pass_counters.txt"}
+            "source": "profiles/" + name}
 
 
 def profile_passes(sweeps: int, copy_mib: int) -> None:
     """A short fixed workload for hardware-counter runs: `sweeps` Metropolis
-    sweeps of the 256^3 lattice (2 colour-pass launches each, no measurement
-    launches), then one device-to-device copy of `copy_mib` MiB whose
-    FETCH_SIZE / WRITE_SIZE calibrate the counters against known bytes."""
+    sweeps of the 256^3 lattice (one fused launch each, or two colour-pass
+    launches with PLQ_FUSED_SWEEP=0; no records), then one device-to-device
+    copy of `copy_mib` MiB whose FETCH_SIZE / WRITE_SIZE calibrate the
+    counters against known bytes."""
     import numpy as np
 
     import plaquette_amd as P
@@ -311,8 +328,9 @@ def profile_passes(sweeps: int, copy_mib: int) -> None:
     hip.hipDeviceSynchronize()
     hip.hipFree(a)
     hip.hipFree(b)
-    print(json.dumps({"profile": "passes", "sweeps": sweeps, "pass_launches": 2 * sweeps,
-                      "copy_bytes": nbytes, "site_updates_per_launch": n // 2}))
+    per = 1 if fused_sweep() else 2
+    print(json.dumps({"profile": "sweeps", "sweeps": sweeps, "launches": per * sweeps,
+                      "copy_bytes": nbytes, "site_updates_per_launch": n // per}))
 
 
 def timed(P, ev, stream_of, run):
@@ -841,8 +859,10 @@ def main():
     metro = run_kernel(P, ev, P.Kernel.METROPOLIS, args.steps, args.warmup, args.kernel_sweeps)
     hb = run_kernel(P, ev, P.Kernel.HEAT_BATH, args.steps, args.warmup, args.kernel_sweeps)
 
-    bytes_per_launch = 2.0 * (n // 2)  # int8 read + write per site-update, N/2 updates per pass
-    achieved = bytes_per_launch / (metro["pass_ms"] * 1e-3) / 1e9
+    # int8 read + write per site-update: N updates per fused sweep launch, N/2 per colour pass
+    updates_per_launch = n if fused_sweep() else n // 2
+    bytes_per_launch = 2.0 * updates_per_launch
+    achieved = bytes_per_launch / (metro["launch_ms"] * 1e-3) / 1e9
     traffic = read_traffic()
     out = {
         "metric": METRIC,
@@ -873,17 +893,18 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "ising_pass_x16u<3,0> (one colour pass)",
-            "avg_launch_us": round(metro["pass_ms"] * 1e3, 3),
-            "site_updates_per_ns_kernel": round((n // 2) / (metro["pass_ms"] * 1e6), 1),
+            "kernel": ("ising_sweep_x16f<0> (one sweep: colour 0 of each 16x16-row tile and its halos, then "
+                       "colour 1)") if fused_sweep() else "ising_pass_x16u<3,0> (one colour pass)",
+            "avg_launch_us": round(metro["launch_ms"] * 1e3, 3),
+            "site_updates_per_ns_kernel": round(updates_per_launch / (metro["launch_ms"] * 1e6), 1),
         },
-        # The pass is bound by VALU issue more than by bytes: instructions per
+        # The launch is bound by VALU issue more than by bytes: instructions per
         # launch from the committed SQ counter summary, over the chip's measured
         # v_mul_lo_u32 issue rate, against this run's launch time.
-        "valu_roof": valu_roof(metro["pass_ms"]),
+        "valu_roof": valu_roof(metro["launch_ms"]),
         "metropolis": {k: (round(v, 6) if isinstance(v, float) else v) for k, v in metro.items()},
         "heat_bath": {k: (round(v, 6) if isinstance(v, float) else v) for k, v in hb.items()},
-        "heat_bath_roofline_frac": round(bytes_per_launch / (hb["pass_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "heat_bath_roofline_frac": round(bytes_per_launch / (hb["launch_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "cpu_baseline": None,
     }
     if not args.no_cpu:

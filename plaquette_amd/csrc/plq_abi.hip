@@ -334,7 +334,42 @@ int enqueue_local_sweep(plq_chain *c) {
         if (record) enqueue_site_measure(c, rec_slot(c));
     } else if (c->kind == KIND_SITE) {
         c->sweeps_done += 1;
-        enqueue_site_passes(c, sweep, mode, record ? rec_slot(c) : nullptr);
+        if (c->fused && c->x16u && !c->site_graph &&
+            (!record || plq::x16f_tiles(c->g) <= plq::MAX_PASS_BLOCKS)) {
+            // both colour passes in one launch; a pending record fold rides on
+            // it, and its own record (one partial row per tile) is folded by
+            // the next sweep's first kernel or at once
+            plq::MeasArgs m{};
+            m.part = c->d_partials;
+            m.on = record ? 1 : 0;
+            if (record && c->fold.rec) {
+                // the pending fold reads its partials during this launch:
+                // write this record's into a half it does not read
+                long long *h1 = c->d_partials + plq::MAX_PASS_BLOCKS * plq::REC_K;
+                auto reads = [&](const long long *h) { return c->fold.pa == h || c->fold.pb == h; };
+                if (reads(m.part)) {
+                    if (!reads(h1)) m.part = h1;
+                    else flush_fold(c);
+                }
+            }
+            long long *rec = record ? rec_slot(c) : nullptr;
+            plq::launch_site_sweep_x16f(c->col[0], c->col[1], c->col_alt[0], c->col_alt[1], c->g, c->ug, mode,
+                                        c->seed32, sweep * plq::GOLDEN, c->d_table, c->fold, m, c->stream);
+            c->fold = plq::RecordFold{};
+            std::swap(c->col[0], c->col_alt[0]);
+            std::swap(c->col[1], c->col_alt[1]);
+            if (rec) {
+                plq::RecordFold f{};
+                f.pa = m.part;
+                f.na = plq::x16f_tiles(c->g);
+                f.nk = 2 + c->q;
+                f.rec = rec;
+                c->fold = f;
+                if (!c->fold_defer) flush_fold(c);
+            }
+        } else {
+            enqueue_site_passes(c, sweep, mode, record ? rec_slot(c) : nullptr);
+        }
     } else {
         for (int dir = 0; dir < c->D; ++dir)
             for (int color = 0; color < 2; ++color) {
@@ -737,6 +772,18 @@ int plq_chain_create_local(const plq_local_desc *d, const uint8_t *start, size_t
     if (c->kind == KIND_SITE) {
         for (int k = 0; k < 2; ++k)
             if ((rc = alloc((void **)&c->col[k], half)) != PLQ_OK) return bail(rc);
+        // The fused two-colour sweep (3D Ising, L0 = 256): on where its
+        // 16 x 16-row tiles give a workgroup to every CU (at least 256
+        // tiles); PLQ_FUSED_SWEEP=1 wherever its shape applies, 0 off.
+        if (c->model == PLQ_MODEL_ISING && !c->pmode && c->vec == 16 && plq::x16f_applies(c->g, 1)) {
+            int on = plq::x16f_tiles(c->g) >= 256 ? 1 : 0;
+            if (const char *e = getenv("PLQ_FUSED_SWEEP")) on = atoi(e) != 0 ? 1 : 0;
+            if (on) {
+                for (int k = 0; k < 2; ++k)
+                    if ((rc = alloc((void **)&c->col_alt[k], half)) != PLQ_OK) return bail(rc);
+                c->fused = true;
+            }
+        }
     } else {
         for (int k = 0; k < 2 * d->dim; ++k)
             if ((rc = alloc((void **)&c->links[k], half)) != PLQ_OK) return bail(rc);

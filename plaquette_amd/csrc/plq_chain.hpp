@@ -39,6 +39,10 @@ struct plq_chain {
     bool small = false;                    // whole sweeps in one workgroup (plq_small.hip, plq_cluster_small.hip)
     int64_t pass_blocks = plq::MAX_PASS_BLOCKS;  // grid cap of a colour pass
     uint8_t *col[2] = {nullptr, nullptr};  // site colours
+    // The fused two-colour sweep reads col and writes col_alt, then the two
+    // pairs swap (ising_sweep_x16f).
+    uint8_t *col_alt[2] = {nullptr, nullptr};
+    bool fused = false;
     uint8_t *links[2 * plq::MAX_D] = {nullptr};
     uint8_t **d_links = nullptr;           // device copy of the link array pointers
     uint32_t *d_table = nullptr;
@@ -115,7 +119,7 @@ struct plq_chain {
         (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
         if (site_exec) (void)hipGraphExecDestroy(site_exec);
-        for (auto p : {(void *)col[0], (void *)col[1], (void *)d_links, (void *)d_table, (void *)labels,
+        for (auto p : {(void *)col[0], (void *)col[1], (void *)col_alt[0], (void *)col_alt[1], (void *)d_links, (void *)d_table, (void *)labels,
                        (void *)parent, (void *)roots, (void *)sizes, (void *)ext, (void *)hist, (void *)d_flag,
                        (void *)d_lat,
                        (void *)d_partials, (void *)d_meas, (void *)d_rec, (void *)d_block_counters, (void *)d_clu, (void *)d_cross,

@@ -122,6 +122,17 @@ void launch_site_pass(const SitePassArgs &a);
 // Whether the uniform-parity sixteen-site pass serves this geometry and
 // threshold table (every threshold at least 1; shapes as X16UGeom needs).
 bool x16u_applies(const Geom &g, const uint32_t *host_table, size_t table_len, int64_t max_blocks);
+// The fused two-colour 3D sweep (ising_sweep_x16f): both colour passes of a
+// sweep in one launch, reading o0/o1 and writing n0/n1 (the caller swaps
+// them after), for chains the uniform pass serves with L0 = 256 and L1, L2
+// multiples of 16 (at least 32); one workgroup per 16 x 16 tile of rows.
+// With m.on the launch also writes the record's partial sums (one column
+// entry per tile, as the colour passes' partials) into m.part.
+bool x16f_applies(const Geom &g, int uniform);
+int64_t x16f_tiles(const Geom &g);
+void launch_site_sweep_x16f(const uint8_t *o0, const uint8_t *o1, uint8_t *n0, uint8_t *n1, const Geom &g,
+                            const X16UGeom &ug, int mode, uint32_t seed, uint32_t skey, const uint32_t *table,
+                            const RecordFold &fold, const MeasArgs &m, hipStream_t s);
 X16UGeom x16u_geom(const Geom &g);
 int site_pass_blocks(const SitePassArgs &a);
 void launch_finalize_record(const long long *pa, int64_t na, const long long *pb, int64_t nb, long long *rec,

@@ -813,6 +813,269 @@ __global__ __launch_bounds__(256) void ising_pass_x16u(const uint4 *__restrict__
     }
 }
 
+// ---------------------------------------------------- fused 3D sweep ---
+
+// Both colour passes of one 3D Ising sweep in one launch, for L0 = 256
+// (eight 16-byte chunks per colour row) and L1, L2 multiples of 16.
+// Workgroup b owns a tile of 16 x 16 rows (x1, x2):
+//   1. it updates colour 0 on the tile and on the tile's four face halos
+//      (the rows one step beyond it along x1 and x2), keeping every new
+//      colour-0 row in LDS and storing the tile's own rows to n0.  A halo
+//      row's update is the one its own tile makes: a site's update reads
+//      only the old configuration around it and its keyed draw;
+//   2. after a barrier it updates colour 1 on the tile from the new
+//      colour-0 rows in LDS and its old colour-1 rows, storing to n1.
+// The old arrays o0, o1 are only read, so no workgroup sees another's new
+// values; the caller swaps old and new arrays after the sweep.  The bits
+// are the two colour passes' (same draws, same decisions, same order of
+// dependence).  Rows of a group of eight lie two apart along x1 (tile rows
+// and the x2 halos: one x2, as ising_pass_x16u's waves) or along x2 (the
+// x1 halos), so the axis-0 colour offset is uniform over the group.
+constexpr int X16F_T = 16;               // tile rows along x1 and x2
+constexpr int X16F_S = X16F_T + 2;       // LDS row slots along each axis, halos included
+constexpr int X16F_GROUPS0 = 40;         // colour-0 row groups: 32 tile, 4 x2-halo, 4 x1-halo
+constexpr int X16F_GROUPS1 = 32;         // colour-1 row groups: the tile's
+
+// LDS row slot of tile-local rows (yl, zl), each in -1 .. 16.
+__device__ __forceinline__ uint32_t x16f_slot(int yl, int zl) { return (uint32_t)((zl + 1) * X16F_S + (yl + 1)); }
+
+// The sixteen draws of a chunk at colour-array byte offset A with axis-0
+// colour offset P (y = the hash before its final 8-bit shift).  UNI: the
+// chunk's wave has one value of A >> 16 (see x16u_chunk), so the first
+// multiply of every draw is an add of a wave-uniform offset.
+template <int P, bool UNI>
+__device__ __forceinline__ void x16f_draws(uint32_t seed, uint32_t A, uint32_t kmix, uint32_t (&y)[16]) {
+    const uint32_t bx = seed ^ (2u * A + (uint32_t)P);
+    const uint32_t v0 = bx ^ (bx >> 16);
+    if (UNI) {
+        const uint32_t b = __builtin_amdgcn_readfirstlane(((seed >> 1) ^ (seed >> 17) ^ (A >> 16)) & 15u);
+        const uint32_t q0 = (v0 & ~30u) * 0x7feb352du;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const uint32_t t = q0 + (b ^ (uint32_t)k) * 0xffd66a5au;  // 2 * 0x7feb352d (mod 2^32)
+            y[k] = lowbias32_tail(((t ^ (t >> 15)) * 0x846ca68bu) ^ kmix);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) y[k] = lowbias32_tail(lowbias32_mid(v0 ^ (2u * (uint32_t)k)) ^ kmix);
+    }
+}
+
+// The new bytes of a chunk: a = the other colour's chunk of the same row,
+// adj = its dword holding the byte the axis-0 shift pulls in, rows = the
+// other colour's chunks of the four rows one step along x1 and x2, s = the
+// chunk's own old bytes; as x16u_chunk's decision for D = 3.
+template <int MODE, int P>
+__device__ __forceinline__ uint4 x16f_decide(const uint4 &a, uint32_t adj, const uint4 (&rows)[4], const uint4 &s,
+                                             const uint32_t (&y)[16], const char *Tb, uint32_t (&ds)[4]) {
+    const uint32_t av[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t sh0 = P ? __builtin_amdgcn_alignbyte(j < 3 ? av[j + 1] : adj, av[j], 1)
+                               : __builtin_amdgcn_alignbyte(av[j], j > 0 ? av[j - 1] : adj, 3);
+        ds[j] = av[j] + sh0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        ds[0] += rows[k].x;
+        ds[1] += rows[k].y;
+        ds[2] += rows[k].z;
+        ds[3] += rows[k].w;
+    }
+    const uint32_t sv[4] = {s.x, s.y, s.z, s.w};
+    uint32_t bits = 0;
+#pragma unroll
+    for (int j = 3; j >= 0; --j) {
+        const uint32_t cls4 = (sv[j] << 3) + ds[j];
+#pragma unroll
+        for (int k = 3; k >= 0; --k) {
+            uint32_t addr;
+            if (k == 0)
+                asm("v_lshlrev_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_0"
+                    : "=v"(addr)
+                    : "v"(2u), "v"(cls4));
+            else
+                addr = ((cls4 >> (8 * k)) & 0xffu) << 2;
+            const uint32_t t2 = *reinterpret_cast<const uint32_t *>(Tb + addr);
+            asm("v_cmp_gt_u32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc"
+                : "+v"(bits)
+                : "v"(y[4 * j + k]), "v"(t2)
+                : "vcc");
+        }
+    }
+    uint32_t ov[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t raw = ((bits >> (4 * j)) & 0xfu) * 0x00204081u;
+        ov[j] = MODE == 0 ? (sv[j] ^ (~raw & 0x01010101u)) : (raw & 0x01010101u);
+    }
+    return make_uint4(ov[0], ov[1], ov[2], ov[3]);
+}
+
+// One colour-0 chunk: loads from the old arrays, the new bytes to LDS slot
+// `slot` and (tile rows) to n0.
+template <int MODE, int P, bool UNI>
+__device__ __forceinline__ void x16f_chunk0(const uint4 *__restrict__ o0, const uint4 *__restrict__ o1,
+                                            uint4 *__restrict__ n0, uint4 *R0, const X16UGeom &ug, uint32_t A,
+                                            uint32_t slot, uint32_t chunk, bool store, uint32_t seed, uint32_t kmix,
+                                            const char *Tb, int &down) {
+    const uint32_t S1 = 1u << ug.sh, M1 = ug.m1 << ug.sh;
+    const uint32_t S2 = 1u << (ug.sh + ug.ll1), M2 = ug.m2 << (ug.sh + ug.ll1);
+    const uint32_t cm = (1u << (ug.lh16 + 4)) - 16u;
+    const uint32_t adjoff = P ? bfi(cm, A + 16u, A) : bfi(cm, A - 16u, A) + 12u;
+    uint4 rows[4];
+    const uint4 a = ldb(o1, A);
+    const uint32_t adj = ldb(reinterpret_cast<const uint32_t *>(o1), adjoff);
+    rows[0] = ldb(o1, bfi(M1, A + S1, A));
+    rows[1] = ldb(o1, bfi(M1, A - S1, A));
+    rows[2] = ldb(o1, bfi(M2, A + S2, A));
+    rows[3] = ldb(o1, bfi(M2, A - S2, A));
+    const uint4 s = ldb(o0, A);
+    uint32_t y[16];
+    __builtin_amdgcn_sched_barrier(0);
+    x16f_draws<P, UNI>(seed, A, kmix, y);
+    __builtin_amdgcn_sched_barrier(0);
+    uint32_t ds[4];
+    const uint4 ov = x16f_decide<MODE, P>(a, adj, rows, s, y, Tb, ds);
+    R0[slot * 8u + chunk] = ov;
+    if (store) {
+        stb(n0, A, ov);
+        down += (int)(__popc(ov.x) + __popc(ov.y) + __popc(ov.z) + __popc(ov.w));
+    }
+}
+
+// One colour-1 chunk of the tile: the new colour-0 neighbours from LDS.
+template <int MODE, int P, bool MEAS>
+__device__ __forceinline__ void x16f_chunk1(const uint4 &s, uint4 *__restrict__ n1, const uint4 *R0, uint32_t A,
+                                            int yl, int zl, uint32_t chunk, uint32_t seed, uint32_t kmix,
+                                            const char *Tb, int &down, int &bond) {
+    const uint32_t sl = x16f_slot(yl, zl) * 8u;
+    const uint4 a = R0[sl + chunk];
+    const uint32_t *R0w = reinterpret_cast<const uint32_t *>(R0);
+    const uint32_t adj = P ? R0w[(sl + ((chunk + 1u) & 7u)) * 4u] : R0w[(sl + ((chunk - 1u) & 7u)) * 4u + 3u];
+    uint4 rows[4];
+    rows[0] = R0[x16f_slot(yl + 1, zl) * 8u + chunk];
+    rows[1] = R0[x16f_slot(yl - 1, zl) * 8u + chunk];
+    rows[2] = R0[x16f_slot(yl, zl + 1) * 8u + chunk];
+    rows[3] = R0[x16f_slot(yl, zl - 1) * 8u + chunk];
+    uint32_t y[16];
+    __builtin_amdgcn_sched_barrier(0);
+    x16f_draws<P, true>(seed, A, kmix, y);
+    __builtin_amdgcn_sched_barrier(0);
+    uint32_t ds[4];
+    const uint4 ov = x16f_decide<MODE, P>(a, adj, rows, s, y, Tb, ds);
+    stb(n1, A, ov);
+    if (MEAS) {
+        // as x16u_chunk's colour-1 bond sum: 16 NN - 2 sum d - 2 NN sum s + 4 sum s d with NN = 6
+        const uint32_t ovv[4] = {ov.x, ov.y, ov.z, ov.w};
+        const int dn = (int)(__popc(ov.x) + __popc(ov.y) + __popc(ov.z) + __popc(ov.w));
+        uint32_t dsum = 0, sd = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            dsum = __builtin_amdgcn_sad_u8(ds[q], 0u, dsum);
+            sd = __builtin_amdgcn_sad_u8(ds[q] & (ovv[q] * 0xffu), 0u, sd);
+        }
+        down += dn;
+        bond += 16 * 6 - 2 * (int)dsum - 2 * 6 * dn + 4 * (int)sd;
+    }
+}
+
+template <int MODE, bool MEAS>
+__global__ __launch_bounds__(1024) void ising_sweep_x16f(const uint4 *__restrict__ o0, const uint4 *__restrict__ o1,
+                                                         uint4 *__restrict__ n0, uint4 *__restrict__ n1, X16UGeom ug,
+                                                         uint32_t seed, uint32_t skey,
+                                                         const uint32_t *__restrict__ table, RecordFold fold,
+                                                         uint32_t tiles1, MeasArgs m) {
+    __shared__ uint32_t T[16];
+    __shared__ uint4 R0[X16F_S * X16F_S * 8];
+    __shared__ long long lds[16];
+    if (threadIdx.x < 14) {
+        // T[d + 8 s] = threshold * 256 - 1 for own spin s and d down neighbours (d <= 6)
+        const int s = threadIdx.x / 7, d = threadIdx.x % 7;
+        T[d + 8 * s] = (table[threadIdx.x] << 8) - 1u;
+    }
+    __syncthreads();
+    if (fold.rec) fold_record(fold, lds);
+    const uint32_t lb = (uint32_t)logical_block();
+    const uint32_t y0 = (lb % tiles1) * X16F_T, z0 = (lb / tiles1) * X16F_T;
+    const uint32_t kmix = sweep_mix(skey);
+    const char *Tb = reinterpret_cast<const char *>(T);
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t j = lane >> 3, chunk = lane & 7u;
+    int down = 0, bond = 0;  // with MEAS: the record's sums over the tile's sites
+    // colour 0: groups wave, wave + 16, wave + 32
+    for (uint32_t g = wave; g < (uint32_t)X16F_GROUPS0; g += 16u) {
+        int yl, zl;
+        bool uni = true, store = false;
+        if (g < 32u) {
+            zl = (int)(g >> 1);
+            yl = (int)(g & 1u) + 2 * (int)j;
+            store = true;
+        } else if (g < 36u) {
+            zl = ((g - 32u) >> 1) ? X16F_T : -1;
+            yl = (int)(g & 1u) + 2 * (int)j;
+        } else {
+            yl = ((g - 36u) >> 1) ? X16F_T : -1;
+            zl = (int)(g & 1u) + 2 * (int)j;
+            uni = false;
+        }
+        const uint32_t y = (y0 + (uint32_t)yl) & ug.m1, z = (z0 + (uint32_t)zl) & ug.m2;
+        const uint32_t A = ((y | (z << ug.ll1)) << ug.sh) | (chunk << 4);
+        const uint32_t p = __builtin_amdgcn_readfirstlane((y + z) & 1u);
+        const uint32_t slot = x16f_slot(yl, zl);
+        if (uni) {
+            if (p) x16f_chunk0<MODE, 1, true>(o0, o1, n0, R0, ug, A, slot, chunk, store, seed, kmix, Tb, down);
+            else x16f_chunk0<MODE, 0, true>(o0, o1, n0, R0, ug, A, slot, chunk, store, seed, kmix, Tb, down);
+        } else {
+            if (p) x16f_chunk0<MODE, 1, false>(o0, o1, n0, R0, ug, A, slot, chunk, false, seed, kmix, Tb, down);
+            else x16f_chunk0<MODE, 0, false>(o0, o1, n0, R0, ug, A, slot, chunk, false, seed, kmix, Tb, down);
+        }
+    }
+    __syncthreads();
+    // colour 1: groups wave, wave + 16
+    for (uint32_t g = wave; g < (uint32_t)X16F_GROUPS1; g += 16u) {
+        const int zl = (int)(g >> 1), yl = (int)(g & 1u) + 2 * (int)j;
+        const uint32_t y = y0 + (uint32_t)yl, z = z0 + (uint32_t)zl;
+        const uint32_t A = ((y | (z << ug.ll1)) << ug.sh) | (chunk << 4);
+        const uint32_t p = __builtin_amdgcn_readfirstlane((y + z + 1u) & 1u);
+        const uint4 s = ldb(o1, A);
+        if (p) x16f_chunk1<MODE, 1, MEAS>(s, n1, R0, A, yl, zl, chunk, seed, kmix, Tb, down, bond);
+        else x16f_chunk1<MODE, 0, MEAS>(s, n1, R0, A, yl, zl, chunk, seed, kmix, Tb, down, bond);
+    }
+    if (MEAS) {
+        // every lane updated 16 sites of each colour per tile-row group: 32 groups of 64 lanes per colour
+        const int total = 16 * 2 * (X16F_GROUPS1 / 16);
+        int counts[2] = {total - down, down};
+        emit_partials(m, (long long)bond, counts, 2);
+    }
+}
+
+bool x16f_applies(const Geom &g, int uniform) {
+    return uniform && g.D == 3 && g.L[0] == 256 && g.L[1] % X16F_T == 0 && g.L[2] % X16F_T == 0 &&
+           g.L[1] >= 2 * X16F_T && g.L[2] >= 2 * X16F_T;
+}
+
+int64_t x16f_tiles(const Geom &g) { return (int64_t)(g.L[1] / X16F_T) * (g.L[2] / X16F_T); }
+
+void launch_site_sweep_x16f(const uint8_t *o0, const uint8_t *o1, uint8_t *n0, uint8_t *n1, const Geom &g,
+                            const X16UGeom &ug, int mode, uint32_t seed, uint32_t skey, const uint32_t *table,
+                            const RecordFold &fold, const MeasArgs &m, hipStream_t s) {
+    const uint32_t tiles1 = (uint32_t)(g.L[1] / X16F_T);
+    dim3 grid((uint32_t)x16f_tiles(g)), block(1024);
+#define PLQ_X16F(MODE, MEAS)                                                                                      \
+    ising_sweep_x16f<MODE, MEAS><<<grid, block, 0, s>>>((const uint4 *)o0, (const uint4 *)o1, (uint4 *)n0,       \
+                                                        (uint4 *)n1, ug, seed, skey, table, fold, tiles1, m)
+    if (mode == 0) {
+        if (m.on) PLQ_X16F(0, true);
+        else PLQ_X16F(0, false);
+    } else {
+        if (m.on) PLQ_X16F(1, true);
+        else PLQ_X16F(1, false);
+    }
+#undef PLQ_X16F
+}
+
 // --------------------------------------------------------------- launchers ---
 
 static void grid_for_work(int64_t work, int64_t max_blocks, int *blocks, int64_t *span) {

@@ -757,6 +757,50 @@ def test_ising_at_256_cubed_is_bit_exact(gpu, kernel):
     assert np.array_equal(got, st), f"{(got != st).sum()} sites differ"
 
 
+FUSED_CASES = [
+    # shape, kernel, beta, record interval, advances
+    ([256, 32, 32], M, 0.2216546, 3, (4, 5)),     # 4 tiles of 16 x 16 rows, every halo wraps
+    ([256, 32, 32], HB, 0.2216546, 3, (4, 5)),
+    ([256, 64, 32], M, 0.3, 2, (3, 4)),
+    ([256, 32, 64], HB, 0.15, 4, (9,)),
+    ([256, 32, 32], M, 0.5, 1, (3, 2)),          # a record every sweep: folds and partials alternate halves
+    ([256, 256, 256], M, 0.2216546, 2, (4,)),     # the headline lattice: on by default (256 tiles)
+]
+
+
+@pytest.mark.parametrize("shape,kernel,beta,every,advances", FUSED_CASES,
+                         ids=["x".join(map(str, c[0])) + ("-m" if c[1] == M else "-hb") for c in FUSED_CASES])
+def test_fused_two_colour_sweep_is_bit_exact(gpu, monkeypatch, shape, kernel, beta, every, advances):
+    # One launch per 3D sweep: colour 0 on each 16 x 16-row tile and its face
+    # halos, kept in LDS, then colour 1 of the tile; a recorded sweep also
+    # writes its partial sums, folded by the next sweep's launch.  Against
+    # the per-pass launches (PLQ_FUSED_SWEEP=0) and the oracle's sequential
+    # sweeps, states and records.
+    n = int(np.prod(shape))
+    start = start_config(n, 2, True, 41)
+    sweeps = sum(advances)
+    outs = []
+    for fused in ("1", "0"):
+        monkeypatch.setenv("PLQ_FUSED_SWEEP", fused)
+        c = gpu.ising_gpu_chain(shape, 1.0, 0.0, beta, 19, start, kernel=kernel)
+        c.record(every, sweeps)
+        for k in advances:
+            c.advance(k)
+        outs.append((c.read(), [(o.sweep, o.bond_sum, o.spin_sum) for o in c.series()]))
+        c.close()
+    assert np.array_equal(outs[0][0], outs[1][0]), f"{(outs[0][0] != outs[1][0]).sum()} sites differ"
+    assert outs[0][1] == outs[1][1]
+    d = O.local_desc(shape, kernel=kernel, beta=beta, seed=19)
+    st = start
+    recs = []
+    for t in range(sweeps):
+        st = O.local_sweeps(d, st, t, 1)
+        if (t + 1) % every == 0:
+            recs.append((t + 1,) + tuple(O.ising_sums(shape, st)))
+    assert outs[0][1] == recs
+    assert np.array_equal(outs[0][0], st), f"{(outs[0][0] != st).sum()} sites differ from the oracle"
+
+
 def test_cluster_statistics_across_a_reused_crossing_tag(gpu):
     # The crossing marks carry a tag 1 + sweep mod 255, so sweeps t and
     # t + 255 share a tag and a stale mark from 255 sweeps earlier is seen
