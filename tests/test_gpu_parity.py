@@ -801,6 +801,33 @@ def test_fused_two_colour_sweep_is_bit_exact(gpu, monkeypatch, shape, kernel, be
     assert np.array_equal(outs[0][0], st), f"{(outs[0][0] != st).sum()} sites differ from the oracle"
 
 
+def test_fused_sweep_chain_state_after_writes_and_batches(gpu, monkeypatch):
+    # The fused sweep swaps the chain's two pairs of colour arrays every
+    # sweep; reads, writes, measurements and next() must see the current
+    # pair.  Heat bath on 256x32x32, against the oracle throughout.
+    shape, beta, seed = [256, 32, 32], 0.3, 23
+    n = int(np.prod(shape))
+    monkeypatch.setenv("PLQ_FUSED_SWEEP", "1")
+    start = start_config(n, 2, True, 51)
+    c = gpu.ising_gpu_chain(shape, 1.0, 0.0, beta, seed, start, kernel=HB, sweeps_between=3)
+    d = O.local_desc(shape, kernel=HB, beta=beta, seed=seed)
+    c.advance(1)  # an odd number of swaps
+    st = O.local_sweeps(d, start, 0, 1)
+    assert np.array_equal(c.read(), st)
+    obs = c.measure()
+    b, m = O.ising_sums(shape, st)
+    assert (obs.bond_sum, obs.spin_sum) == (b, m)
+    flipped = st ^ 1  # a write between sweeps replaces the state
+    c.write(flipped)
+    got = next(c)  # three more sweeps
+    st = O.local_sweeps(d, flipped, 1, 3)
+    assert np.array_equal(got, st)
+    c.advance(2)
+    st = O.local_sweeps(d, st, 4, 2)
+    assert np.array_equal(c.read(), st)
+    c.close()
+
+
 def test_cluster_statistics_across_a_reused_crossing_tag(gpu):
     # The crossing marks carry a tag 1 + sweep mod 255, so sweeps t and
     # t + 255 share a tag and a stale mark from 255 sweeps earlier is seen
