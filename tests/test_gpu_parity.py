@@ -765,6 +765,7 @@ FUSED_CASES = [
     ([256, 32, 64], HB, 0.15, 4, (9,)),
     ([256, 32, 32], M, 0.5, 1, (3, 2)),          # a record every sweep: folds and partials alternate halves
     ([256, 256, 256], M, 0.2216546, 2, (4,)),     # the headline lattice: on by default (256 tiles)
+    ([256, 256, 256], HB, 0.2216546, 3, (3,)),
 ]
 
 
