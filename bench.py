@@ -88,20 +88,24 @@ def run_kernel(P, ev, kernel, steps, warmup, kernel_sweeps):
     # events are set up before the warm-up, so the timed sweeps follow the
     # warm-up's synchronisation with no host work between (the recording
     # phase counts from sweep 0: the timed sweeps record as they would after
-    # a set-up between warm-up and timing).
-    chain.record(10, (steps + warmup) // 10 + 1)
+    # a set-up between warm-up and timing).  The wall-clock region holds the
+    # chain's own calls only; the GPU clock of the job comes from K more
+    # sweeps between two events (an event record costs the host about 3 us).
+    chain.record(10, (2 * steps + warmup) // 10 + 1)
     stream = chain.stream()
     e0, e1 = ev.event(), ev.event()
     chain.advance(warmup)
     chain.sync()
     t0 = time.perf_counter()
+    chain.advance(steps)
+    chain.sync()
+    wall = time.perf_counter() - t0
     ev.record(e0, stream)
     chain.advance(steps)
     ev.record(e1, stream)
     chain.sync()
-    wall = time.perf_counter() - t0
     gpu_ms = ev.elapsed_ms(e0, e1)
-    series = [o for o in chain.series() if o.sweep > warmup]  # the timed sweeps' records
+    series = [o for o in chain.series() if warmup < o.sweep <= warmup + steps]  # the timed sweeps' records
     # kernel-only region: the sweep launches alone (no records)
     chain.record(0, 0)
     chain.sync()
@@ -335,8 +339,8 @@ def profile_passes(sweeps: int, copy_mib: int) -> None:
 
 def timed(P, ev, stream_of, run):
     """Wall and event time of run() bracketed by device synchronisation."""
-    t0 = time.perf_counter()
     e0, e1 = ev.event(), ev.event()
+    t0 = time.perf_counter()
     ev.record(e0, stream_of())
     run()
     ev.record(e1, stream_of())
