@@ -143,6 +143,7 @@ int upload_table(plq_chain *c) {
     PLQ_TRY(alloc((void **)&c->d_table, table.size() * sizeof(uint32_t)));
     PLQ_HIP(hipMemcpy(c->d_table, table.data(), table.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     c->table_len = table.size();
+    c->h_table = table;
     return PLQ_OK;
 }
 
@@ -335,7 +336,7 @@ int enqueue_local_sweep(plq_chain *c) {
         if (record) enqueue_site_measure(c, rec_slot(c));
     } else if (c->kind == KIND_SITE) {
         c->sweeps_done += 1;
-        if (c->fused && c->x16u && !c->site_graph &&
+        if (c->fused && c->x16u && !c->site_graph && c->h_table.size() == (size_t)plq::SWEEP_TH_LEN &&
             (!record || plq::x16f_tiles(c->g) <= plq::MAX_PASS_BLOCKS)) {
             // both colour passes in one launch; a pending record fold rides on
             // it, and its own record (one partial row per tile) is folded by
@@ -354,8 +355,10 @@ int enqueue_local_sweep(plq_chain *c) {
                 }
             }
             long long *rec = record ? rec_slot(c) : nullptr;
+            plq::SweepThresholds th;
+            for (int i = 0; i < plq::SWEEP_TH_LEN; ++i) th.t[i] = c->h_table[(size_t)i];
             plq::launch_site_sweep_x16f(c->col[0], c->col[1], c->col_alt[0], c->col_alt[1], c->g, c->ug, mode,
-                                        c->seed32, sweep * plq::GOLDEN, c->d_table, c->fold, m, c->stream);
+                                        c->seed32, sweep * plq::GOLDEN, th, c->fold, m, c->stream);
             c->fold = plq::RecordFold{};
             std::swap(c->col[0], c->col_alt[0]);
             std::swap(c->col[1], c->col_alt[1]);

@@ -47,6 +47,7 @@ struct plq_chain {
     uint8_t **d_links = nullptr;           // device copy of the link array pointers
     uint32_t *d_table = nullptr;
     size_t table_len = 0;
+    std::vector<uint32_t> h_table;  // host copy of d_table
     // cluster
     plq::CCGeom cg{};
     uint8_t *labels = nullptr;

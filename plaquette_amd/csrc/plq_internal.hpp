@@ -94,6 +94,14 @@ struct X16UGeom {
     uint32_t m1, m2, m3; // L1 - 1, L2 - 1, L3 - 1
 };
 
+// The 3D Ising threshold table (own spin s, d of 6 neighbours down: entry
+// d + 7 s), passed by value to the fused sweep so that a launch fills its LDS
+// copy from kernel arguments instead of a dependent global load.
+constexpr int SWEEP_TH_LEN = 14;
+struct SweepThresholds {
+    uint32_t t[SWEEP_TH_LEN];
+};
+
 struct SitePassArgs {
     const uint8_t *oth;  // the other colour's array (read)
     uint8_t *own;        // this colour's array (updated)
@@ -131,7 +139,7 @@ bool x16u_applies(const Geom &g, const uint32_t *host_table, size_t table_len, i
 bool x16f_applies(const Geom &g, int uniform);
 int64_t x16f_tiles(const Geom &g);
 void launch_site_sweep_x16f(const uint8_t *o0, const uint8_t *o1, uint8_t *n0, uint8_t *n1, const Geom &g,
-                            const X16UGeom &ug, int mode, uint32_t seed, uint32_t skey, const uint32_t *table,
+                            const X16UGeom &ug, int mode, uint32_t seed, uint32_t skey, const SweepThresholds &th,
                             const RecordFold &fold, const MeasArgs &m, hipStream_t s);
 X16UGeom x16u_geom(const Geom &g);
 int site_pass_blocks(const SitePassArgs &a);

@@ -983,16 +983,15 @@ __device__ __forceinline__ void x16f_chunk1(const uint4 &s, uint4 *__restrict__ 
 template <int MODE, bool MEAS>
 __global__ __launch_bounds__(1024) void ising_sweep_x16f(const uint4 *__restrict__ o0, const uint4 *__restrict__ o1,
                                                          uint4 *__restrict__ n0, uint4 *__restrict__ n1, X16UGeom ug,
-                                                         uint32_t seed, uint32_t skey,
-                                                         const uint32_t *__restrict__ table, RecordFold fold,
-                                                         uint32_t tiles1, MeasArgs m) {
+                                                         uint32_t seed, uint32_t skey, SweepThresholds th,
+                                                         RecordFold fold, uint32_t tiles1, MeasArgs m) {
     __shared__ uint32_t T[16];
     __shared__ uint4 R0[X16F_S * X16F_S * 8];
     __shared__ long long lds[16];
-    if (threadIdx.x < 14) {
+    if (threadIdx.x == 0) {
         // T[d + 8 s] = threshold * 256 - 1 for own spin s and d down neighbours (d <= 6)
-        const int s = threadIdx.x / 7, d = threadIdx.x % 7;
-        T[d + 8 * s] = (table[threadIdx.x] << 8) - 1u;
+#pragma unroll
+        for (int i = 0; i < SWEEP_TH_LEN; ++i) T[i % 7 + 8 * (i / 7)] = (th.t[i] << 8) - 1u;
     }
     __syncthreads();
     if (fold.rec) fold_record(fold, lds);
@@ -1059,13 +1058,13 @@ bool x16f_applies(const Geom &g, int uniform) {
 int64_t x16f_tiles(const Geom &g) { return (int64_t)(g.L[1] / X16F_T) * (g.L[2] / X16F_T); }
 
 void launch_site_sweep_x16f(const uint8_t *o0, const uint8_t *o1, uint8_t *n0, uint8_t *n1, const Geom &g,
-                            const X16UGeom &ug, int mode, uint32_t seed, uint32_t skey, const uint32_t *table,
+                            const X16UGeom &ug, int mode, uint32_t seed, uint32_t skey, const SweepThresholds &th,
                             const RecordFold &fold, const MeasArgs &m, hipStream_t s) {
     const uint32_t tiles1 = (uint32_t)(g.L[1] / X16F_T);
     dim3 grid((uint32_t)x16f_tiles(g)), block(1024);
 #define PLQ_X16F(MODE, MEAS)                                                                                      \
     ising_sweep_x16f<MODE, MEAS><<<grid, block, 0, s>>>((const uint4 *)o0, (const uint4 *)o1, (uint4 *)n0,       \
-                                                        (uint4 *)n1, ug, seed, skey, table, fold, tiles1, m)
+                                                        (uint4 *)n1, ug, seed, skey, th, fold, tiles1, m)
     if (mode == 0) {
         if (m.on) PLQ_X16F(0, true);
         else PLQ_X16F(0, false);
