@@ -218,19 +218,14 @@ __device__ __forceinline__ EnsDeltas ens_deltas(const EnsQuad &Q, const LinkSite
 template <int D, int MODE, int MU>
 __device__ __forceinline__ void ens_pass2_block(uint32_t *__restrict__ words, const EnsQuad *__restrict__ quads,
                                                 const uint16_t *__restrict__ block_quad,
-                                                const uint32_t *__restrict__ tables,
-                                                const uint32_t *__restrict__ beta_slot, uint32_t kmix, int c,
+                                                const uint32_t *__restrict__ qtab, uint32_t kmix, int c,
                                                 int per_lane, uint32_t lb, uint32_t *T) {
     constexpr int K = 2 * (D - 1);
     constexpr int TL = 2 * (K + 1);
     const int qi = block_quad[lb];
     if (qi == NO_QUAD) return;  // the same for the whole block
     const EnsQuad Q = quads[qi];
-    if (threadIdx.x < 4 * TL) {
-        const int j = threadIdx.x / TL, e = threadIdx.x - j * TL;
-        const int rj = quads[qi].rep[j];  // runtime index: read from memory, not the register copy
-        T[threadIdx.x] = rj >= 0 ? (tables[beta_slot[rj] * TL + e] << 8) - 1u : 0u;
-    }
+    if (threadIdx.x < 4 * TL) T[threadIdx.x] = qtab[(uint32_t)qi * (4 * TL) + threadIdx.x];
     __syncthreads();
     const uint32_t uc = (uint32_t)c, uo = 1u - uc;
     const uint32_t half = Q.nsites >> 1;
@@ -298,15 +293,13 @@ __device__ __forceinline__ void ens_pass2_block(uint32_t *__restrict__ words, co
 template <int D, int MODE, int MU>
 __global__ __launch_bounds__(256) void ens_link_pass2(uint32_t *__restrict__ words, const EnsQuad *__restrict__ quads,
                                                       const uint16_t *__restrict__ block_quad,
-                                                      const uint32_t *__restrict__ tables,
-                                                      const uint32_t *__restrict__ beta_slot,
+                                                      const uint32_t *__restrict__ qtab,
                                                       const uint32_t *__restrict__ sweep_base, uint32_t sweep_off,
                                                       int c, int per_lane) {
     constexpr int TL = 2 * (2 * (D - 1) + 1);
     __shared__ uint32_t T[4 * TL];  // T[j * TL + cur * (K + 1) + n] = threshold * 256 - 1
     const uint32_t kmix = sweep_mix((*sweep_base + sweep_off) * GOLDEN);
-    ens_pass2_block<D, MODE, MU>(words, quads, block_quad, tables, beta_slot, kmix, c, per_lane, ens_logical_block(),
-                                 T);
+    ens_pass2_block<D, MODE, MU>(words, quads, block_quad, qtab, kmix, c, per_lane, ens_logical_block(), T);
 }
 
 // Every colour pass of nsweeps sweeps of every replica in one cooperative
@@ -317,8 +310,7 @@ __global__ __launch_bounds__(256) void ens_link_pass2(uint32_t *__restrict__ wor
 template <int D, int MODE>
 __global__ __launch_bounds__(256) void ens_sweeps_coop(uint32_t *__restrict__ words, const EnsQuad *__restrict__ quads,
                                                        const uint16_t *__restrict__ block_quad,
-                                                       const uint32_t *__restrict__ tables,
-                                                       const uint32_t *__restrict__ beta_slot,
+                                                       const uint32_t *__restrict__ qtab,
                                                        const uint32_t *__restrict__ sweep_base, uint32_t nsweeps,
                                                        int per_lane, uint32_t nblocks) {
     constexpr int TL = 2 * (2 * (D - 1) + 1);
@@ -333,10 +325,10 @@ __global__ __launch_bounds__(256) void ens_sweeps_coop(uint32_t *__restrict__ wo
                 for (uint32_t lb = blockIdx.x; lb < nblocks; lb += gridDim.x) {
                     __syncthreads();  // the previous logical block is done with T
                     switch (mu) {
-                        case 0: ens_pass2_block<D, MODE, 0>(words, quads, block_quad, tables, beta_slot, kmix, c, per_lane, lb, T); break;
-                        case 1: ens_pass2_block<D, MODE, 1>(words, quads, block_quad, tables, beta_slot, kmix, c, per_lane, lb, T); break;
-                        case 2: ens_pass2_block<D, MODE, (2 < D ? 2 : 0)>(words, quads, block_quad, tables, beta_slot, kmix, c, per_lane, lb, T); break;
-                        default: ens_pass2_block<D, MODE, (3 < D ? 3 : 0)>(words, quads, block_quad, tables, beta_slot, kmix, c, per_lane, lb, T); break;
+                        case 0: ens_pass2_block<D, MODE, 0>(words, quads, block_quad, qtab, kmix, c, per_lane, lb, T); break;
+                        case 1: ens_pass2_block<D, MODE, 1>(words, quads, block_quad, qtab, kmix, c, per_lane, lb, T); break;
+                        case 2: ens_pass2_block<D, MODE, (2 < D ? 2 : 0)>(words, quads, block_quad, qtab, kmix, c, per_lane, lb, T); break;
+                        default: ens_pass2_block<D, MODE, (3 < D ? 3 : 0)>(words, quads, block_quad, qtab, kmix, c, per_lane, lb, T); break;
                     }
                 }
                 grid.sync();
@@ -445,6 +437,20 @@ __device__ __forceinline__ void ens_exchange_pair(int p, uint32_t round, const E
 }
 
 
+// EnsQtab's entries from the beta slots, by the threads of one workgroup.
+__device__ __forceinline__ void ens_qtab_fill(const EnsQtab &qt, const uint32_t *__restrict__ slot_of) {
+    const int per_quad = 4 * qt.TL, n = qt.nquads * per_quad;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int q = i / per_quad, r = i - q * per_quad, j = r / qt.TL, e = r - j * qt.TL;
+        const int rep = qt.quads[q].rep[j];
+        qt.qtab[i] = rep >= 0 ? (qt.tables[slot_of[rep] * (uint32_t)qt.TL + (uint32_t)e] << 8) - 1u : 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void ens_qtab(EnsQtab qt, const uint32_t *__restrict__ slot_of) {
+    ens_qtab_fill(qt, slot_of);
+}
+
 // counters[0] = sweep base, counters[1] = exchange round, counters[2] = records
 __global__ void ens_tick(uint32_t *__restrict__ counters, uint32_t sweeps, uint32_t rounds) {
     counters[0] += sweeps;
@@ -455,17 +461,19 @@ __global__ void ens_tick(uint32_t *__restrict__ counters, uint32_t sweeps, uint3
 // The measurement tail of a block of sweeps in one workgroup (one launch
 // where four small ones ran): the sums and beta slots into the ring, then
 // (npairs > 0) one exchange round, then the sums cleared for the next
-// block's ens_plaquette, and the counters advanced as ens_tick does.
-// Barriers order the phases: the record reads the slots before the exchange
-// swaps them, and every thread reads the counters before thread 0 moves
-// them.
+// block's ens_plaquette, the fast pass's thresholds rewritten for the new
+// slots, and the counters advanced as ens_tick does.  Barriers order the
+// phases: the record reads the slots before the exchange swaps them, the
+// thresholds read them after, and every thread reads the counters before
+// thread 0 moves them.
 __global__ __launch_bounds__(256) void ens_post(unsigned long long *__restrict__ sums, uint32_t *__restrict__ slot_of,
                                                 int nrep, long long *__restrict__ ring_sums,
                                                 uint32_t *__restrict__ ring_slots, uint32_t cap,
                                                 uint32_t *__restrict__ counters, const EnsGroupPair *__restrict__ pairs,
                                                 int npairs, uint32_t *__restrict__ rep_at,
                                                 const uint32_t *__restrict__ ex_tables, uint32_t ex_seed,
-                                                unsigned long long *__restrict__ accepted, uint32_t sweeps) {
+                                                unsigned long long *__restrict__ accepted, uint32_t sweeps,
+                                                EnsQtab qt) {
     const uint32_t slot = counters[2] % cap;
     const uint32_t round = counters[1];
     for (int i = threadIdx.x; i < nrep; i += blockDim.x) {
@@ -476,6 +484,7 @@ __global__ __launch_bounds__(256) void ens_post(unsigned long long *__restrict__
     for (int p = threadIdx.x; p < npairs; p += blockDim.x)
         ens_exchange_pair(p, round, pairs, slot_of, rep_at, sums, ex_tables, ex_seed, accepted);
     __syncthreads();
+    if (npairs > 0) ens_qtab_fill(qt, slot_of);
     for (int i = threadIdx.x; i < nrep; i += blockDim.x) sums[i] = 0ull;
     if (threadIdx.x == 0) {
         counters[0] += sweeps;
@@ -530,9 +539,8 @@ static void ens_pass_mu(const EnsPassArgs &a) {
     dim3 grid(a.blocks), block(a.block_size);
 #define PLQ_ENS_MU(M)                                                                                             \
     if (a.fast)                                                                                                   \
-        ens_link_pass2<D, MODE, M><<<grid, block, 0, a.stream>>>(a.words, a.quads, a.block_quad, a.tables,       \
-                                                                 a.beta_slot, a.sweep_base, a.sweep_off, a.c,     \
-                                                                 a.per_lane);                                     \
+        ens_link_pass2<D, MODE, M><<<grid, block, 0, a.stream>>>(a.words, a.quads, a.block_quad, a.qtab,         \
+                                                                 a.sweep_base, a.sweep_off, a.c, a.per_lane);     \
     else                                                                                                          \
         ens_link_pass<D, MODE, M><<<grid, block, 0, a.stream>>>(a.words, a.quads, a.block_quad, a.tables,        \
                                                                 a.beta_slot, a.sweep_base, a.sweep_off, a.c,      \
@@ -562,10 +570,10 @@ static hipError_t ens_coop(const EnsPassArgs &a, uint32_t nsweeps) {
     uint32_t *words = a.words;
     const EnsQuad *quads = a.quads;
     const uint16_t *block_quad = a.block_quad;
-    const uint32_t *tables = a.tables, *beta_slot = a.beta_slot, *sweep_base = a.sweep_base;
+    const uint32_t *qtab = a.qtab, *sweep_base = a.sweep_base;
     int per_lane = a.per_lane;
     uint32_t nblocks = a.blocks;
-    void *args[] = {&words, &quads, &block_quad, &tables, &beta_slot, &sweep_base, &nsweeps, &per_lane, &nblocks};
+    void *args[] = {&words, &quads, &block_quad, &qtab, &sweep_base, &nsweeps, &per_lane, &nblocks};
     return hipLaunchCooperativeKernel((const void *)ens_sweeps_coop<D, MODE>, dim3(grid), dim3(a.block_size), args, 0,
                                       a.stream);
 }
@@ -599,9 +607,13 @@ void launch_ens_zero(unsigned long long *sums, int nrep, hipStream_t s) {
 void launch_ens_post(unsigned long long *sums, uint32_t *slot_of, int nrep, long long *ring_sums, uint32_t *ring_slots,
                      uint32_t cap, uint32_t *counters, const EnsGroupPair *pairs, int npairs, uint32_t *rep_at,
                      const uint32_t *ex_tables, uint32_t ex_seed, unsigned long long *accepted, uint32_t sweeps,
-                     hipStream_t s) {
+                     const EnsQtab &qt, hipStream_t s) {
     ens_post<<<dim3(1), dim3(256), 0, s>>>(sums, slot_of, nrep, ring_sums, ring_slots, cap, counters, pairs, npairs,
-                                           rep_at, ex_tables, ex_seed, accepted, sweeps);
+                                           rep_at, ex_tables, ex_seed, accepted, sweeps, qt);
+}
+
+void launch_ens_qtab(const EnsQtab &qt, const uint32_t *slot_of, hipStream_t s) {
+    ens_qtab<<<dim3(1), dim3(256), 0, s>>>(qt, slot_of);
 }
 
 void launch_ens_plaquette(const uint32_t *words, const EnsQuad *quads, const uint16_t *block_quad, int nrep, int D,

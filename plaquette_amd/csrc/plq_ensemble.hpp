@@ -43,6 +43,19 @@ struct EnsGroupPair {
     uint32_t table_len;
 };
 
+// The fast pass's thresholds per quad, kept in step with the replicas' beta
+// slots: qtab[(q * 4 + j) * TL + e] = tables[slot_of[r] * TL + e] * 256 - 1
+// for the replica r in byte j of quad q (0 for an empty byte), TL = 2 (2D - 1).
+// A pass's workgroup then fills its LDS table with one load per entry
+// instead of the chain quads -> slot_of -> tables.
+struct EnsQtab {
+    const EnsQuad *quads;
+    int nquads;
+    const uint32_t *tables;
+    uint32_t *qtab;
+    int TL;
+};
+
 struct EnsPassArgs {
     uint32_t *words;
     const EnsQuad *quads;
@@ -51,6 +64,7 @@ struct EnsPassArgs {
     int D;
     const uint32_t *tables;      // per beta slot: 2 * (2D - 1) thresholds
     const uint32_t *beta_slot;   // per replica
+    const uint32_t *qtab;        // per quad: the fast pass's 4 x TL thresholds * 256 - 1 (EnsQtab)
     const uint32_t *sweep_base;  // device sweep counter
     uint32_t sweep_off;
     int mu, c;
@@ -77,10 +91,13 @@ void launch_ens_plaquette(const uint32_t *words, const EnsQuad *quads, const uin
 void launch_ens_zero(unsigned long long *sums, int nrep, hipStream_t s);
 // A measured block's tail in one launch (ens_post): record, exchange round
 // (npairs > 0), sums cleared, counters advanced by sweeps and the round.
+// The exchange round rewrites the fast pass's thresholds of every quad (qt).
 void launch_ens_post(unsigned long long *sums, uint32_t *slot_of, int nrep, long long *ring_sums, uint32_t *ring_slots,
                      uint32_t cap, uint32_t *counters, const EnsGroupPair *pairs, int npairs, uint32_t *rep_at,
                      const uint32_t *ex_tables, uint32_t ex_seed, unsigned long long *accepted, uint32_t sweeps,
-                     hipStream_t s);
+                     const EnsQtab &qt, hipStream_t s);
+// EnsQtab's thresholds from the current beta slots (after any host write of them).
+void launch_ens_qtab(const EnsQtab &qt, const uint32_t *slot_of, hipStream_t s);
 void launch_ens_tick(uint32_t *counters, uint32_t sweeps, uint32_t rounds, hipStream_t s);
 
 }  // namespace plq

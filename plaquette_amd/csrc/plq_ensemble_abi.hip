@@ -71,6 +71,7 @@ struct plq_ensemble {
     uint8_t *d_lat = nullptr;
     uint32_t *d_tables = nullptr;
     uint32_t *d_slot_of = nullptr;
+    uint32_t *d_qtab = nullptr;  // the fast pass's thresholds per quad (plq::EnsQtab)
     uint32_t *d_rep_at = nullptr;
     std::vector<plq::EnsGroupPair> pairs;
     plq::EnsGroupPair *d_pairs = nullptr;
@@ -98,7 +99,7 @@ struct plq_ensemble {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         for (void *p : {(void *)d_quads, (void *)d_block_rep, (void *)d_words, (void *)d_lat, (void *)d_tables, (void *)d_slot_of,
                         (void *)d_rep_at, (void *)d_pairs, (void *)d_ex_tables, (void *)d_sums, (void *)d_accepted,
-                        (void *)d_counters, (void *)d_ring_sums, (void *)d_ring_slots})
+                        (void *)d_counters, (void *)d_ring_sums, (void *)d_ring_slots, (void *)d_qtab})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -107,6 +108,16 @@ struct plq_ensemble {
 namespace {
 
 // Blocks of each replica for the current launch shape; uploads the replica table.
+plq::EnsQtab qtab_of(const plq_ensemble *e) {
+    plq::EnsQtab qt{};
+    qt.quads = e->d_quads;
+    qt.nquads = (int)e->quads.size();
+    qt.tables = e->d_tables;
+    qt.qtab = e->d_qtab;
+    qt.TL = 2 * (2 * e->D - 1);
+    return qt;
+}
+
 int layout_blocks(plq_ensemble *e) {
     uint32_t b = 0;
     uint32_t per_block = (uint32_t)e->block_size * (uint32_t)e->per_lane;
@@ -135,6 +146,8 @@ int layout_blocks(plq_ensemble *e) {
     }
     ENS_HIP(hipMemcpy(e->d_block_rep, block_rep.data(), sizeof(uint16_t) * b, hipMemcpyHostToDevice));
     ENS_HIP(hipMemcpy(e->d_quads, e->quads.data(), sizeof(plq::EnsQuad) * e->quads.size(), hipMemcpyHostToDevice));
+    plq::launch_ens_qtab(qtab_of(e), e->d_slot_of, e->stream);
+    ENS_HIP(hipGetLastError());
     if (e->graph_exec) {
         (void)hipGraphExecDestroy(e->graph_exec);
         e->graph_exec = nullptr;
@@ -151,6 +164,7 @@ void enqueue_block(plq_ensemble *e, uint32_t nsweeps, bool measure) {
     a.D = e->D;
     a.tables = e->d_tables;
     a.beta_slot = e->d_slot_of;
+    a.qtab = e->d_qtab;
     a.sweep_base = e->d_counters;
     a.mode = e->kernel == PLQ_KERNEL_METROPOLIS ? 0 : 1;
     a.fast = e->fast;
@@ -184,7 +198,7 @@ void enqueue_block(plq_ensemble *e, uint32_t nsweeps, bool measure) {
                                   e->per_lane, e->d_sums, e->stream, false);
         plq::launch_ens_post(e->d_sums, e->d_slot_of, e->nrep, e->d_ring_sums, e->d_ring_slots, (uint32_t)e->rec_cap,
                              e->d_counters, e->d_pairs, e->exchange ? (int)e->pairs.size() : 0, e->d_rep_at,
-                             e->d_ex_tables, e->ex_seed32, e->d_accepted, nsweeps, e->stream);
+                             e->d_ex_tables, e->ex_seed32, e->d_accepted, nsweeps, qtab_of(e), e->stream);
         return;
     }
     plq::launch_ens_tick(e->d_counters, nsweeps, 0, e->stream);
@@ -434,6 +448,7 @@ int plq_ensemble_create(const plq_ensemble_desc *d, const uint8_t *start, size_t
         (rc = dalloc((void **)&e->d_lat, e->lat_bytes)) ||
         (rc = dalloc((void **)&e->d_tables, sizeof(uint32_t) * tables.size())) ||
         (rc = dalloc((void **)&e->d_slot_of, sizeof(uint32_t) * e->nrep)) ||
+        (rc = dalloc((void **)&e->d_qtab, sizeof(uint32_t) * e->quads.size() * 4 * (2 * (2 * e->D - 1)))) ||
         (rc = dalloc((void **)&e->d_rep_at, sizeof(uint32_t) * rep_at.size())) ||
         (rc = dalloc((void **)&e->d_pairs, sizeof(plq::EnsGroupPair) * std::max<size_t>(1, e->pairs.size()))) ||
         (rc = dalloc((void **)&e->d_ex_tables, sizeof(uint32_t) * std::max<size_t>(1, ex_tables.size()))) ||
@@ -643,6 +658,8 @@ int plq_ensemble_tune(plq_ensemble *e, int32_t *block_size, int32_t *per_lane, i
     // restore
     ENS_HIP(hipMemcpy(e->d_words, words.data(), sizeof(uint32_t) * e->nwords, hipMemcpyHostToDevice));
     ENS_HIP(hipMemcpy(e->d_slot_of, slot.data(), sizeof(uint32_t) * e->nrep, hipMemcpyHostToDevice));
+    plq::launch_ens_qtab(qtab_of(e), e->d_slot_of, e->stream);
+    ENS_HIP(hipGetLastError());
     ENS_HIP(hipMemcpy(e->d_rep_at, rep_at.data(), sizeof(uint32_t) * e->nrep, hipMemcpyHostToDevice));
     ENS_HIP(hipMemcpy(e->d_counters, counters.data(), sizeof(uint32_t) * 4, hipMemcpyHostToDevice));
     ENS_HIP(hipMemcpy(e->d_accepted, acc.data(), sizeof(unsigned long long) * acc.size(), hipMemcpyHostToDevice));
